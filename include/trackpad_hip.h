@@ -1,0 +1,138 @@
+/*
+ * trackpad_hip.h -- C ABI of libtrackpad_hip, the MI355X (gfx950) backend for the
+ * multi-particle tracking path of TrackPad.jl.
+ *
+ * The reference path this library replaces is
+ *   linepass!(coords::Matrix{T}, lat::Lattice, beam::Beam{T}, lost_flags::Vector{Int};
+ *             time, turn)                                       src/lattice.jl:260-297
+ *   ringpass!(coords, lat, beam, lost_flags, nturns; time, dt_turn, turn)
+ *                                                               src/lattice.jl:305-317
+ * with the per-element maps pass!(elem, r, beti) of src/tracking.jl:115-973.
+ *
+ * Conventions (all entry points):
+ *   - plain C types only; no exceptions cross the boundary;
+ *   - return 0 (TP_OK) on success or a negative TP_E* code; tp_last_error() then holds a
+ *     message (thread-local), including the hipError_t name for TP_EHIP;
+ *   - coords are N x 6 column-major doubles, i.e. six planes x|px|y|py|z|delta of N
+ *     values each -- the memory layout of a Julia Matrix{Float64}(N, 6)
+ *     (src/lattice.jl:274-275, 287-292);
+ *   - loss: lost_flags are the reference's Int64 0/1 (src/lattice.jl:269-284);
+ *     lost_turn / lost_elem are int32, 0-based turn and lattice position at whose exit
+ *     the particle was lost, -1 while alive, -2 for a particle that arrived already
+ *     flagged (lost_flags == 1);
+ *   - handles are owned by the library and freed by *_destroy; host buffers are owned by
+ *     the caller; a context is not re-entrant (serialise calls per context), distinct
+ *     contexts may be used from different threads.
+ */
+#ifndef TRACKPAD_HIP_H
+#define TRACKPAD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "trackpad_table.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TP_ABI_VERSION 1
+
+enum tp_status {
+  TP_OK = 0,
+  TP_EINVAL = -1,        /* bad argument or handle */
+  TP_EUNSUPPORTED = -2,  /* element or option the device path does not implement */
+  TP_EHIP = -3,          /* HIP runtime error */
+  TP_ENOMEM = -4,        /* device or host allocation failed */
+  TP_ETABLE = -5         /* malformed element table */
+};
+
+/* tp_track flags */
+#define TP_TRACK_FAST 0x1u  /* FMA-contracted kernel with rsqrt drifts: tolerance, not bits */
+#define TP_TRACK_ASYNC 0x2u /* return after enqueueing; tp_sync() waits */
+#define TP_TRACK_TIMED 0x4u /* record HIP events around every launch (tp_timing) */
+
+typedef struct tp_ctx tp_ctx;
+typedef struct tp_table tp_table;
+typedef struct tp_particles tp_particles;
+
+typedef struct tp_timing_t {
+  double kernel_ms;    /* summed duration of the timed launches of the last timed call */
+  int32_t launches;    /* kernel launches in that call */
+  int32_t block;       /* work-group size used */
+  int32_t ppl;         /* particles per lane used */
+  int32_t turns_per_launch;
+} tp_timing_t;
+
+/* Library version (TP_ABI_VERSION). No reference counterpart. */
+int tp_version(void);
+
+/* Message of the last failed call on this thread. No reference counterpart (the
+ * reference raises Julia exceptions, e.g. src/time_dependence.jl:138). */
+const char* tp_last_error(void);
+
+/* Number of visible HIP devices. */
+int tp_device_count(int* n);
+
+/* One device and one HIP stream per context. Replaces the implicit single-threaded CPU
+ * "device" of src/lattice.jl:266-294. */
+int tp_ctx_create(int device, tp_ctx** out);
+int tp_ctx_destroy(tp_ctx* ctx);
+
+/* Launch shape for later tp_track calls: work-group size (64..1024, multiple of 64, 0 =
+ * default 256), particles per lane (1 or 2, 0 = default 1), turns per kernel launch
+ * (0 = all turns in one launch). */
+int tp_set_launch(tp_ctx* ctx, int block, int ppl, int turns_per_launch);
+
+/* Kernel timing of the last call made with TP_TRACK_TIMED. */
+int tp_timing(tp_ctx* ctx, tp_timing_t* out);
+
+/* Upload a validated element table (trackpad_table.h). Replaces the reference's
+ * Lattice{E} element vector (src/lattice.jl:87-105) and the materialised-lattice
+ * precondition (src/lattice.jl:128-132, src/time_dependence.jl:137-139): time-varying
+ * elements cannot be expressed in a table, the writer refuses them. */
+int tp_table_create(tp_ctx* ctx, const void* bytes, size_t nbytes, tp_table** out);
+int tp_table_destroy(tp_table* t);
+/* n_seq, n_rec and the table's beti. */
+int tp_table_info(const tp_table* t, int32_t* n_seq, int32_t* n_rec, double* beti);
+
+/* Device-resident particle set (6 planes of n doubles + lost_turn + lost_elem). */
+int tp_particles_create(tp_ctx* ctx, int64_t n, tp_particles** out);
+int tp_particles_destroy(tp_particles* p);
+/* coords: N x 6 column-major (required). lost_flags01 (reference Int64 flags, may be
+ * NULL): 1 marks a particle lost before the call. lost_turn / lost_elem (may be NULL)
+ * restore the library's own loss record and take precedence over lost_flags01. */
+int tp_particles_upload(tp_particles* p, const double* coords, const int64_t* lost_flags01,
+                        const int32_t* lost_turn, const int32_t* lost_elem);
+/* Any output pointer may be NULL. lost_flags01[i] is written 1 for lost particles and
+ * 0 for live ones. */
+int tp_particles_download(tp_particles* p, double* coords, int64_t* lost_flags01,
+                          int32_t* lost_turn, int32_t* lost_elem);
+/* Device pointers of the particle planes (for zero-copy interop); n is returned too. */
+int tp_particles_device_ptrs(tp_particles* p, double** coords, int32_t** lost_turn,
+                             int32_t** lost_elem, int64_t* n);
+
+/* Track device-resident particles for nturns turns starting at turn number turn0.
+ * nturns == 1 is linepass! (src/lattice.jl:260-297); nturns > 1 is ringpass!
+ * (src/lattice.jl:305-317). Loss is checked at every element exit with check_lost
+ * (src/tracking.jl:49-52) plus the table's aperture; a lost particle keeps its
+ * element-entry coordinates and is skipped afterwards (src/lattice.jl:269-284). */
+int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns,
+             uint32_t flags);
+
+/* Wait for all work enqueued on the context's stream. */
+int tp_sync(tp_ctx* ctx);
+
+/* One-shot mirror of ringpass!/linepass! on host arrays (src/lattice.jl:260-317):
+ * uploads coords and lost_flags, tracks, downloads. lost_flags follows the reference:
+ * entries == 1 are skipped, newly lost particles are set to 1, other entries are left
+ * as they were. lost_turn / lost_elem (may be NULL) receive the loss record. */
+int tp_track_host(tp_ctx* ctx, const tp_table* t, double* coords, int64_t n, int64_t* lost_flags,
+                  int32_t* lost_turn, int32_t* lost_elem, int32_t turn0, int32_t nturns,
+                  uint32_t flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TRACKPAD_HIP_H */

@@ -1,0 +1,235 @@
+"""Device parity: libtrackpad_hip (through its C ABI) against the CPU oracle.
+
+Tolerances (written where they are used):
+  * parity build, maps with only + - * / sqrt on coordinates: bit-exact (array_equal);
+  * parity build with per-particle transcendentals (exact bends, RF, tan edges): within
+    the reference tests' own atol = 1e-15 for single passes of the reference's 10 x 6 input
+    (test/verify_elements.jl:6), and, for rings, |gpu - oracle| <= rtol * max(|oracle|,
+    rms of that coordinate plane) with rtol = 1e-12 after 1 turn and 1e-9 after 100 turns;
+  * fast build: the same ring tolerances;
+  * loss flags, lost turn and lost element: identical (the test inputs stay away from the
+    aperture boundary; DA-grid particles within a boundary band are excluded).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import ENERGY_VAL, PARITY_ATOL, PARTICLES_INITIAL
+import lattices as LT
+from oracle import pyoracle as O
+from trackpad_hip import (Beam, Drift, HIPParticles, Lattice, Quadrupole, compile_table, track_)
+from trackpad_hip import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_track(ctx, lat, beam, coords, nturns, *, fast=False, aperture=None, lost_flags=None, turn=0,
+              block=0, ppl=0, tpl=0):
+    ctx.set_launch(block, ppl, tpl)
+    try:
+        tab = ctx.table(compile_table(lat, beam, aperture=aperture))
+        p = HIPParticles.from_coords(coords, lost_flags, ctx)
+        track_(p, tab, nturns=nturns, turn=turn, fast=fast)
+        c, lf, lt, le = p.download()
+        return np.ascontiguousarray(c), lf, lt, le
+    finally:
+        ctx.set_launch(0, 0, 0)
+
+
+def ring_close(a, b, rtol):
+    """max over entries of |a - b| / max(|b|, rms of b's coordinate plane)."""
+    rms = np.sqrt(np.mean(b * b, axis=0, keepdims=True))
+    scale = np.maximum(np.abs(b), rms)
+    scale = np.where(scale > 0, scale, 1.0)
+    err = np.max(np.abs(a - b) / scale)
+    return err <= rtol, err
+
+
+def p10():
+    return np.array(PARTICLES_INITIAL, dtype=np.float64)
+
+
+# --------------------------------------------------------------------------------------
+# Single passes of the reference's own test inputs
+# --------------------------------------------------------------------------------------
+@pytest.mark.parametrize("case", [c[0] for c in LT.element_cases()])
+def test_element_single_pass(ctx, case):
+    """Each hot-path element of test/verify_elements.jl on the reference's 10 x 6 input."""
+    elem = dict(LT.element_cases())[case]
+    beam = Beam(ENERGY_VAL)
+    lat = Lattice([elem])
+    oc, olf, _, _ = O.linepass(p10(), lat, beam, np.zeros(10, np.int64))
+    gc, glf, _, _ = gpu_track(ctx, lat, beam, p10(), 1)
+    assert np.array_equal(glf, olf)
+    if case in LT.BITEXACT:
+        assert np.array_equal(gc, oc), np.max(np.abs(gc - oc))
+    else:
+        assert np.max(np.abs(gc - oc)) < PARITY_ATOL
+    fc, flf, _, _ = gpu_track(ctx, lat, beam, p10(), 1, fast=True)
+    assert np.array_equal(flf, olf)
+    assert np.max(np.abs(fc - oc)) < 1e-15
+
+
+@pytest.mark.parametrize("name", ["fodo", "dba", "bend", "exact_bend"])
+def test_reference_lines(ctx, name):
+    """The lines of test/verify_{fodo,dba,bend,exact_bend}.jl, one pass (atol 1e-15)."""
+    lat = {"fodo": LT.verify_fodo_line, "dba": LT.verify_dba_line, "bend": LT.verify_bend_line,
+           "exact_bend": LT.verify_exact_bend_line}[name]()
+    beam = Beam(ENERGY_VAL)
+    oc, olf, _, _ = O.linepass(p10(), lat, beam, np.zeros(10, np.int64))
+    gc, glf, _, _ = gpu_track(ctx, lat, beam, p10(), 1)
+    assert np.array_equal(glf, olf)
+    if name != "exact_bend":
+        assert np.array_equal(gc, oc)
+    assert np.max(np.abs(gc - oc)) < PARITY_ATOL
+    fc, _, _, _ = gpu_track(ctx, lat, beam, p10(), 1, fast=True)
+    assert np.max(np.abs(fc - oc)) < PARITY_ATOL
+
+
+# --------------------------------------------------------------------------------------
+# Multi-turn rings
+# --------------------------------------------------------------------------------------
+def test_fodo_bitexact_multiturn(ctx):
+    """FODO workload (n=4 Yoshida, R = 2 cm aperture), 3,000 particles x 100 turns: the
+    parity build reproduces the oracle bit for bit, coordinates and loss record."""
+    beam = Beam(W.ENERGY)
+    lat = W.fodo_lattice()
+    c0 = W.fodo_particles(3000, seed=5)
+    c0[:20, 0] = np.linspace(0.0195, 0.0205, 20)  # straddle the aperture at the start
+    oc, olf, olt, ole = O.ringpass(c0, lat, beam, np.zeros(3000, np.int64), 100, aperture=0.02, nthreads=8)
+    gc, glf, glt, gle = gpu_track(ctx, lat, beam, c0, 100, aperture=0.02)
+    assert np.array_equal(glf, olf) and np.array_equal(glt, olt) and np.array_equal(gle, ole)
+    assert np.array_equal(gc, oc), np.max(np.abs(gc - oc))
+    assert olf.sum() > 0  # the aperture is exercised
+    ok, err = ring_close(gpu_track(ctx, lat, beam, c0, 100, aperture=0.02, fast=True)[0], oc, 1e-10)
+    assert ok, err
+
+
+@pytest.mark.parametrize("nturns,rtol", [(1, 1e-12), (100, 1e-9)])
+@pytest.mark.parametrize("fast", [False, True])
+def test_dba_ring(ctx, nturns, rtol, fast):
+    """DBA workload ring (exact bends with linear edges, sextupoles, RF), 512 particles."""
+    beam = Beam(W.ENERGY)
+    lat = W.dba_ring(beam=beam)
+    c0 = W.dba_particles(512, seed=3)
+    oc, olf, olt, ole = O.ringpass(c0, lat, beam, np.zeros(512, np.int64), nturns, nthreads=8)
+    gc, glf, glt, gle = gpu_track(ctx, lat, beam, c0, nturns, fast=fast)
+    assert np.array_equal(glf, olf) and np.array_equal(glt, olt) and np.array_equal(gle, ole)
+    ok, err = ring_close(gc, oc, rtol)
+    assert ok, err
+
+
+def test_forest_exact_bend_ring(ctx):
+    """Reference ExactSBend (yrot, Forest fringe, wedge) with e1, e2 != 0 in a ring of
+    quads: 5 turns, tolerance 1e-12."""
+    beam = Beam(W.ENERGY)
+    from trackpad_hip import ExactSBend
+    cell = [Drift(1.0), Quadrupole(0.4, 1.2), Drift(0.5), ExactSBend(1.0, 0.1, 0.03, 0.02, gk=0.01),
+            Drift(0.5), Quadrupole(0.4, -1.2), Drift(1.0)]
+    lat = Lattice(cell * 4)
+    c0 = W.gaussian_particles(256, 9, 1e-4, 1e-5, 1e-4, 1e-5, 1e-3)
+    oc, olf, _, _ = O.ringpass(c0, lat, beam, np.zeros(256, np.int64), 5)
+    gc, glf, _, _ = gpu_track(ctx, lat, beam, c0, 5)
+    assert np.array_equal(glf, olf)
+    ok, err = ring_close(gc, oc, 1e-12)
+    assert ok, err
+
+
+def test_da_grid_orders_to_10(ctx):
+    """DA ring (order-10 normal/skew errors on every magnet) on a 16 x 16 x 3 grid, 10
+    turns: loss records identical away from the boundary, survivors within 1e-9."""
+    beam = Beam(W.ENERGY)
+    lat = W.da_ring(ncell=80, beam=beam)
+    grid = W.da_grid(16, 16)
+    n = grid.shape[0]
+    oc, olf, olt, ole = O.ringpass(grid, lat, beam, np.zeros(n, np.int64), 10, nthreads=8)
+    gc, glf, glt, gle = gpu_track(ctx, lat, beam, grid, 10)
+    # boundary band: grid points with a neighbour of different fate
+    lost = olf.reshape(3, 16, 16).astype(bool)
+    band = np.zeros_like(lost)
+    for ax in (1, 2):
+        for sh in (1, -1):
+            band |= lost != np.roll(lost, sh, axis=ax)
+    keep = ~band.reshape(-1)
+    assert np.array_equal(glf[keep], olf[keep])
+    assert np.array_equal(glt[keep], olt[keep]) and np.array_equal(gle[keep], ole[keep])
+    alive = keep & (olf == 0)
+    assert alive.sum() > 100
+    ok, err = ring_close(gc[alive], oc[alive], 1e-9)
+    assert ok, err
+
+
+# --------------------------------------------------------------------------------------
+# Loss semantics and edge cases
+# --------------------------------------------------------------------------------------
+def test_loss_semantics(ctx):
+    """src/lattice.jl:269-284: flag-1 inputs are skipped; a particle lost at an element
+    exit keeps its element-entry coordinates; NaN / pz^2 <= 0 / |x| > 1 all lose."""
+    beam = Beam(W.ENERGY)
+    lat = Lattice([Drift(1.0), Quadrupole(0.5, 1.0), Drift(1.0)])
+    c = np.zeros((6, 6))
+    c[0] = [0.999, 0.01, 0, 0, 0, 0]      # |x| > 1 after the first drift
+    c[1] = [0.0, 0.9, 0, 0.9, 0, 0]       # pz^2 < 0 -> NaN -> lost at the first drift
+    c[2] = [np.nan, 0, 0, 0, 0, 0]        # NaN input
+    c[3] = [1e-3, 1e-4, 0, 0, 0, 0]       # survives
+    c[4] = [5.0, 0, 0, 0, 0, 0]           # arrives flagged: skipped, untouched
+    c[5] = [0.0, 0.0, 0.5, 0.6, 0, 0]     # |y| > 1 inside the quad-drift region
+    lf = np.array([0, 0, 0, 0, 1, 0], np.int64)
+    oc, olf, olt, ole = O.ringpass(c, lat, beam, lf, 2)
+    gc, glf, glt, gle = gpu_track(ctx, lat, beam, c, 2, lost_flags=lf)
+    assert list(olf) == [1, 1, 1, 0, 1, 1]
+    assert np.array_equal(glf, olf) and np.array_equal(glt, olt) and np.array_equal(gle, ole)
+    assert list(glt) == [0, 0, 0, -1, -2, 0]
+    np.testing.assert_array_equal(gc[[0, 1, 3, 4, 5]], oc[[0, 1, 3, 4, 5]])
+    assert np.isnan(gc[2, 0]) and np.array_equal(gc[2, 1:], oc[2, 1:])
+    assert np.array_equal(gc[0], c[0]) and np.array_equal(gc[4], c[4])
+
+
+def test_empty_and_ragged(ctx):
+    beam = Beam(W.ENERGY)
+    lat = W.fodo_lattice()
+    # empty particle set
+    gc, glf, _, _ = gpu_track(ctx, lat, beam, np.zeros((0, 6)), 3)
+    assert gc.shape == (0, 6)
+    # empty lattice: identity
+    c0 = W.fodo_particles(77, seed=8)
+    gc, glf, _, _ = gpu_track(ctx, Lattice([]), beam, c0, 5)
+    assert np.array_equal(gc, c0) and glf.sum() == 0
+    # ragged sizes, launch shapes and turn chunking all give the same bits
+    for n in (1, 63, 65, 257, 1001):
+        c0 = W.fodo_particles(n, seed=n)
+        ref, rlf, _, _ = O.ringpass(c0, lat, beam, np.zeros(n, np.int64), 7, aperture=0.02)
+        for block, ppl, tpl in ((256, 1, 0), (64, 2, 3), (128, 1, 1), (192, 2, 0)):
+            gc, glf, _, _ = gpu_track(ctx, lat, beam, c0, 7, aperture=0.02, block=block, ppl=ppl, tpl=tpl)
+            assert np.array_equal(gc, ref) and np.array_equal(glf, rlf), (n, block, ppl, tpl)
+
+
+def test_turn_continuation(ctx):
+    """ringpass! resumes from (coords, lost_flags, turn): 2 x 5 turns == 10 turns."""
+    beam = Beam(W.ENERGY)
+    lat = W.dba_ring(beam=beam)
+    c0 = W.dba_particles(200, seed=4)
+    a = gpu_track(ctx, lat, beam, c0, 10)
+    b1 = gpu_track(ctx, lat, beam, c0, 5)
+    b2 = gpu_track(ctx, lat, beam, b1[0], 5, lost_flags=b1[1], turn=5)
+    assert np.array_equal(a[0], b2[0])
+
+
+def test_host_entry_points(ctx):
+    """linepass_/ringpass_ mutate host arrays in place like linepass!/ringpass!."""
+    from trackpad_hip import linepass_, ringpass_
+
+    beam = Beam(ENERGY_VAL)
+    lat = LT.verify_dba_line()
+    c = p10()
+    lf = np.zeros(10, np.int64)
+    linepass_(c, lat, beam, lf, ctx=ctx)
+    oc, olf, _, _ = O.linepass(p10(), lat, beam, np.zeros(10, np.int64))
+    assert np.array_equal(c, oc) and np.array_equal(lf, olf)
+    c = np.asfortranarray(p10())
+    lf = np.zeros(10, np.int64)
+    ringpass_(c, lat, beam, lf, 3, ctx=ctx)
+    oc, _, _, _ = O.ringpass(p10(), lat, beam, np.zeros(10, np.int64), 3)
+    assert np.array_equal(c, oc)

@@ -1,0 +1,480 @@
+// tp_abi.hip -- the C ABI of libtrackpad_hip (declared in include/trackpad_hip.h).
+//
+// Host-side runtime: contexts (device + stream), table validation and upload, device
+// particle sets, turn chunking and kernel dispatch.  No exception crosses the ABI; every
+// failure returns a TP_E* code and leaves a message in tp_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "trackpad_hip.h"
+#include "tp_device.h"
+
+struct tp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int block = 256;
+  int ppl = 1;
+  int tpl = 0;  // turns per launch, 0 = all
+  tp_timing_t timing{};
+};
+
+struct tp_table {
+  tp_ctx* ctx = nullptr;
+  void* dev = nullptr;
+  size_t nbytes = 0;
+  tp_table_header hdr{};
+  const int32_t* d_seq = nullptr;
+  const tp_rec* d_recs = nullptr;
+  const double* d_pool = nullptr;
+  unsigned features = 0;
+  int variant = TP_VARIANT_ALL;
+  std::vector<tp_rec> recs;      // host copy with the kick specialisation filled in
+  std::vector<int32_t> seq;      // host copy
+  std::vector<double> pool;      // host copy
+};
+
+struct tp_particles {
+  tp_ctx* ctx = nullptr;
+  int64_t n = 0;
+  double* d_coords = nullptr;
+  int32_t* d_lt = nullptr;
+  int32_t* d_le = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define TP_HIP(call)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess) return fail(TP_EHIP, "%s failed: %s (%s)", #call, hipGetErrorName(e_), \
+                                      hipGetErrorString(e_));                             \
+  } while (0)
+
+// Length in doubles of a record's parameter block (trackpad_table.h).
+int64_t block_len(int tag, int m) {
+  const int64_t poly = 2 * (int64_t)(m + 1);
+  switch (tag) {
+    case TP_MARKER: return 0;
+    case TP_DRIFT: return 2;
+    case TP_MPOLE: return 6 + poly;
+    case TP_THINMP: return poly;
+    case TP_SBEND: return 17 + poly;
+    case TP_ESBEND:
+    case TP_ESBEND_LE: return 24 + poly;
+    case TP_RFCAV: return 7;
+    default: return -1;
+  }
+}
+
+int64_t poly_off(int tag) {
+  switch (tag) {
+    case TP_MPOLE: return 6;
+    case TP_THINMP: return 0;
+    case TP_SBEND: return 17;
+    case TP_ESBEND:
+    case TP_ESBEND_LE: return 24;
+    default: return -1;
+  }
+}
+
+// Kick specialisation from the structural zeros of the polynomial (indices <= m).
+int classify_kick(const double* poly, int m) {
+  bool nz_b1 = false, nz_b2 = false, nz_other = false;
+  for (int i = 0; i <= m; ++i) {
+    const double b = poly[2 * i], a = poly[2 * i + 1];
+    if (a != 0.0) nz_other = true;
+    if (b != 0.0) {
+      if (i == 1) nz_b1 = true;
+      else if (i == 2) nz_b2 = true;
+      else nz_other = true;
+    }
+  }
+  if (nz_other || (nz_b1 && nz_b2)) return TP_KICK_GEN;
+  if (nz_b1) return TP_KICK_QUAD;
+  if (nz_b2) return TP_KICK_SEXT;
+  return TP_KICK_NONE;
+}
+
+int set_dev(tp_ctx* ctx) {
+  TP_HIP(hipSetDevice(ctx->device));
+  return TP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tp_version(void) { return TP_ABI_VERSION; }
+
+const char* tp_last_error(void) { return g_err.c_str(); }
+
+int tp_device_count(int* n) {
+  if (!n) return fail(TP_EINVAL, "tp_device_count: n is NULL");
+  TP_HIP(hipGetDeviceCount(n));
+  return TP_OK;
+}
+
+int tp_ctx_create(int device, tp_ctx** out) {
+  if (!out) return fail(TP_EINVAL, "tp_ctx_create: out is NULL");
+  *out = nullptr;
+  int nd = 0;
+  TP_HIP(hipGetDeviceCount(&nd));
+  if (device < 0 || device >= nd) return fail(TP_EINVAL, "tp_ctx_create: device %d out of range [0, %d)", device, nd);
+  TP_HIP(hipSetDevice(device));
+  tp_ctx* c = new (std::nothrow) tp_ctx();
+  if (!c) return fail(TP_ENOMEM, "tp_ctx_create: out of host memory");
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e != hipSuccess) {
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(TP_EHIP, "tp_ctx_create: %s", hipGetErrorName(e));
+  }
+  *out = c;
+  return TP_OK;
+}
+
+int tp_ctx_destroy(tp_ctx* ctx) {
+  if (!ctx) return TP_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipEventDestroy(ctx->ev0);
+  (void)hipEventDestroy(ctx->ev1);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return TP_OK;
+}
+
+int tp_set_launch(tp_ctx* ctx, int block, int ppl, int turns_per_launch) {
+  if (!ctx) return fail(TP_EINVAL, "tp_set_launch: ctx is NULL");
+  if (block == 0) block = 256;
+  if (ppl == 0) ppl = 1;
+  if (block < 64 || block > 256 || block % 64 != 0)
+    return fail(TP_EINVAL, "tp_set_launch: block %d not in {64, 128, 192, 256}", block);
+  if (ppl != 1 && ppl != 2) return fail(TP_EINVAL, "tp_set_launch: ppl %d not 1 or 2", ppl);
+  if (turns_per_launch < 0) return fail(TP_EINVAL, "tp_set_launch: turns_per_launch %d < 0", turns_per_launch);
+  ctx->block = block;
+  ctx->ppl = ppl;
+  ctx->tpl = turns_per_launch;
+  return TP_OK;
+}
+
+int tp_timing(tp_ctx* ctx, tp_timing_t* out) {
+  if (!ctx || !out) return fail(TP_EINVAL, "tp_timing: NULL argument");
+  *out = ctx->timing;
+  return TP_OK;
+}
+
+int tp_table_create(tp_ctx* ctx, const void* bytes, size_t nbytes, tp_table** out) {
+  if (!ctx || !bytes || !out) return fail(TP_EINVAL, "tp_table_create: NULL argument");
+  *out = nullptr;
+  if (nbytes < sizeof(tp_table_header)) return fail(TP_ETABLE, "table: %zu bytes is shorter than the header", nbytes);
+  tp_table_header h;
+  std::memcpy(&h, bytes, sizeof h);
+  if (h.magic != TP_TABLE_MAGIC) return fail(TP_ETABLE, "table: bad magic 0x%08x", h.magic);
+  if (h.version != TP_TABLE_VERSION) return fail(TP_ETABLE, "table: version %u, library reads %u", h.version, TP_TABLE_VERSION);
+  const size_t seq_bytes = ((size_t)h.n_seq * 4 + 7) / 8 * 8;
+  const size_t need = sizeof h + seq_bytes + (size_t)h.n_rec * sizeof(tp_rec) + (size_t)h.pool_len * 8;
+  if (need != nbytes) return fail(TP_ETABLE, "table: size %zu, header implies %zu", nbytes, need);
+  if (h.n_seq > 0 && h.n_rec == 0) return fail(TP_ETABLE, "table: %u positions but no records", h.n_seq);
+  if (!(std::isfinite(h.beti) && h.beti >= 1.0)) return fail(TP_ETABLE, "table: beti %g is not >= 1", h.beti);
+  if (!(h.coord_limit > 0.0) || !(h.angle_limit > 0.0) || !(h.aperture_r2 >= 0.0))
+    return fail(TP_ETABLE, "table: bad loss limits");
+
+  tp_table* t = new (std::nothrow) tp_table();
+  if (!t) return fail(TP_ENOMEM, "tp_table_create: out of host memory");
+  t->ctx = ctx;
+  t->hdr = h;
+  const char* b = (const char*)bytes;
+  t->seq.resize(h.n_seq);
+  std::memcpy(t->seq.data(), b + sizeof h, (size_t)h.n_seq * 4);
+  t->recs.resize(h.n_rec);
+  std::memcpy(t->recs.data(), b + sizeof h + seq_bytes, (size_t)h.n_rec * sizeof(tp_rec));
+  t->pool.resize(h.pool_len);
+  std::memcpy(t->pool.data(), b + sizeof h + seq_bytes + (size_t)h.n_rec * sizeof(tp_rec), (size_t)h.pool_len * 8);
+
+  auto bad = [&](int code, const std::string& msg) {
+    delete t;
+    return fail(code, "%s", msg.c_str());
+  };
+  for (uint32_t i = 0; i < h.n_seq; ++i)
+    if (t->seq[i] < 0 || (uint32_t)t->seq[i] >= h.n_rec)
+      return bad(TP_ETABLE, "table: seq[" + std::to_string(i) + "] = " + std::to_string(t->seq[i]) + " out of range");
+  unsigned feat = 0;
+  if (!(h.flags & TP_H_EXACT_HAMILTONIAN)) feat |= FT_LINH;
+  for (uint32_t i = 0; i < h.n_rec; ++i) {
+    tp_rec& r = t->recs[i];
+    const std::string where = "table: record " + std::to_string(i) + ": ";
+    if (r.tag < 0 || r.tag >= TP_NTAGS) return bad(TP_EUNSUPPORTED, where + "unknown tag " + std::to_string(r.tag));
+    if (r.max_order < 0 || r.max_order > TP_MAXORD)
+      return bad(TP_EUNSUPPORTED, where + "max_order " + std::to_string(r.max_order) + " outside [0, 10]");
+    if (r.nsteps < 0 || r.nsteps > 1000000) return bad(TP_ETABLE, where + "nsteps " + std::to_string(r.nsteps));
+    const int64_t len = block_len(r.tag, r.max_order);
+    if (r.off < 0 || (int64_t)r.off + len > (int64_t)h.pool_len) return bad(TP_ETABLE, where + "parameter block outside the pool");
+    if (r.flags & TP_F_MISALIGN) {
+      if (r.tag == TP_MARKER || r.tag == TP_RFCAV) return bad(TP_ETABLE, where + "misalignment on an element without one");
+      if (r.moff < 0 || (int64_t)r.moff + 84 > (int64_t)h.pool_len) return bad(TP_ETABLE, where + "misalignment block outside the pool");
+      feat |= FT_MISALIGN;
+    }
+    r.kick = TP_KICK_NONE;
+    const int64_t po = poly_off(r.tag);
+    if (po >= 0) {
+      r.kick = classify_kick(t->pool.data() + r.off + po, r.max_order);
+      if (r.kick == TP_KICK_GEN) feat |= FT_GENKICK;
+    }
+    if (r.tag == TP_SBEND) feat |= FT_SBEND;
+    if (r.tag == TP_ESBEND) feat |= FT_ESBEND;
+    if (r.tag == TP_ESBEND_LE) feat |= FT_ESBEND_LE;
+    if (r.tag == TP_RFCAV) feat |= FT_RF;
+  }
+  t->features = feat;
+  if ((feat & ~TP_FEAT_A) == 0) t->variant = TP_VARIANT_A;
+  else if ((feat & ~TP_FEAT_B) == 0) t->variant = TP_VARIANT_B;
+  else if ((feat & ~TP_FEAT_C) == 0) t->variant = TP_VARIANT_C;
+  else t->variant = TP_VARIANT_ALL;
+
+  // Device copy: the same layout with the library's kick field written.
+  std::vector<char> blob(b, b + nbytes);
+  std::memcpy(blob.data() + sizeof h + seq_bytes, t->recs.data(), (size_t)h.n_rec * sizeof(tp_rec));
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hipMalloc(&t->dev, nbytes);
+  if (e == hipSuccess) e = hipMemcpyAsync(t->dev, blob.data(), nbytes, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    if (t->dev) (void)hipFree(t->dev);
+    delete t;
+    return fail(e == hipErrorOutOfMemory ? TP_ENOMEM : TP_EHIP, "tp_table_create: %s", hipGetErrorName(e));
+  }
+  t->nbytes = nbytes;
+  const char* d = (const char*)t->dev;
+  t->d_seq = (const int32_t*)(d + sizeof h);
+  t->d_recs = (const tp_rec*)(d + sizeof h + seq_bytes);
+  t->d_pool = (const double*)(d + sizeof h + seq_bytes + (size_t)h.n_rec * sizeof(tp_rec));
+  *out = t;
+  return TP_OK;
+}
+
+int tp_table_destroy(tp_table* t) {
+  if (!t) return TP_OK;
+  (void)hipSetDevice(t->ctx->device);
+  (void)hipStreamSynchronize(t->ctx->stream);
+  if (t->dev) (void)hipFree(t->dev);
+  delete t;
+  return TP_OK;
+}
+
+int tp_table_info(const tp_table* t, int32_t* n_seq, int32_t* n_rec, double* beti) {
+  if (!t) return fail(TP_EINVAL, "tp_table_info: table is NULL");
+  if (n_seq) *n_seq = (int32_t)t->hdr.n_seq;
+  if (n_rec) *n_rec = (int32_t)t->hdr.n_rec;
+  if (beti) *beti = t->hdr.beti;
+  return TP_OK;
+}
+
+int tp_particles_create(tp_ctx* ctx, int64_t n, tp_particles** out) {
+  if (!ctx || !out) return fail(TP_EINVAL, "tp_particles_create: NULL argument");
+  *out = nullptr;
+  if (n < 0 || n > ((int64_t)1 << 40)) return fail(TP_EINVAL, "tp_particles_create: n = %lld", (long long)n);
+  int rc = set_dev(ctx);
+  if (rc) return rc;
+  tp_particles* p = new (std::nothrow) tp_particles();
+  if (!p) return fail(TP_ENOMEM, "tp_particles_create: out of host memory");
+  p->ctx = ctx;
+  p->n = n;
+  const size_t m = (size_t)(n > 0 ? n : 1);
+  hipError_t e = hipMalloc(&p->d_coords, 6 * m * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&p->d_lt, m * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMalloc(&p->d_le, m * sizeof(int32_t));
+  if (e != hipSuccess) {
+    if (p->d_coords) (void)hipFree(p->d_coords);
+    if (p->d_lt) (void)hipFree(p->d_lt);
+    delete p;
+    return fail(e == hipErrorOutOfMemory ? TP_ENOMEM : TP_EHIP, "tp_particles_create(%lld): %s", (long long)n,
+                hipGetErrorName(e));
+  }
+  *out = p;
+  return TP_OK;
+}
+
+int tp_particles_destroy(tp_particles* p) {
+  if (!p) return TP_OK;
+  (void)hipSetDevice(p->ctx->device);
+  (void)hipStreamSynchronize(p->ctx->stream);
+  (void)hipFree(p->d_coords);
+  (void)hipFree(p->d_lt);
+  (void)hipFree(p->d_le);
+  delete p;
+  return TP_OK;
+}
+
+int tp_particles_upload(tp_particles* p, const double* coords, const int64_t* lost_flags01,
+                        const int32_t* lost_turn, const int32_t* lost_elem) {
+  if (!p || !coords) return fail(TP_EINVAL, "tp_particles_upload: NULL argument");
+  int rc = set_dev(p->ctx);
+  if (rc) return rc;
+  const int64_t n = p->n;
+  if (n == 0) return TP_OK;
+  std::vector<int32_t> lt((size_t)n), le((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t t = -1, e = -1;
+    if (lost_turn) {
+      t = lost_turn[i];
+      e = lost_elem ? lost_elem[i] : (t == -1 ? -1 : -2);
+    } else if (lost_flags01 && lost_flags01[i] == 1) {
+      t = -2;
+      e = -2;
+    }
+    lt[(size_t)i] = t;
+    le[(size_t)i] = e;
+  }
+  hipStream_t s = p->ctx->stream;
+  TP_HIP(hipMemcpyAsync(p->d_coords, coords, 6 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s));
+  TP_HIP(hipMemcpyAsync(p->d_lt, lt.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  TP_HIP(hipMemcpyAsync(p->d_le, le.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  TP_HIP(hipStreamSynchronize(s));
+  return TP_OK;
+}
+
+int tp_particles_download(tp_particles* p, double* coords, int64_t* lost_flags01, int32_t* lost_turn,
+                          int32_t* lost_elem) {
+  if (!p) return fail(TP_EINVAL, "tp_particles_download: particles is NULL");
+  int rc = set_dev(p->ctx);
+  if (rc) return rc;
+  const int64_t n = p->n;
+  if (n == 0) return TP_OK;
+  hipStream_t s = p->ctx->stream;
+  std::vector<int32_t> lt;
+  int32_t* ltp = lost_turn;
+  if (lost_flags01 && !ltp) {
+    lt.resize((size_t)n);
+    ltp = lt.data();
+  }
+  if (coords) TP_HIP(hipMemcpyAsync(coords, p->d_coords, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (ltp) TP_HIP(hipMemcpyAsync(ltp, p->d_lt, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (lost_elem) TP_HIP(hipMemcpyAsync(lost_elem, p->d_le, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  TP_HIP(hipStreamSynchronize(s));
+  if (lost_flags01)
+    for (int64_t i = 0; i < n; ++i) lost_flags01[i] = (ltp[i] != -1) ? 1 : 0;
+  return TP_OK;
+}
+
+int tp_particles_device_ptrs(tp_particles* p, double** coords, int32_t** lost_turn, int32_t** lost_elem,
+                             int64_t* n) {
+  if (!p) return fail(TP_EINVAL, "tp_particles_device_ptrs: particles is NULL");
+  if (coords) *coords = p->d_coords;
+  if (lost_turn) *lost_turn = p->d_lt;
+  if (lost_elem) *lost_elem = p->d_le;
+  if (n) *n = p->n;
+  return TP_OK;
+}
+
+int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, uint32_t flags) {
+  if (!ctx || !t || !p) return fail(TP_EINVAL, "tp_track: NULL argument");
+  if (t->ctx != ctx || p->ctx != ctx) return fail(TP_EINVAL, "tp_track: table and particles belong to another context");
+  if (nturns < 0) return fail(TP_EINVAL, "tp_track: nturns = %d", nturns);
+  if (flags & ~(TP_TRACK_FAST | TP_TRACK_ASYNC | TP_TRACK_TIMED)) return fail(TP_EINVAL, "tp_track: unknown flags 0x%x", flags);
+  if ((int64_t)turn0 + (int64_t)nturns > 0x7fffffffLL) return fail(TP_EINVAL, "tp_track: turn range overflows int32");
+  int rc = set_dev(ctx);
+  if (rc) return rc;
+  TpKArgs a{};
+  a.seq = t->d_seq;
+  a.recs = t->d_recs;
+  a.pool = t->d_pool;
+  a.n_seq = (int32_t)t->hdr.n_seq;
+  a.exact_h = (t->hdr.flags & TP_H_EXACT_HAMILTONIAN) ? 1 : 0;
+  a.beti = t->hdr.beti;
+  a.coord_limit = t->hdr.coord_limit;
+  a.angle_limit = t->hdr.angle_limit;
+  a.ap_r2 = t->hdr.aperture_r2;
+  a.coords = p->d_coords;
+  a.n = p->n;
+  a.ld = p->n;
+  a.lost_turn = p->d_lt;
+  a.lost_elem = p->d_le;
+  a.variant = t->variant;
+  const bool timed = (flags & TP_TRACK_TIMED) != 0;
+  const int chunk = ctx->tpl > 0 ? ctx->tpl : (nturns > 0 ? nturns : 1);
+  double ms_total = 0.0;
+  int launches = 0;
+  if (p->n > 0 && a.n_seq > 0) {
+    for (int32_t done = 0; done < nturns; done += chunk) {
+      a.turn0 = turn0 + done;
+      a.nturns = (nturns - done < chunk) ? (nturns - done) : chunk;
+      if (timed) TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+      hipError_t e = (flags & TP_TRACK_FAST) ? tp_launch_track_fast(a, ctx->block, ctx->ppl, ctx->stream)
+                                             : tp_launch_track_exact(a, ctx->block, ctx->ppl, ctx->stream);
+      if (e != hipSuccess) return fail(TP_EHIP, "tp_track: kernel launch failed: %s", hipGetErrorName(e));
+      ++launches;
+      if (timed) {
+        TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        TP_HIP(hipEventSynchronize(ctx->ev1));
+        float ms = 0.f;
+        TP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ms_total += ms;
+      }
+    }
+  }
+  if (timed) {
+    ctx->timing.kernel_ms = ms_total;
+    ctx->timing.launches = launches;
+    ctx->timing.block = ctx->block;
+    ctx->timing.ppl = ctx->ppl;
+    ctx->timing.turns_per_launch = chunk;
+  }
+  if (!(flags & TP_TRACK_ASYNC)) TP_HIP(hipStreamSynchronize(ctx->stream));
+  return TP_OK;
+}
+
+int tp_sync(tp_ctx* ctx) {
+  if (!ctx) return fail(TP_EINVAL, "tp_sync: ctx is NULL");
+  int rc = set_dev(ctx);
+  if (rc) return rc;
+  TP_HIP(hipStreamSynchronize(ctx->stream));
+  return TP_OK;
+}
+
+int tp_track_host(tp_ctx* ctx, const tp_table* t, double* coords, int64_t n, int64_t* lost_flags, int32_t* lost_turn,
+                  int32_t* lost_elem, int32_t turn0, int32_t nturns, uint32_t flags) {
+  if (!ctx || !t || (!coords && n > 0) || !lost_flags) return fail(TP_EINVAL, "tp_track_host: NULL argument");
+  tp_particles* p = nullptr;
+  int rc = tp_particles_create(ctx, n, &p);
+  if (rc) return rc;
+  std::vector<int32_t> lt((size_t)(n > 0 ? n : 1)), le((size_t)(n > 0 ? n : 1));
+  rc = tp_particles_upload(p, coords, lost_flags, nullptr, nullptr);
+  if (!rc) rc = tp_track(ctx, t, p, turn0, nturns, flags & ~TP_TRACK_ASYNC);
+  if (!rc) rc = tp_particles_download(p, coords, nullptr, lt.data(), le.data());
+  tp_particles_destroy(p);
+  if (rc) return rc;
+  for (int64_t i = 0; i < n; ++i) {
+    // reference semantics: flags equal to 1 were skipped; newly lost particles get 1;
+    // every other entry is left as it was (src/lattice.jl:269-284).
+    if (lt[(size_t)i] >= 0) lost_flags[i] = 1;
+    if (lost_turn) lost_turn[i] = lt[(size_t)i];
+    if (lost_elem) lost_elem[i] = le[(size_t)i];
+  }
+  return TP_OK;
+}
+
+}  // extern "C"

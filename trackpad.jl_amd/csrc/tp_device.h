@@ -1,0 +1,63 @@
+// tp_device.h -- declarations shared by the kernel translation units and the C ABI.
+#ifndef TP_DEVICE_H
+#define TP_DEVICE_H
+
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+#include "trackpad_table.h"
+
+// Kick specialisations, chosen per record at table load from the structural zeros of
+// the polynomial.  Every specialisation evaluates the same products and sums as the
+// full Horner recurrence of src/tracking.jl:290-298 minus the terms that are a product
+// with an exact zero, so finite nonzero results are bit-identical; only the sign of an
+// exactly-zero result can differ.
+enum tp_kick {
+  TP_KICK_NONE = 0,  // all coefficients zero: the kick is the identity
+  TP_KICK_QUAD = 1,  // m = 1, only b1 nonzero
+  TP_KICK_SEXT = 2,  // m = 2, only b2 nonzero
+  TP_KICK_GEN = 3    // anything else: full Horner over max_order
+};
+
+// Kernel feature bits: a kernel variant contains only the element maps its mask names.
+enum : unsigned {
+  FT_SBEND = 1u,      // expanded-Hamiltonian bends (TP_SBEND)
+  FT_ESBEND = 2u,     // Forest exact bends with yrot / fringe / wedge (TP_ESBEND)
+  FT_ESBEND_LE = 4u,  // exact bend body with linear edges (TP_ESBEND_LE)
+  FT_RF = 8u,         // RF cavities
+  FT_MISALIGN = 16u,  // t1/t2/r1/r2 misalignments
+  FT_GENKICK = 32u,   // general Horner kicks up to order 10
+  FT_LINH = 64u       // linearised drift (USE_EXACT_HAMILTONIAN = false)
+};
+// Compiled variants, smallest first; the host picks the first whose mask covers the table.
+#define TP_FEAT_A 0u                                              // drift/multipole lattices
+#define TP_FEAT_B (FT_ESBEND_LE | FT_RF)                          // rings of exact bends + RF
+#define TP_FEAT_C (FT_ESBEND_LE | FT_RF | FT_SBEND | FT_GENKICK)  // + high-order multipoles
+#define TP_FEAT_ALL 127u
+enum { TP_VARIANT_A = 0, TP_VARIANT_B = 1, TP_VARIANT_C = 2, TP_VARIANT_ALL = 3 };
+
+struct TpKArgs {
+  const int32_t* seq;
+  const tp_rec* recs;
+  const double* pool;
+  int32_t n_seq;
+  int32_t exact_h;  // USE_EXACT_HAMILTONIAN
+  double beti;
+  double coord_limit;
+  double angle_limit;
+  double ap_r2;
+  double* coords;  // 6 planes of ld doubles
+  int64_t n;
+  int64_t ld;
+  int32_t* lost_turn;
+  int32_t* lost_elem;
+  int32_t turn0;
+  int32_t nturns;
+  int32_t variant;  // TP_VARIANT_*
+};
+
+// Host launchers exported by the kernel translation units (one per build flavour).
+hipError_t tp_launch_track_exact(const TpKArgs& a, int block, int ppl, hipStream_t stream);
+hipError_t tp_launch_track_fast(const TpKArgs& a, int block, int ppl, hipStream_t stream);
+
+#endif  // TP_DEVICE_H

@@ -1,0 +1,345 @@
+// tp_maps.h -- per-particle element maps for the fused gfx950 tracking kernels.
+//
+// Included by tp_track_kernels.hip, which is compiled twice:
+//   TP_FAST=0  with -ffp-contract=off: the parity build.  Every map performs the
+//              reference's floating-point operations in the reference's order, so for
+//              + - * / sqrt it reproduces the reference bit for bit; per-particle
+//              transcendentals (asin, sin, tan, atan, cos) come from the device math
+//              library and agree to a few ulp.
+//   TP_FAST=1  with -ffp-contract=fast: FMA contraction, reciprocal square roots by
+//              Newton-Raphson, multiplications by hoisted reciprocals and a series
+//              asin for small arguments.  Validated against the parity build and the
+//              oracle to tolerance (1e-12 relative after one turn).
+//
+// Per-particle state lives in registers; every element parameter is read from the pool
+// at a wave-uniform address (scalar loads).  Quantities that depend only on delta
+// (which changes only at RF cavities and misalignments) are hoisted into Cx; each hoist
+// is the exact leading subexpression of the reference formula, so hoisting does not
+// change a single bit.
+#ifndef TP_MAPS_H
+#define TP_MAPS_H
+
+#include <math.h>
+#include "tp_device.h"
+
+#define TP_DEV __device__ __forceinline__
+
+namespace tpk {
+
+// Yoshida-4 constants, src/tracking.jl:29-32 (only used by the adjoint; the forward
+// kernels read the writer's precomputed L1, L2, K1, K2).
+struct St {
+  double x, px, y, py, z, d;
+};
+
+// delta-dependent hoists.  A and B are the leading subexpressions of drift6's
+// pz2 = ((1 + (2 delta) beti) + delta^2) - px^2 - py^2 and of (beti + delta)
+// (src/tracking.jl:90, 98); BB = dp1^2 of pxyz (src/tracking.jl:718) with dp1 = beti+delta;
+// C = 1 + delta of the linearised drift (src/tracking.jl:102).
+struct Cx {
+  double A, B, BB, C;
+};
+
+TP_DEV void mkctx(const St& s, Cx& c, double beti) {
+  c.A = 1.0 + 2.0 * s.d * beti + s.d * s.d;
+  c.B = beti + s.d;
+  c.BB = c.B * c.B;
+  c.C = 1.0 + s.d;
+}
+
+#if TP_FAST
+// 1/sqrt(v) for v > 0: hardware estimate refined by two Newton-Raphson steps
+// y <- y + y (1 - v y^2) / 2 (quadratic convergence from the ~2^-23 estimate).
+TP_DEV double rsqrt_nr(double v) {
+  double y = __builtin_amdgcn_rsq(v);
+  double e = 1.0 - v * y * y;
+  y = y + 0.5 * y * e;
+  e = 1.0 - v * y * y;
+  y = y + 0.5 * y * e;
+  return y;
+}
+// 1/v by two Newton-Raphson steps from the hardware estimate.
+TP_DEV double rcp_nr(double v) {
+  double y = __builtin_amdgcn_rcp(v);
+  double e = 1.0 - v * y;
+  y = y + y * e;
+  e = 1.0 - v * y;
+  y = y + y * e;
+  return y;
+}
+#endif
+
+// Outlined library calls: executed rarely (large arguments, once-per-element edge maps,
+// one RF kick per turn), kept out of line so their temporaries do not raise the register
+// allocation of the hot loop.
+__device__ __noinline__ double asin_lib(double u) { return asin(u); }
+__device__ __noinline__ double tan_lib(double u) { return tan(u); }
+__device__ __noinline__ double sin_lib(double u) { return sin(u); }
+
+// asin(u) for |u| <= 1/8 by its Maclaurin series
+//   asin(u) = sum_k C(2k,k) / (4^k (2k+1)) u^(2k+1),
+// truncated after k = 8 (remainder below 2^-60 |u|, so the result is within rounding
+// of the true value); larger |u| calls the library asin.
+TP_DEV double asin_s(double u) {
+  const double w = u * u;
+  double p = 6435.0 / 557056.0;
+  p = p * w + 143.0 / 10240.0;
+  p = p * w + 231.0 / 13312.0;
+  p = p * w + 63.0 / 2816.0;
+  p = p * w + 35.0 / 1152.0;
+  p = p * w + 5.0 / 112.0;
+  p = p * w + 3.0 / 40.0;
+  p = p * w + 1.0 / 6.0;
+  const double r = u + (u * w) * p;
+  if (__builtin_expect(!(fabs(u) <= 0.125), 0)) return asin_lib(u);
+  return r;
+}
+
+TP_DEV double nan_d() { return __builtin_nan(""); }
+
+// ---------------------------------------------------------------------------------------
+// drift6, src/tracking.jl:87-108
+// ---------------------------------------------------------------------------------------
+template <bool EXH>
+TP_DEV void drift(St& s, const Cx& c, double L, double Lb) {
+  if (EXH) {
+    double pz2 = c.A - s.px * s.px - s.py * s.py;
+#if TP_FAST
+    double NormL = L * rsqrt_nr(pz2);
+    NormL = (pz2 > 0.0) ? NormL : nan_d();
+#else
+    double NormL = L / sqrt(pz2);
+    NormL = (pz2 <= 0.0) ? nan_d() : NormL;  // reference returns NaN x 6 (:91-94)
+#endif
+    s.x = s.x + NormL * s.px;
+    s.y = s.y + NormL * s.py;
+    s.z = s.z + NormL * c.B - Lb;
+  } else {
+    double NormL = L / c.C;
+    s.x = s.x + NormL * s.px;
+    s.y = s.y + NormL * s.py;
+    s.z = s.z + NormL * (s.px * s.px + s.py * s.py) / (2.0 * c.C);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Horner sums of strthinkick / bndthinkick, src/tracking.jl:251-259, 290-298.
+// poly is interleaved (b0, a0, b1, a1, ...).
+// ---------------------------------------------------------------------------------------
+template <int KK>
+TP_DEV void horner(const St& s, const double* __restrict__ poly, int m, double& re, double& im) {
+  if (KK == TP_KICK_NONE) {
+    re = 0.0;
+    im = 0.0;
+  } else if (KK == TP_KICK_QUAD) {
+    const double b1 = poly[2];
+    re = b1 * s.x;
+    im = b1 * s.y;
+  } else if (KK == TP_KICK_SEXT) {
+    const double b2 = poly[4];
+    const double rx = b2 * s.x, iy = b2 * s.y;
+    re = rx * s.x - iy * s.y;
+    im = iy * s.x + rx * s.y;
+  } else {
+    double ReSum = poly[2 * m];
+    double ImSum = poly[2 * m + 1];
+    for (int i = m - 1; i >= 0; --i) {
+      double t = ReSum * s.x - ImSum * s.y + poly[2 * i];
+      ImSum = ImSum * s.x + ReSum * s.y + poly[2 * i + 1];
+      ReSum = t;
+    }
+    re = ReSum;
+    im = ImSum;
+  }
+}
+
+// strthinkick, src/tracking.jl:284-305
+template <int KK>
+TP_DEV void kick(St& s, const double* __restrict__ poly, int m, double K) {
+  if (KK == TP_KICK_NONE) return;
+  double re, im;
+  horner<KK>(s, poly, m, re, im);
+  s.px = s.px - K * re;
+  s.py = s.py + K * im;
+}
+
+// bndthinkick, src/tracking.jl:243-270 (Ki = K*irho, the reference's leading L*irho)
+template <int KK>
+TP_DEV void bkick(St& s, const double* __restrict__ poly, int m, double K, double Ki, double irho, double beti) {
+  double re, im;
+  horner<KK>(s, poly, m, re, im);
+  s.px = s.px - K * (re - (s.d - s.x * irho) * irho);
+  if (KK != TP_KICK_NONE) s.py = s.py + K * im;
+  s.z = s.z + Ki * s.x * beti;
+}
+
+// ---------------------------------------------------------------------------------------
+// Linear dipole edge, src/tracking.jl:160-221.  blk = [fx, fy_const, e, fc].
+// ---------------------------------------------------------------------------------------
+TP_DEV void edge_lin(St& s, int method, const double* __restrict__ blk, double irho, bool is_exit) {
+  const double fx = blk[0], fyc = blk[1], e = blk[2], fc = blk[3];
+  double fy;
+  if (method == 2) {
+    double t = (fc == 0.0) ? fyc : irho * tan_lib(e - fc / (1.0 + s.d));
+    fy = t / (1.0 + s.d);
+  } else if (method == 3) {
+    fy = is_exit ? irho * tan_lib(e - fc - s.px / (1.0 + s.d)) : irho * tan_lib(e - fc + s.px / (1.0 + s.d));
+  } else {  // Brown (1) and the fallback
+    fy = (fc == 0.0) ? fyc : irho * tan_lib(e - fc / (1.0 + s.d));
+  }
+  const double px_new = s.px + s.x * fx;
+  s.py = s.py - s.y * fy;
+  s.px = px_new;
+}
+
+// ---------------------------------------------------------------------------------------
+// Exact-bend helpers (Forest), src/tracking.jl:717-894
+// ---------------------------------------------------------------------------------------
+// pxyz(dp1, px, py) with dp1^2 = c.BB, src/tracking.jl:717-720
+TP_DEV double pxyz(const Cx& c, double px, double py) {
+  double v = c.BB - px * px - py * py;
+#if TP_FAST
+  double r = v * rsqrt_nr(v);
+  return v > 0.0 ? r : 0.0;
+#else
+  return v > 0.0 ? sqrt(v) : 0.0;
+#endif
+}
+
+TP_DEV double clamp1(double v) { return v > 1.0 ? 1.0 : (v < -1.0 ? -1.0 : v); }
+
+// yrot, src/tracking.jl:727-745 (caller skips phi == 0)
+__device__ __noinline__ St yrot(St s, Cx c, double co, double si) {
+  const double pz = pxyz(c, s.px, s.py);
+  const double p = co * pz - si * s.px;
+  const double px_new = si * pz + co * s.px;
+  const double x_new = s.x * pz / p;
+  s.y = s.y + s.x * s.py * si / p;
+  s.z = s.z + c.B * s.x * si / p;
+  s.x = x_new;
+  s.px = px_new;
+  return s;
+}
+
+// bend_fringe (Forest 13.13), src/tracking.jl:752-815
+__device__ __noinline__ St bend_fringe(St s, Cx c, double irho, double gK) {
+  const double b0 = irho;
+  const double dp1 = c.B;
+  const double pz = pxyz(c, s.px, s.py);
+  const double px = s.px;
+  const double py = s.py;
+  const double xp = px / pz;
+  const double yp = py / pz;
+  const double term1 = 1.0 + xp * xp * (2.0 + yp * yp);
+  const double term2 = xp / (1.0 + yp * yp);
+  const double phi = -b0 * tan(b0 * gK * term1 * pz - atan(term2));
+  const double px2 = px * px;
+  const double py2 = py * py;
+  const double pz2 = pz * pz;
+  const double pz4 = pz2 * pz2;
+  const double py2z2 = (py2 + pz2);
+  const double arg_sec = (b0 * gK * (pz4 + px2 * (py2 + 2.0 * pz2))) / (pz2 * pz) - atan(px * pz / (py2 + pz2));
+  const double secv = 1.0 / cos(arg_sec);
+  const double powsec = secv * secv;
+  const double py4 = py2 * py2;
+  const double py6 = py4 * py2;
+  const double pz5 = pz4 * pz;
+  const double pz6 = pz4 * pz2;
+  const double py2z2_sq = py2z2 * py2z2;
+  const double denom = pz5 * (py4 + px2 * pz2 + 2.0 * py2 * pz2 + pz4);
+  const double dpx_num = -(b0 * (px2 * pz4 * (py2 - pz2) - pz6 * (py2 + pz2) +
+                                 b0 * gK * px * (pz2 * py2z2_sq * (2.0 * py2 + 3.0 * pz2) +
+                                                 px2 * px2 * (3.0 * py2 * pz2 + 2.0 * pz4) +
+                                                 px2 * (3.0 * py6 + 8.0 * py4 * pz2 + 9.0 * py2 * pz4 + 5.0 * pz6)))) *
+                         powsec;
+  const double dpx = dpx_num / denom;
+  const double dpy_num = -(b0 * py * (px * pz4 * (py2 + pz2) +
+                                      b0 * gK * (-(pz4 * py2z2_sq) + px2 * px2 * (3.0 * py2 * pz2 + 4.0 * pz4) +
+                                                 px2 * (3.0 * py6 + 10.0 * py4 * pz2 + 11.0 * py2 * pz4 + 3.0 * pz6)))) *
+                         powsec;
+  const double dpy = dpy_num / denom;
+  const double dd_num = (b0 * dp1 * (px * pz4 * (py2 - pz2) + b0 * gK *
+                                     (-(pz4 * py2z2_sq) + px2 * px2 * (3.0 * py2 * pz2 + 2.0 * pz4) +
+                                      px2 * (3.0 * py6 + 8.0 * py4 * pz2 + 7.0 * py2 * pz4 + pz6)))) *
+                        powsec;
+  const double dd = dd_num / denom;
+  const double yf = (2.0 * s.y) / (1.0 + sqrt(1.0 - 2.0 * dpy * s.y));
+  const double dxf = 0.5 * dpx * (yf * yf);
+  const double dct = 0.5 * dd * (yf * yf);
+  const double dpyf = phi * yf;
+  s.x = s.x + dxf;
+  s.y = yf;
+  s.py = s.py - dpyf;
+  s.z = s.z - dct;
+  return s;
+}
+
+// bend_edge (Forest 12.41), src/tracking.jl:822-851 (caller skips |rhoinv| < 1e-6).
+__device__ __noinline__ St bend_edge(St s, Cx c, double rhoinv, double co, double si, double s2t, double theta) {
+  const double pz = pxyz(c, s.px, s.py);
+  const double d2 = pxyz(c, 0.0, s.py);
+  const double px_new = s.px * co + (pz - rhoinv * s.x) * si;
+  const double val1 = clamp1(s.px / d2);
+  const double val2 = clamp1(px_new / d2);
+  const double dasin = asin_s(val1) - asin_s(val2);
+  const double num = s.x * (s.px * s2t + si * si * (2.0 * pz - rhoinv * s.x));
+  const double den = pxyz(c, px_new, s.py) + pz * co - s.px * si;
+  const double x_new = s.x * co + num / den;
+  s.y = s.y + s.py * (theta / rhoinv + dasin / rhoinv);
+  s.z = s.z + c.B / rhoinv * (theta + dasin);
+  s.x = x_new;
+  s.px = px_new;
+  return s;
+}
+
+// exact_bend_body (Forest 12.18), src/tracking.jl:858-894, for |irho| >= 1e-6.
+// d2 = pxyz(dp1, 0, py) is passed in: py is unchanged by the body, and by the kicks
+// between bodies when the kick is the identity, so the caller may hoist it.
+// rho = 1/irho and id2 = 1/d2 are used by the fast build only.
+TP_DEV void ebody(St& s, const Cx& c, double irho, double rho, double L, double Lb, double cs, double sn,
+                  double d2, double id2) {
+  const double pz = pxyz(c, s.px, s.py);
+  const double pzmx = pz - (1.0 + s.x * irho);
+  const double px_new = s.px * cs + pzmx * sn;
+  const double pzn = pxyz(c, px_new, s.py);
+#if TP_FAST
+  // asin(a) - asin(b) = asin(a sqrt(1-b^2) - b sqrt(1-a^2)) with sqrt(1 - (px/d2)^2) = pz/d2:
+  // one series asin of (px*pzn - px_new*pz)/d2^2 instead of two library calls; the
+  // clamp of the reference is kept by falling back to the two-call form when either
+  // argument leaves (-1, 1).
+  const double a1 = s.px * id2, a2 = px_new * id2;
+  double dif;
+  if (fabs(a1) < 0.5 && fabs(a2) < 0.5) {
+    dif = asin_s((s.px * pzn - px_new * pz) * (id2 * id2));
+  } else {
+    dif = asin_s(clamp1(a1)) - asin_s(clamp1(a2));
+  }
+  const double dasin = L + dif * rho;
+  const double x_new = (pzn - pzmx * cs + s.px * sn - 1.0) * rho;
+  (void)d2;
+#else
+  const double val1 = clamp1(s.px / d2);
+  const double val2 = clamp1(px_new / d2);
+  const double dasin = L + (asin_s(val1) - asin_s(val2)) / irho;
+  const double x_new = (pzn - pzmx * cs + s.px * sn - 1.0) / irho;
+  (void)rho;
+  (void)id2;
+#endif
+  s.y = s.y + s.py * dasin;
+  s.z = s.z + c.B * dasin - Lb;
+  s.x = x_new;
+  s.px = px_new;
+}
+
+// Drift limit of exact_bend_body (|irho| < 1e-6), src/tracking.jl:862-872
+TP_DEV void ebody_driftlimit(St& s, const Cx& c, double L, double Lb) {
+  const double pz = pxyz(c, s.px, s.py);
+  const double NormL = L / pz;
+  s.x = s.x + s.px * NormL;
+  s.y = s.y + s.py * NormL;
+  s.z = s.z + NormL * c.B - Lb;
+}
+
+}  // namespace tpk
+
+#endif  // TP_MAPS_H

@@ -1,0 +1,23 @@
+"""trackpad_hip -- MI355X (gfx950) backend for TrackPad.jl's multi-particle tracking path.
+
+Host-side mirror of the reference interface (src/TrackPad.jl:37-61 exports used by the
+path): element types, Beam, Lattice, ``linepass_`` / ``ringpass_`` (Julia's ``linepass!`` /
+``ringpass!``) and the device-resident ``track_``; all tracking runs in libtrackpad_hip
+(hand-written HIP kernels behind a C ABI, include/trackpad_hip.h).
+"""
+
+from .elements import (  # noqa: F401
+    MAXORD, C_LIGHT, AbstractCavity, AbstractDrift, AbstractElement, AbstractMagnet, Drift, ERBend,
+    ExactSBend, ExactSBendLinearEdge, Marker, Octupole, Quadrupole, RBend, RFCavity, SBend, Sextupole,
+    ThinMultipole, TimeVaryingElement, get_length,
+)
+from .lattice import (  # noqa: F401
+    M_ELECTRON, M_PROTON, Beam, Lattice, beti, findelem, materialize_lattice, spos, total_length,
+)
+from .table import TableInfo, compile_table  # noqa: F401
+from ._lib import LIB_PATH, TrackPadHIPError  # noqa: F401
+from .tracking import (  # noqa: F401
+    DeviceTable, HIPContext, HIPParticles, default_context, device_table, linepass_, ringpass_, track_,
+)
+
+__version__ = "0.1.0"

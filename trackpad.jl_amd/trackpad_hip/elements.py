@@ -1,0 +1,338 @@
+"""Element types of the tracking path, mirroring TrackPad.jl's element structs.
+
+Field names, positional arguments, keyword arguments and defaults follow the reference
+constructors so that a lattice written for TrackPad reads the same here:
+
+    Drift          src/elements/drift.jl:6-50
+    Marker         src/elements/marker.jl
+    Quadrupole     src/elements/quad.jl:6-111
+    Sextupole      src/elements/multipole.jl:6-47
+    Octupole       src/elements/multipole.jl:61-102
+    ThinMultipole  src/elements/multipole.jl:116-155
+    SBend, RBend   src/elements/bend.jl:6-74, 243
+    ExactSBend     src/elements/bend.jl:88-149
+    ERBend         src/elements/SCelements.jl:339
+    RFCavity       src/elements/rf.jl:6-21
+
+Two deliberate extensions, both absent from the reference:
+
+* polynomials may hold up to ``MAXORD + 1 = 11`` coefficients (the reference stores
+  ``SVector{4}``, i.e. ``max_order <= 3``, src/elements/quad.jl:10-11); the kick is the
+  same Horner recurrence (src/tracking.jl:290-298) run to ``max_order``;
+* :class:`ExactSBendLinearEdge`, the exact sector-bend body of ``ExactSBend``
+  (src/tracking.jl:858-894, 923-953) entered and left through the linear edge maps of
+  ``SBend`` (src/tracking.jl:160-221) instead of Forest's fringe/wedge maps.
+
+Elements are plain mutable Python objects; nothing here touches a device.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+MAXORD = 10
+
+__all__ = [
+    "MAXORD", "AbstractElement", "AbstractMagnet", "AbstractDrift", "AbstractCavity",
+    "Drift", "Marker", "Quadrupole", "Sextupole", "Octupole", "ThinMultipole",
+    "SBend", "RBend", "ExactSBend", "ERBend", "ExactSBendLinearEdge", "RFCavity",
+    "TimeVaryingElement", "C_LIGHT", "get_length",
+]
+
+# src/elements/rf.jl:349
+C_LIGHT = 2.99792458e8
+
+
+class AbstractElement:
+    """Root of the element hierarchy (src/elements/core.jl:1)."""
+
+    name: str = "ELEMENT"
+
+    def __repr__(self) -> str:
+        fields = ", ".join(f"{k}={v!r}" for k, v in vars(self).items()
+                           if not isinstance(v, np.ndarray))
+        return f"{type(self).__name__}({fields})"
+
+
+class AbstractMagnet(AbstractElement):
+    """src/elements/core.jl:2"""
+
+
+class AbstractDrift(AbstractElement):
+    """src/elements/core.jl:3"""
+
+
+class AbstractCavity(AbstractElement):
+    """src/elements/core.jl:4"""
+
+
+def _vec(v, n: int, what: str) -> np.ndarray:
+    """_default_vec (src/elements/core.jl:19): zeros when not given."""
+    if v is None:
+        return np.zeros(n, dtype=np.float64)
+    a = np.array(v, dtype=np.float64).reshape(-1)
+    if a.shape[0] != n:
+        raise ValueError(f"{what} must have {n} entries, got {a.shape[0]}")
+    return a
+
+
+def _poly(v, what: str) -> np.ndarray:
+    """Polynomial coefficients: the reference's SVector{4} default, extendable to 11."""
+    if v is None:
+        return np.zeros(4, dtype=np.float64)
+    a = np.array(v, dtype=np.float64).reshape(-1)
+    if a.shape[0] < 1 or a.shape[0] > MAXORD + 1:
+        raise ValueError(f"{what} must have 1..{MAXORD + 1} coefficients, got {a.shape[0]}")
+    return a
+
+
+def _mat(v, what: str) -> np.ndarray:
+    """_default_mat (src/elements/core.jl:20): 6x6 zeros when not given."""
+    if v is None:
+        return np.zeros((6, 6), dtype=np.float64)
+    a = np.array(v, dtype=np.float64)
+    if a.shape != (6, 6):
+        raise ValueError(f"{what} must be 6x6, got {a.shape}")
+    return a
+
+
+class _Misaligned:
+    def _init_misalign(self, t1, t2, r1, r2, r_apertures, e_apertures):
+        self.t1 = _vec(t1, 6, "t1")
+        self.t2 = _vec(t2, 6, "t2")
+        self.r1 = _mat(r1, "r1")
+        self.r2 = _mat(r2, "r2")
+        # stored but never read by the tracker (src/elements/drift.jl:13-14; grep)
+        self.r_apertures = _vec(r_apertures, 6, "r_apertures")
+        self.e_apertures = _vec(e_apertures, 6, "e_apertures")
+
+
+class Drift(AbstractDrift, _Misaligned):
+    """Drift(L; name=:DRIFT, t1, t2, r1, r2, r_apertures, e_apertures) -- src/elements/drift.jl:37-50"""
+
+    def __init__(self, L, *, name="DRIFT", t1=None, t2=None, r1=None, r2=None,
+                 r_apertures=None, e_apertures=None):
+        self.name = str(name)
+        self.L = float(L)
+        self._init_misalign(t1, t2, r1, r2, r_apertures, e_apertures)
+
+
+class Marker(AbstractElement):
+    """Marker(; name=:MARKER) -- src/elements/marker.jl"""
+
+    def __init__(self, *, name="MARKER"):
+        self.name = str(name)
+
+
+class _ThickMultipole(AbstractMagnet, _Misaligned):
+    _default_max_order = 1
+
+    def _init_mp(self, L, polynom_a, polynom_b, max_order, num_int_steps, rad,
+                 fringe_entrance, fringe_exit, t1, t2, r1, r2, r_apertures, e_apertures,
+                 kick_angle):
+        self.L = float(L)
+        self.polynom_a = _poly(polynom_a, "polynom_a")
+        self.polynom_b = _poly(polynom_b, "polynom_b")
+        self.max_order = int(max_order)
+        self.num_int_steps = int(num_int_steps)
+        self.rad = int(rad)
+        self.fringe_entrance = int(fringe_entrance)
+        self.fringe_exit = int(fringe_exit)
+        self._init_misalign(t1, t2, r1, r2, r_apertures, e_apertures)
+        self.kick_angle = _vec(kick_angle, 2, "kick_angle")
+
+
+class Quadrupole(_ThickMultipole):
+    """Quadrupole(L, k1; max_order=1, num_int_steps=10, ...) -- src/elements/quad.jl:90-111"""
+
+    def __init__(self, L, k1, *, name="QUAD", polynom_a=None, polynom_b=None, max_order=1,
+                 num_int_steps=10, rad=0, fringe_entrance=0, fringe_exit=0, t1=None, t2=None,
+                 r1=None, r2=None, r_apertures=None, e_apertures=None, kick_angle=None):
+        self.name = str(name)
+        self.k1 = float(k1)
+        self._init_mp(L, polynom_a, polynom_b, max_order, num_int_steps, rad, fringe_entrance,
+                      fringe_exit, t1, t2, r1, r2, r_apertures, e_apertures, kick_angle)
+
+
+class Sextupole(_ThickMultipole):
+    """Sextupole(L, k2; max_order=2, num_int_steps=10, ...) -- src/elements/multipole.jl:26-47"""
+
+    def __init__(self, L, k2, *, name="SEXT", polynom_a=None, polynom_b=None, max_order=2,
+                 num_int_steps=10, rad=0, fringe_entrance=0, fringe_exit=0, t1=None, t2=None,
+                 r1=None, r2=None, r_apertures=None, e_apertures=None, kick_angle=None):
+        self.name = str(name)
+        self.k2 = float(k2)
+        self._init_mp(L, polynom_a, polynom_b, max_order, num_int_steps, rad, fringe_entrance,
+                      fringe_exit, t1, t2, r1, r2, r_apertures, e_apertures, kick_angle)
+
+
+class Octupole(_ThickMultipole):
+    """Octupole(L, k3; max_order=3, num_int_steps=10, ...) -- src/elements/multipole.jl:81-102"""
+
+    def __init__(self, L, k3, *, name="OCT", polynom_a=None, polynom_b=None, max_order=3,
+                 num_int_steps=10, rad=0, fringe_entrance=0, fringe_exit=0, t1=None, t2=None,
+                 r1=None, r2=None, r_apertures=None, e_apertures=None, kick_angle=None):
+        self.name = str(name)
+        self.k3 = float(k3)
+        self._init_mp(L, polynom_a, polynom_b, max_order, num_int_steps, rad, fringe_entrance,
+                      fringe_exit, t1, t2, r1, r2, r_apertures, e_apertures, kick_angle)
+
+
+class ThinMultipole(AbstractElement, _Misaligned):
+    """ThinMultipole(L, polynom_a, polynom_b; max_order=1, ...) -- src/elements/multipole.jl:135-155"""
+
+    def __init__(self, L, polynom_a, polynom_b, *, name="MULTIPOLE", max_order=1,
+                 num_int_steps=1, rad=0, fringe_entrance=0, fringe_exit=0, t1=None, t2=None,
+                 r1=None, r2=None, r_apertures=None, e_apertures=None, kick_angle=None):
+        self.name = str(name)
+        self.L = float(L)
+        self.polynom_a = _poly(polynom_a, "polynom_a")
+        self.polynom_b = _poly(polynom_b, "polynom_b")
+        self.max_order = int(max_order)
+        self.num_int_steps = int(num_int_steps)
+        self.rad = int(rad)
+        self.fringe_entrance = int(fringe_entrance)
+        self.fringe_exit = int(fringe_exit)
+        self._init_misalign(t1, t2, r1, r2, r_apertures, e_apertures)
+        self.kick_angle = _vec(kick_angle, 2, "kick_angle")
+
+
+class _Bend(AbstractMagnet, _Misaligned):
+    def _init_bend(self, L, angle, e1, e2, polynom_a, polynom_b, max_order, num_int_steps, rad,
+                   fint1, fint2, gap, fringe_bend_entrance, fringe_bend_exit,
+                   fringe_quad_entrance, fringe_quad_exit, fringe_int_m0, fringe_int_p0,
+                   t1, t2, r1, r2, r_apertures, e_apertures, kick_angle):
+        self.L = float(L)
+        self.angle = float(angle)
+        self.e1 = float(e1)
+        self.e2 = float(e2)
+        self.polynom_a = _poly(polynom_a, "polynom_a")
+        self.polynom_b = _poly(polynom_b, "polynom_b")
+        self.max_order = int(max_order)
+        self.num_int_steps = int(num_int_steps)
+        self.rad = int(rad)
+        self.fint1 = float(fint1)
+        self.fint2 = float(fint2)
+        self.gap = float(gap)
+        self.fringe_bend_entrance = int(fringe_bend_entrance)
+        self.fringe_bend_exit = int(fringe_bend_exit)
+        self.fringe_quad_entrance = int(fringe_quad_entrance)
+        self.fringe_quad_exit = int(fringe_quad_exit)
+        self.fringe_int_m0 = _vec(fringe_int_m0, 5, "fringe_int_m0")
+        self.fringe_int_p0 = _vec(fringe_int_p0, 5, "fringe_int_p0")
+        self._init_misalign(t1, t2, r1, r2, r_apertures, e_apertures)
+        self.kick_angle = _vec(kick_angle, 2, "kick_angle")
+
+
+class SBend(_Bend):
+    """SBend(L, angle, e1=0, e2=0; max_order=0, num_int_steps=10, fint1, fint2, gap,
+    fringe_bend_entrance=1, fringe_bend_exit=1, ...) -- src/elements/bend.jl:197-226"""
+
+    def __init__(self, L, angle, e1=0.0, e2=0.0, *, name="SBEND", polynom_a=None, polynom_b=None,
+                 max_order=0, num_int_steps=10, rad=0, fint1=0.0, fint2=0.0, gap=0.0,
+                 fringe_bend_entrance=1, fringe_bend_exit=1, fringe_quad_entrance=0,
+                 fringe_quad_exit=0, fringe_int_m0=None, fringe_int_p0=None, t1=None, t2=None,
+                 r1=None, r2=None, r_apertures=None, e_apertures=None, kick_angle=None):
+        self.name = str(name)
+        self._init_bend(L, angle, e1, e2, polynom_a, polynom_b, max_order, num_int_steps, rad,
+                        fint1, fint2, gap, fringe_bend_entrance, fringe_bend_exit,
+                        fringe_quad_entrance, fringe_quad_exit, fringe_int_m0, fringe_int_p0,
+                        t1, t2, r1, r2, r_apertures, e_apertures, kick_angle)
+
+
+def RBend(L, angle, **kwargs) -> SBend:
+    """Rectangular bend: SBend with e1 = e2 = angle/2 (src/elements/bend.jl:243)."""
+    return SBend(L, angle, angle / 2, angle / 2, **kwargs)
+
+
+class ExactSBend(_Bend):
+    """ExactSBend(L, angle, e1=0, e2=0; ..., gk=0.0) -- src/elements/bend.jl:280-311"""
+
+    def __init__(self, L, angle, e1=0.0, e2=0.0, *, name="ESBEND", polynom_a=None,
+                 polynom_b=None, max_order=0, num_int_steps=10, rad=0, fint1=0.0, fint2=0.0,
+                 gap=0.0, fringe_bend_entrance=1, fringe_bend_exit=1, fringe_quad_entrance=0,
+                 fringe_quad_exit=0, fringe_int_m0=None, fringe_int_p0=None, t1=None, t2=None,
+                 r1=None, r2=None, r_apertures=None, e_apertures=None, kick_angle=None, gk=0.0):
+        self.name = str(name)
+        self._init_bend(L, angle, e1, e2, polynom_a, polynom_b, max_order, num_int_steps, rad,
+                        fint1, fint2, gap, fringe_bend_entrance, fringe_bend_exit,
+                        fringe_quad_entrance, fringe_quad_exit, fringe_int_m0, fringe_int_p0,
+                        t1, t2, r1, r2, r_apertures, e_apertures, kick_angle)
+        self.gk = float(gk)
+
+
+def ERBend(L, angle, **kwargs) -> ExactSBend:
+    """Exact rectangular bend: ExactSBend with e1 = e2 = angle/2 (src/elements/SCelements.jl:339)."""
+    return ExactSBend(L, angle, angle / 2, angle / 2, **kwargs)
+
+
+class ExactSBendLinearEdge(_Bend):
+    """Exact sector-bend body with linear edge maps (no reference counterpart).
+
+    Tracking order: entrance misalignment; ``edge_fringe_entrance(irho, e1, fint1, gap,
+    fringe_bend_entrance)`` if ``fringe_bend_entrance != 0`` (src/tracking.jl:160-186);
+    the ExactSBend body (src/tracking.jl:923-953); ``edge_fringe_exit(irho, e2, fint2, gap,
+    fringe_bend_exit)`` (src/tracking.jl:194-221); exit misalignment.  Keywords are those of
+    :class:`SBend`; the fringe values select the linear edge method (1 Brown, 2 SOLEIL,
+    3 THOMX) exactly as for SBend.
+    """
+
+    def __init__(self, L, angle, e1=0.0, e2=0.0, *, name="ESBEND_LE", polynom_a=None,
+                 polynom_b=None, max_order=0, num_int_steps=10, rad=0, fint1=0.0, fint2=0.0,
+                 gap=0.0, fringe_bend_entrance=1, fringe_bend_exit=1, fringe_quad_entrance=0,
+                 fringe_quad_exit=0, fringe_int_m0=None, fringe_int_p0=None, t1=None, t2=None,
+                 r1=None, r2=None, r_apertures=None, e_apertures=None, kick_angle=None):
+        self.name = str(name)
+        self._init_bend(L, angle, e1, e2, polynom_a, polynom_b, max_order, num_int_steps, rad,
+                        fint1, fint2, gap, fringe_bend_entrance, fringe_bend_exit,
+                        fringe_quad_entrance, fringe_quad_exit, fringe_int_m0, fringe_int_p0,
+                        t1, t2, r1, r2, r_apertures, e_apertures, kick_angle)
+
+
+class RFCavity(AbstractCavity):
+    """RFCavity(L, volt, freq, lag=0.0; h=1.0, philag=0.0, energy=0.0) -- src/elements/rf.jl:338-342
+
+    As in the reference, ``h`` is stored but not used by the map (src/tracking.jl:510) and
+    the cavity gives no kick unless ``energy > 0`` (src/tracking.jl:508).
+    """
+
+    def __init__(self, L, volt, freq, lag=0.0, *, name="RFCA", h=1.0, philag=0.0, energy=0.0):
+        self.name = str(name)
+        self.L = float(L)
+        self.volt = float(volt)
+        self.freq = float(freq)
+        self.h = float(h)
+        self.lag = float(lag)
+        self.philag = float(philag)
+        self.energy = float(energy)
+
+
+class TimeVaryingElement(AbstractElement):
+    """Element with time- or turn-dependent parameters (src/time_dependence.jl:110-155).
+
+    Present so that lattices can carry one; the device path refuses it, as the reference's
+    GPU adaptation does (src/time_dependence.jl:137-139): materialise the lattice first.
+    """
+
+    def __init__(self, base: AbstractElement, params: dict):
+        self.base = base
+        self.params = dict(params)
+        self.name = getattr(base, "name", "TVE")
+
+    def materialize(self, time: float = 0.0, turn: int = 0) -> AbstractElement:
+        """Static copy with every callable parameter evaluated at (time, turn)."""
+        import copy
+
+        e = copy.deepcopy(self.base)
+        for k, f in self.params.items():
+            setattr(e, k, float(f(time, turn)) if callable(f) else f)
+        return e
+
+
+def get_length(elem: AbstractElement) -> float:
+    """get_length (src/lattice.jl:139-169)."""
+    if isinstance(elem, Marker):
+        return 0.0
+    if isinstance(elem, TimeVaryingElement):
+        return get_length(elem.base)
+    return float(elem.L)
