@@ -75,11 +75,13 @@ const char* tp_last_error(void);
 int tp_device_count(int* n);
 
 /* One device and one HIP stream per context. Replaces the implicit single-threaded CPU
- * "device" of src/lattice.jl:266-294. */
+ * "device" of src/lattice.jl:266-294.  tp_ctx_destroy may be called before the tables and
+ * particle sets created on the context are destroyed (finalizers run in any order): the
+ * context is released when the last of them goes; no new work is accepted after it. */
 int tp_ctx_create(int device, tp_ctx** out);
 int tp_ctx_destroy(tp_ctx* ctx);
 
-/* Launch shape for later tp_track calls: work-group size (64..1024, multiple of 64, 0 =
+/* Launch shape for later tp_track calls: work-group size (64, 128, 192 or 256; 0 =
  * default 256), particles per lane (1 or 2, 0 = default 1), turns per kernel launch
  * (0 = all turns in one launch). */
 int tp_set_launch(tp_ctx* ctx, int block, int ppl, int turns_per_launch);
@@ -93,6 +95,9 @@ int tp_timing(tp_ctx* ctx, tp_timing_t* out);
  * elements cannot be expressed in a table, the writer refuses them. */
 int tp_table_create(tp_ctx* ctx, const void* bytes, size_t nbytes, tp_table** out);
 int tp_table_destroy(tp_table* t);
+/* Validate a table blob without a device (the checks tp_table_create makes); on success
+ * *features receives the kernel feature mask the table needs. */
+int tp_table_check(const void* bytes, size_t nbytes, uint32_t* features);
 /* n_seq, n_rec and the table's beti. */
 int tp_table_info(const tp_table* t, int32_t* n_seq, int32_t* n_rec, double* beti);
 
@@ -108,6 +113,11 @@ int tp_particles_upload(tp_particles* p, const double* coords, const int64_t* lo
  * 0 for live ones. */
 int tp_particles_download(tp_particles* p, double* coords, int64_t* lost_flags01,
                           int32_t* lost_turn, int32_t* lost_elem);
+/* Device-to-device copy of a whole particle set (same n, same device), enqueued on
+ * dst's context stream, so it is ordered before later tp_track calls there; tp_sync waits
+ * for it.  Restores a saved state for checkpoint/resume (the reference resumes by re-passing
+ * coords/lost_flags/turn to ringpass!, src/lattice.jl:305-317). */
+int tp_particles_copy(tp_particles* dst, const tp_particles* src);
 /* Device pointers of the particle planes (for zero-copy interop); n is returned too. */
 int tp_particles_device_ptrs(tp_particles* p, double** coords, int32_t** lost_turn,
                              int32_t** lost_elem, int64_t* n);

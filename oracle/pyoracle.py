@@ -59,6 +59,7 @@ def lib():
         L.orc_linepass.argtypes = [PE, C.c_int, PD, C.c_int64, PI64, C.c_double, C.c_double, C.c_int, PI32, PI32, C.c_int32]
         L.orc_ringpass.argtypes = [PE, C.c_int, PD, C.c_int64, PI64, C.c_double, C.c_double, C.c_int, PI32, PI32,
                                    C.c_int32, C.c_int32, C.c_int]
+        L.orc_ringpass_ld.argtypes = L.orc_ringpass.argtypes
         L.orc_jacobian.argtypes = [PE, C.c_int, PD, C.c_int32, C.c_double, C.c_int, PD, PD]
         L.orc_loss_dir.argtypes = [PE, C.c_int, PD, C.c_int64, PD, PD, C.c_int32, C.c_double, C.c_double, C.c_int,
                                    PD, PD, PI64]
@@ -128,8 +129,9 @@ def _f64(a):
 
 
 def ringpass(coords: np.ndarray, lat, beam, lost_flags: np.ndarray, nturns: int, *, turn: int = 0,
-             aperture=None, exact: bool = True, nthreads: int = 1):
-    """Oracle ringpass! on host arrays; returns (coords, lost_flags, lost_turn, lost_elem) copies."""
+             aperture=None, exact: bool = True, nthreads: int = 1, extended: bool = False):
+    """Oracle ringpass! on host arrays; returns (coords, lost_flags, lost_turn, lost_elem) copies.
+    ``extended`` runs the same algorithm in long double (results rounded to double)."""
     c = np.asfortranarray(np.array(coords, dtype=np.float64))
     n = c.shape[0]
     lf = np.ascontiguousarray(np.array(lost_flags, dtype=np.int64))
@@ -137,7 +139,8 @@ def ringpass(coords: np.ndarray, lat, beam, lost_flags: np.ndarray, nturns: int,
     le = lt.copy()
     arr, ne = elems(lat)
     ap2 = 0.0 if aperture is None else float(aperture) ** 2
-    lib().orc_ringpass(arr, ne, _f64(c), n, lf.ctypes.data_as(C.POINTER(C.c_int64)), beti(beam), ap2, int(exact),
+    fn = lib().orc_ringpass_ld if extended else lib().orc_ringpass
+    fn(arr, ne, _f64(c), n, lf.ctypes.data_as(C.POINTER(C.c_int64)), beti(beam), ap2, int(exact),
                        lt.ctypes.data_as(C.POINTER(C.c_int32)), le.ctypes.data_as(C.POINTER(C.c_int32)),
                        int(turn), int(nturns), int(nthreads))
     return np.ascontiguousarray(c), lf, lt, le

@@ -87,6 +87,8 @@ template <int K> inline double val(const Dual<K>& a) { return a.v; }
 
 inline bool isnan_(double a) { return std::isnan(a); }
 inline double val(double a) { return a; }
+inline bool isnan_(long double a) { return std::isnan(a); }
+inline double val(long double a) { return (double)a; }
 using std::sqrt; using std::sin; using std::cos; using std::tan; using std::asin; using std::atan; using std::fabs;
 
 // ---------------------------------------------------------------------------------------
@@ -561,33 +563,67 @@ static inline bool lost_after(const R6<T>& r, double ap_r2) {
   return false;
 }
 
-static std::vector<Elem<double>> load_all(const orc_elem* elems, int n_elem) {
-  std::vector<Elem<double>> v(n_elem);
+template <class T>
+static std::vector<Elem<T>> load_all(const orc_elem* elems, int n_elem) {
+  std::vector<Elem<T>> v(n_elem);
   for (int i = 0; i < n_elem; ++i) load_elem(elems[i], v[i]);
   return v;
 }
 
 // linepass! over particles [i0, i1): src/lattice.jl:260-297 (element-outer, particle-inner).
-static void linepass_range(const std::vector<Elem<double>>& el, double* coords, int64_t n, int64_t i0, int64_t i1,
-                           int64_t* lost_flags, double beti, double ap_r2, bool exact, int32_t* lost_turn,
-                           int32_t* lost_elem, int32_t turn) {
+// T = double restates the reference; T = long double runs the same algorithm with a
+// 64-bit significand (the accuracy yardstick for implementations that round differently).
+template <class T>
+static void linepass_range(const std::vector<Elem<T>>& el, double* coords, int64_t n, int64_t i0, int64_t i1,
+                           int64_t* lost_flags, T beti, double ap_r2, bool exact, int32_t* lost_turn,
+                           int32_t* lost_elem, int32_t turn, T* wide) {
   const int ne = (int)el.size();
   for (int ie = 0; ie < ne; ++ie) {
-    const Elem<double>& e = el[ie];
+    const Elem<T>& e = el[ie];
     for (int64_t i = i0; i < i1; ++i) {
       if (lost_flags[i] == 1) continue;
-      R6<double> r;
-      for (int k = 0; k < 6; ++k) r.v[k] = coords[i + k * n];
-      R6<double> rn = pass_elem(e, r, beti, exact);
+      R6<T> r;
+      for (int k = 0; k < 6; ++k) r.v[k] = wide ? wide[i + k * n] : T(coords[i + k * n]);
+      R6<T> rn = pass_elem(e, r, beti, exact);
       if (lost_after(rn, ap_r2)) {
         lost_flags[i] = 1;
         if (lost_turn) lost_turn[i] = turn;
         if (lost_elem) lost_elem[i] = ie;
         continue;
       }
-      for (int k = 0; k < 6; ++k) coords[i + k * n] = rn.v[k];
+      for (int k = 0; k < 6; ++k) {
+        if (wide) wide[i + k * n] = rn.v[k];
+        coords[i + k * n] = (double)rn.v[k];
+      }
     }
   }
+}
+
+template <class T>
+static void ringpass_t(const orc_elem* elems, int n_elem, double* coords, int64_t n, int64_t* lost_flags,
+                       T beti, double ap_r2, int exact, int32_t* lost_turn, int32_t* lost_elem, int32_t turn0,
+                       int32_t nturns, int nthreads) {
+  std::vector<Elem<T>> el = load_all<T>(elems, n_elem);
+  std::vector<T> wide;
+  T* wp = nullptr;
+  if (sizeof(T) > sizeof(double)) {
+    wide.resize((size_t)(6 * n));
+    for (int64_t i = 0; i < 6 * n; ++i) wide[(size_t)i] = T(coords[i]);
+    wp = wide.data();
+  }
+  if (nthreads < 1) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+  {
+    int t = omp_get_thread_num(), nt = omp_get_num_threads();
+    int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
+    for (int32_t k = 0; k < nturns; ++k)
+      linepass_range<T>(el, coords, n, i0, i1, lost_flags, beti, ap_r2, exact != 0, lost_turn, lost_elem, turn0 + k, wp);
+  }
+#else
+  for (int32_t k = 0; k < nturns; ++k)
+    linepass_range<T>(el, coords, n, 0, n, lost_flags, beti, ap_r2, exact != 0, lost_turn, lost_elem, turn0 + k, wp);
+#endif
 }
 
 extern "C" {
@@ -615,8 +651,7 @@ int orc_pass(const orc_elem* e, double* r6, double beti, int exact) {
 // linepass!: src/lattice.jl:260-297.  coords is N x 6 column-major (6 planes of N).
 int orc_linepass(const orc_elem* elems, int n_elem, double* coords, int64_t n, int64_t* lost_flags,
                  double beti, double ap_r2, int exact, int32_t* lost_turn, int32_t* lost_elem, int32_t turn) {
-  std::vector<Elem<double>> el = load_all(elems, n_elem);
-  linepass_range(el, coords, n, 0, n, lost_flags, beti, ap_r2, exact != 0, lost_turn, lost_elem, turn);
+  ringpass_t<double>(elems, n_elem, coords, n, lost_flags, beti, ap_r2, exact, lost_turn, lost_elem, turn, 1, 1);
   return 0;
 }
 
@@ -626,20 +661,17 @@ int orc_linepass(const orc_elem* elems, int n_elem, double* coords, int64_t n, i
 int orc_ringpass(const orc_elem* elems, int n_elem, double* coords, int64_t n, int64_t* lost_flags,
                  double beti, double ap_r2, int exact, int32_t* lost_turn, int32_t* lost_elem,
                  int32_t turn0, int32_t nturns, int nthreads) {
-  std::vector<Elem<double>> el = load_all(elems, n_elem);
-  if (nthreads < 1) nthreads = 1;
-#ifdef _OPENMP
-#pragma omp parallel num_threads(nthreads)
-  {
-    int t = omp_get_thread_num(), nt = omp_get_num_threads();
-    int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
-    for (int32_t k = 0; k < nturns; ++k)
-      linepass_range(el, coords, n, i0, i1, lost_flags, beti, ap_r2, exact != 0, lost_turn, lost_elem, turn0 + k);
-  }
-#else
-  for (int32_t k = 0; k < nturns; ++k)
-    linepass_range(el, coords, n, 0, n, lost_flags, beti, ap_r2, exact != 0, lost_turn, lost_elem, turn0 + k);
-#endif
+  ringpass_t<double>(elems, n_elem, coords, n, lost_flags, beti, ap_r2, exact, lost_turn, lost_elem, turn0, nturns,
+                     nthreads);
+  return 0;
+}
+
+// The same ringpass! in long double (x87 extended precision), results rounded to double.
+int orc_ringpass_ld(const orc_elem* elems, int n_elem, double* coords, int64_t n, int64_t* lost_flags,
+                    double beti, double ap_r2, int exact, int32_t* lost_turn, int32_t* lost_elem,
+                    int32_t turn0, int32_t nturns, int nthreads) {
+  ringpass_t<long double>(elems, n_elem, coords, n, lost_flags, (long double)beti, ap_r2, exact, lost_turn,
+                          lost_elem, turn0, nturns, nthreads);
   return 0;
 }
 

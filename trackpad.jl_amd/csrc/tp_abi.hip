@@ -5,6 +5,7 @@
 // failure returns a TP_E* code and leaves a message in tp_last_error().
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -15,7 +16,12 @@
 #include "trackpad_hip.h"
 #include "tp_device.h"
 
+// A context is reference counted: every table and particle set created on it holds a
+// reference, so destroy calls may come in any order (Julia and Python finalizers run in
+// arbitrary order) and the stream outlives everything enqueued on it.
 struct tp_ctx {
+  std::atomic<int> refs{1};
+  std::atomic<bool> closed{false};
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -119,6 +125,16 @@ int set_dev(tp_ctx* ctx) {
   return TP_OK;
 }
 
+void ctx_release(tp_ctx* ctx) {
+  if (ctx->refs.fetch_sub(1) != 1) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipEventDestroy(ctx->ev0);
+  (void)hipEventDestroy(ctx->ev1);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
 }  // namespace
 
 extern "C" {
@@ -158,17 +174,13 @@ int tp_ctx_create(int device, tp_ctx** out) {
 
 int tp_ctx_destroy(tp_ctx* ctx) {
   if (!ctx) return TP_OK;
-  (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  (void)hipEventDestroy(ctx->ev0);
-  (void)hipEventDestroy(ctx->ev1);
-  (void)hipStreamDestroy(ctx->stream);
-  delete ctx;
+  ctx->closed = true;  // a second destroy of the same handle is undefined, as for free()
+  ctx_release(ctx);
   return TP_OK;
 }
 
 int tp_set_launch(tp_ctx* ctx, int block, int ppl, int turns_per_launch) {
-  if (!ctx) return fail(TP_EINVAL, "tp_set_launch: ctx is NULL");
+  if (!ctx || ctx->closed) return fail(TP_EINVAL, "tp_set_launch: ctx is NULL or destroyed");
   if (block == 0) block = 256;
   if (ppl == 0) ppl = 1;
   if (block < 64 || block > 256 || block % 64 != 0)
@@ -187,9 +199,14 @@ int tp_timing(tp_ctx* ctx, tp_timing_t* out) {
   return TP_OK;
 }
 
-int tp_table_create(tp_ctx* ctx, const void* bytes, size_t nbytes, tp_table** out) {
-  if (!ctx || !bytes || !out) return fail(TP_EINVAL, "tp_table_create: NULL argument");
-  *out = nullptr;
+}  // extern "C"
+
+namespace {
+
+// Parse and validate a table blob into t (host side); fills the kick specialisation and
+// the feature mask.  Returns TP_OK or a TP_E* code with the message set.
+int parse_table(const void* bytes, size_t nbytes, tp_table* t) {
+  if (!bytes) return fail(TP_EINVAL, "table: bytes is NULL");
   if (nbytes < sizeof(tp_table_header)) return fail(TP_ETABLE, "table: %zu bytes is shorter than the header", nbytes);
   tp_table_header h;
   std::memcpy(&h, bytes, sizeof h);
@@ -202,10 +219,6 @@ int tp_table_create(tp_ctx* ctx, const void* bytes, size_t nbytes, tp_table** ou
   if (!(std::isfinite(h.beti) && h.beti >= 1.0)) return fail(TP_ETABLE, "table: beti %g is not >= 1", h.beti);
   if (!(h.coord_limit > 0.0) || !(h.angle_limit > 0.0) || !(h.aperture_r2 >= 0.0))
     return fail(TP_ETABLE, "table: bad loss limits");
-
-  tp_table* t = new (std::nothrow) tp_table();
-  if (!t) return fail(TP_ENOMEM, "tp_table_create: out of host memory");
-  t->ctx = ctx;
   t->hdr = h;
   const char* b = (const char*)bytes;
   t->seq.resize(h.n_seq);
@@ -214,30 +227,28 @@ int tp_table_create(tp_ctx* ctx, const void* bytes, size_t nbytes, tp_table** ou
   std::memcpy(t->recs.data(), b + sizeof h + seq_bytes, (size_t)h.n_rec * sizeof(tp_rec));
   t->pool.resize(h.pool_len);
   std::memcpy(t->pool.data(), b + sizeof h + seq_bytes + (size_t)h.n_rec * sizeof(tp_rec), (size_t)h.pool_len * 8);
-
-  auto bad = [&](int code, const std::string& msg) {
-    delete t;
-    return fail(code, "%s", msg.c_str());
-  };
   for (uint32_t i = 0; i < h.n_seq; ++i)
     if (t->seq[i] < 0 || (uint32_t)t->seq[i] >= h.n_rec)
-      return bad(TP_ETABLE, "table: seq[" + std::to_string(i) + "] = " + std::to_string(t->seq[i]) + " out of range");
+      return fail(TP_ETABLE, "table: seq[%u] = %d out of range", i, t->seq[i]);
   unsigned feat = 0;
   if (!(h.flags & TP_H_EXACT_HAMILTONIAN)) feat |= FT_LINH;
   for (uint32_t i = 0; i < h.n_rec; ++i) {
     tp_rec& r = t->recs[i];
-    const std::string where = "table: record " + std::to_string(i) + ": ";
-    if (r.tag < 0 || r.tag >= TP_NTAGS) return bad(TP_EUNSUPPORTED, where + "unknown tag " + std::to_string(r.tag));
+    if (r.tag < 0 || r.tag >= TP_NTAGS) return fail(TP_EUNSUPPORTED, "table: record %u: unknown tag %d", i, r.tag);
     if (r.max_order < 0 || r.max_order > TP_MAXORD)
-      return bad(TP_EUNSUPPORTED, where + "max_order " + std::to_string(r.max_order) + " outside [0, 10]");
-    if (r.nsteps < 0 || r.nsteps > 1000000) return bad(TP_ETABLE, where + "nsteps " + std::to_string(r.nsteps));
+      return fail(TP_EUNSUPPORTED, "table: record %u: max_order %d outside [0, 10]", i, r.max_order);
+    if (r.nsteps < 0 || r.nsteps > 1000000) return fail(TP_ETABLE, "table: record %u: nsteps %d", i, r.nsteps);
     const int64_t len = block_len(r.tag, r.max_order);
-    if (r.off < 0 || (int64_t)r.off + len > (int64_t)h.pool_len) return bad(TP_ETABLE, where + "parameter block outside the pool");
+    if (r.off < 0 || (int64_t)r.off + len > (int64_t)h.pool_len)
+      return fail(TP_ETABLE, "table: record %u: parameter block outside the pool", i);
     if (r.flags & TP_F_MISALIGN) {
-      if (r.tag == TP_MARKER || r.tag == TP_RFCAV) return bad(TP_ETABLE, where + "misalignment on an element without one");
-      if (r.moff < 0 || (int64_t)r.moff + 84 > (int64_t)h.pool_len) return bad(TP_ETABLE, where + "misalignment block outside the pool");
+      if (r.tag == TP_MARKER || r.tag == TP_RFCAV)
+        return fail(TP_ETABLE, "table: record %u: misalignment on an element without one", i);
+      if (r.moff < 0 || (int64_t)r.moff + 84 > (int64_t)h.pool_len)
+        return fail(TP_ETABLE, "table: record %u: misalignment block outside the pool", i);
       feat |= FT_MISALIGN;
     }
+    if (r.gorder < -1 || r.gorder > r.max_order) return fail(TP_ETABLE, "table: record %u: gorder %d", i, r.gorder);
     r.kick = TP_KICK_NONE;
     const int64_t po = poly_off(r.tag);
     if (po >= 0) {
@@ -254,8 +265,37 @@ int tp_table_create(tp_ctx* ctx, const void* bytes, size_t nbytes, tp_table** ou
   else if ((feat & ~TP_FEAT_B) == 0) t->variant = TP_VARIANT_B;
   else if ((feat & ~TP_FEAT_C) == 0) t->variant = TP_VARIANT_C;
   else t->variant = TP_VARIANT_ALL;
+  return TP_OK;
+}
 
+}  // namespace
+
+extern "C" {
+
+int tp_table_check(const void* bytes, size_t nbytes, uint32_t* features) {
+  tp_table t;
+  int rc = parse_table(bytes, nbytes, &t);
+  if (rc) return rc;
+  if (features) *features = t.features;
+  return TP_OK;
+}
+
+int tp_table_create(tp_ctx* ctx, const void* bytes, size_t nbytes, tp_table** out) {
+  if (!ctx || !bytes || !out) return fail(TP_EINVAL, "tp_table_create: NULL argument");
+  if (ctx->closed) return fail(TP_EINVAL, "tp_table_create: context destroyed");
+  *out = nullptr;
+  tp_table* t = new (std::nothrow) tp_table();
+  if (!t) return fail(TP_ENOMEM, "tp_table_create: out of host memory");
+  int rc = parse_table(bytes, nbytes, t);
+  if (rc) {
+    delete t;
+    return rc;
+  }
+  t->ctx = ctx;
+  const tp_table_header& h = t->hdr;
+  const size_t seq_bytes = ((size_t)h.n_seq * 4 + 7) / 8 * 8;
   // Device copy: the same layout with the library's kick field written.
+  const char* b = (const char*)bytes;
   std::vector<char> blob(b, b + nbytes);
   std::memcpy(blob.data() + sizeof h + seq_bytes, t->recs.data(), (size_t)h.n_rec * sizeof(tp_rec));
   hipError_t e = hipSetDevice(ctx->device);
@@ -268,6 +308,7 @@ int tp_table_create(tp_ctx* ctx, const void* bytes, size_t nbytes, tp_table** ou
     return fail(e == hipErrorOutOfMemory ? TP_ENOMEM : TP_EHIP, "tp_table_create: %s", hipGetErrorName(e));
   }
   t->nbytes = nbytes;
+  ctx->refs.fetch_add(1);
   const char* d = (const char*)t->dev;
   t->d_seq = (const int32_t*)(d + sizeof h);
   t->d_recs = (const tp_rec*)(d + sizeof h + seq_bytes);
@@ -278,10 +319,12 @@ int tp_table_create(tp_ctx* ctx, const void* bytes, size_t nbytes, tp_table** ou
 
 int tp_table_destroy(tp_table* t) {
   if (!t) return TP_OK;
-  (void)hipSetDevice(t->ctx->device);
-  (void)hipStreamSynchronize(t->ctx->stream);
+  tp_ctx* ctx = t->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   if (t->dev) (void)hipFree(t->dev);
   delete t;
+  ctx_release(ctx);
   return TP_OK;
 }
 
@@ -295,6 +338,7 @@ int tp_table_info(const tp_table* t, int32_t* n_seq, int32_t* n_rec, double* bet
 
 int tp_particles_create(tp_ctx* ctx, int64_t n, tp_particles** out) {
   if (!ctx || !out) return fail(TP_EINVAL, "tp_particles_create: NULL argument");
+  if (ctx->closed) return fail(TP_EINVAL, "tp_particles_create: context destroyed");
   *out = nullptr;
   if (n < 0 || n > ((int64_t)1 << 40)) return fail(TP_EINVAL, "tp_particles_create: n = %lld", (long long)n);
   int rc = set_dev(ctx);
@@ -314,18 +358,21 @@ int tp_particles_create(tp_ctx* ctx, int64_t n, tp_particles** out) {
     return fail(e == hipErrorOutOfMemory ? TP_ENOMEM : TP_EHIP, "tp_particles_create(%lld): %s", (long long)n,
                 hipGetErrorName(e));
   }
+  ctx->refs.fetch_add(1);
   *out = p;
   return TP_OK;
 }
 
 int tp_particles_destroy(tp_particles* p) {
   if (!p) return TP_OK;
-  (void)hipSetDevice(p->ctx->device);
-  (void)hipStreamSynchronize(p->ctx->stream);
+  tp_ctx* ctx = p->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   (void)hipFree(p->d_coords);
   (void)hipFree(p->d_lt);
   (void)hipFree(p->d_le);
   delete p;
+  ctx_release(ctx);
   return TP_OK;
 }
 
@@ -380,6 +427,22 @@ int tp_particles_download(tp_particles* p, double* coords, int64_t* lost_flags01
   return TP_OK;
 }
 
+int tp_particles_copy(tp_particles* dst, const tp_particles* src) {
+  if (!dst || !src) return fail(TP_EINVAL, "tp_particles_copy: NULL argument");
+  if (dst->n != src->n) return fail(TP_EINVAL, "tp_particles_copy: sizes %lld and %lld differ", (long long)dst->n,
+                                    (long long)src->n);
+  if (dst->ctx->device != src->ctx->device) return fail(TP_EINVAL, "tp_particles_copy: different devices");
+  int rc = set_dev(dst->ctx);
+  if (rc) return rc;
+  const size_t n = (size_t)dst->n;
+  if (n == 0) return TP_OK;
+  hipStream_t s = dst->ctx->stream;
+  TP_HIP(hipMemcpyAsync(dst->d_coords, src->d_coords, 6 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  TP_HIP(hipMemcpyAsync(dst->d_lt, src->d_lt, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  TP_HIP(hipMemcpyAsync(dst->d_le, src->d_le, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+  return TP_OK;
+}
+
 int tp_particles_device_ptrs(tp_particles* p, double** coords, int32_t** lost_turn, int32_t** lost_elem,
                              int64_t* n) {
   if (!p) return fail(TP_EINVAL, "tp_particles_device_ptrs: particles is NULL");
@@ -393,6 +456,7 @@ int tp_particles_device_ptrs(tp_particles* p, double** coords, int32_t** lost_tu
 int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, uint32_t flags) {
   if (!ctx || !t || !p) return fail(TP_EINVAL, "tp_track: NULL argument");
   if (t->ctx != ctx || p->ctx != ctx) return fail(TP_EINVAL, "tp_track: table and particles belong to another context");
+  if (ctx->closed) return fail(TP_EINVAL, "tp_track: context destroyed");
   if (nturns < 0) return fail(TP_EINVAL, "tp_track: nturns = %d", nturns);
   if (flags & ~(TP_TRACK_FAST | TP_TRACK_ASYNC | TP_TRACK_TIMED)) return fail(TP_EINVAL, "tp_track: unknown flags 0x%x", flags);
   if ((int64_t)turn0 + (int64_t)nturns > 0x7fffffffLL) return fail(TP_EINVAL, "tp_track: turn range overflows int32");

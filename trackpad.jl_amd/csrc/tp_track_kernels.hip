@@ -18,12 +18,14 @@
 
 #if TP_FAST
 #define TP_LAUNCH tp_launch_track_fast
+#define TPK_BUILD fast
 #else
 #define TP_LAUNCH tp_launch_track_exact
+#define TPK_BUILD exact
 #endif
 
 namespace tpk {
-namespace {
+namespace TPK_BUILD {  // kernel names carry the build flavour (profiles tell them apart)
 
 // Thick multipole: Yoshida-4 with strthinkick, src/tracking.jl:317-341 / :386-491
 template <bool EXH, int KK, int PPL>
@@ -404,7 +406,7 @@ __global__ __launch_bounds__(256) void track_kernel(const int32_t* __restrict__ 
   }
 }
 
-}  // namespace
+}  // namespace TPK_BUILD
 }  // namespace tpk
 
 #define TP_KPTRS(a) a.seq, a.recs, a.pool, a.coords, a.lost_turn, a.lost_elem
@@ -412,9 +414,9 @@ __global__ __launch_bounds__(256) void track_kernel(const int32_t* __restrict__ 
 template <bool EXH, unsigned FEAT>
 static void launch_v(const TpKArgs& a, int ppl, unsigned grid, int block, hipStream_t stream) {
   if (ppl == 2)
-    hipLaunchKernelGGL((tpk::track_kernel<EXH, FEAT, 2>), dim3(grid), dim3(block), 0, stream, TP_KPTRS(a), a);
+    hipLaunchKernelGGL((tpk::TPK_BUILD::track_kernel<EXH, FEAT, 2>), dim3(grid), dim3(block), 0, stream, TP_KPTRS(a), a);
   else
-    hipLaunchKernelGGL((tpk::track_kernel<EXH, FEAT, 1>), dim3(grid), dim3(block), 0, stream, TP_KPTRS(a), a);
+    hipLaunchKernelGGL((tpk::TPK_BUILD::track_kernel<EXH, FEAT, 1>), dim3(grid), dim3(block), 0, stream, TP_KPTRS(a), a);
 }
 
 hipError_t TP_LAUNCH(const TpKArgs& a, int block, int ppl, hipStream_t stream) {

@@ -56,6 +56,8 @@ _SIGS = {
     "tp_timing": ([_P, C.POINTER(TimingT)], C.c_int),
     "tp_table_create": ([_P, C.c_void_p, C.c_size_t, C.POINTER(_P)], C.c_int),
     "tp_table_destroy": ([_P], C.c_int),
+    "tp_table_check": ([C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32)], C.c_int),
+    "tp_particles_copy": ([_P, _P], C.c_int),
     "tp_table_info": ([_P, _PI32, _PI32, _PD], C.c_int),
     "tp_particles_create": ([_P, C.c_int64, C.POINTER(_P)], C.c_int),
     "tp_particles_destroy": ([_P], C.c_int),

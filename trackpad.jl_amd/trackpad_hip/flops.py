@@ -1,0 +1,68 @@
+"""Algorithmic FP64 operation counts of the reference maps (the roofline's numerator).
+
+Counting rule: every IEEE + - * / sqrt applied to a particle-dependent value as written in
+the reference is one flop; subexpressions of element or beam constants are folded (the
+reference recomputes them per call, the device table holds them); transcendental calls
+(sin, tan, asin, atan, cos) are not flops.  Per-primitive counts, with the reference lines
+they follow:
+
+    drift6                      19            src/tracking.jl:87-99
+    strthinkick (order m)       8m + 4, m = 0: 2     :284-305
+    bndthinkick (order m)       8m + 11, m = 0: 10   :243-270
+    exact_bend_body             39            :858-894
+    linear edge (one side)      8             :160-221
+    RF kick                     7             :508-513
+    yrot                        23            :727-745
+    bend_edge                   40            :822-851
+    bend_fringe                 128           :752-815
+
+so one Yoshida-4 step of a thick multipole is 4 x 19 + 3 x (8m + 4).  With these counts
+the 20-cell DBA ring of the benchmark is 313,587 flops per particle-turn and the FODO
+workload 7,472.
+"""
+
+from __future__ import annotations
+
+from .elements import (Drift, ExactSBend, ExactSBendLinearEdge, Marker, Octupole, Quadrupole, RFCavity, SBend,
+                       Sextupole, ThinMultipole)
+from .lattice import Lattice
+
+__all__ = ["elem_flops", "flops_per_turn"]
+
+DRIFT, EBODY, EDGE, RF, YROT, BEDGE, BFRINGE = 19, 39, 8, 7, 23, 40, 128
+
+
+def _kick(m: int) -> int:
+    return 8 * m + 4 if m >= 1 else 2
+
+
+def _bkick(m: int) -> int:
+    return 8 * m + 11 if m >= 1 else 10
+
+
+def elem_flops(e) -> int:
+    if isinstance(e, Marker):
+        return 0
+    if isinstance(e, Drift):
+        return DRIFT
+    if isinstance(e, (Quadrupole, Sextupole, Octupole)):
+        return e.num_int_steps * (4 * DRIFT + 3 * _kick(e.max_order))
+    if isinstance(e, ThinMultipole):
+        return _kick(e.max_order)
+    if isinstance(e, SBend):
+        edges = EDGE * ((e.fringe_bend_entrance != 0) + (e.fringe_bend_exit != 0))
+        return edges + e.num_int_steps * (4 * DRIFT + 3 * _bkick(e.max_order))
+    if isinstance(e, (ExactSBend, ExactSBendLinearEdge)):
+        body = EBODY if e.num_int_steps == 0 else e.num_int_steps * (4 * EBODY + 3 * _kick(e.max_order))
+        if isinstance(e, ExactSBendLinearEdge):
+            return body + EDGE * ((e.fringe_bend_entrance != 0) + (e.fringe_bend_exit != 0))
+        extra = 2 * BEDGE + BFRINGE * ((e.fringe_bend_entrance != 0) + (e.fringe_bend_exit != 0))
+        extra += YROT * ((e.e1 != 0.0) + (e.e2 != 0.0))
+        return body + extra
+    if isinstance(e, RFCavity):
+        return (2 * DRIFT if e.L > 0 else 0) + (RF if e.energy > 0 else 0)
+    raise NotImplementedError(type(e).__name__)
+
+
+def flops_per_turn(lat: Lattice) -> int:
+    return sum(elem_flops(e) for e in lat.elements)

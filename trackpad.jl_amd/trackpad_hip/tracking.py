@@ -20,6 +20,7 @@ Every call runs on the GPU through libtrackpad_hip; there is no CPU path.
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import hashlib
 import threading
@@ -40,6 +41,21 @@ _PI64 = C.POINTER(C.c_int64)
 _PI32 = C.POINTER(C.c_int32)
 
 
+_SHUTTING_DOWN = [False]
+
+
+@atexit.register
+def _mark_shutdown() -> None:
+    # Past this point the process is ending: device memory goes with it, and no HIP call
+    # is made from objects collected during interpreter teardown.
+    _SHUTTING_DOWN[0] = True
+
+
+def _release(fn, handle) -> None:
+    if not _SHUTTING_DOWN[0]:
+        fn(handle)
+
+
 def _ptr(a: Optional[np.ndarray], t):
     return None if a is None else a.ctypes.data_as(t)
 
@@ -54,7 +70,8 @@ class HIPContext:
         self.handle = h
         self.device = int(device)
         self._tables: Dict[bytes, "DeviceTable"] = {}
-        self._fin = weakref.finalize(self, self._lib.tp_ctx_destroy, h)
+        self._fin = weakref.finalize(self, _release, self._lib.tp_ctx_destroy, h)
+        self._fin.atexit = False
 
     def set_launch(self, block: int = 0, ppl: int = 0, turns_per_launch: int = 0) -> None:
         L.check(self._lib.tp_set_launch(self.handle, int(block), int(ppl), int(turns_per_launch)))
@@ -111,7 +128,8 @@ class DeviceTable:
         ns, nr, bt = C.c_int32(), C.c_int32(), C.c_double()
         L.check(ctx._lib.tp_table_info(h, C.byref(ns), C.byref(nr), C.byref(bt)))
         self.n_seq, self.n_rec, self.beti = ns.value, nr.value, bt.value
-        self._fin = weakref.finalize(self, ctx._lib.tp_table_destroy, h)
+        self._fin = weakref.finalize(self, _release, ctx._lib.tp_table_destroy, h)
+        self._fin.atexit = False
 
 
 def device_table(lat: Lattice, beam: Beam, ctx: Optional[HIPContext] = None, **table_kw) -> DeviceTable:
@@ -128,7 +146,8 @@ class HIPParticles:
         L.check(self.ctx._lib.tp_particles_create(self.ctx.handle, int(n), C.byref(h)))
         self.handle = h
         self.n = int(n)
-        self._fin = weakref.finalize(self, self.ctx._lib.tp_particles_destroy, h)
+        self._fin = weakref.finalize(self, _release, self.ctx._lib.tp_particles_destroy, h)
+        self._fin.atexit = False
 
     @classmethod
     def from_coords(cls, coords: np.ndarray, lost_flags: Optional[np.ndarray] = None,
@@ -165,6 +184,10 @@ class HIPParticles:
     @property
     def coords(self) -> np.ndarray:
         return self.download()[0]
+
+    def copy_from(self, other: "HIPParticles") -> None:
+        """Device-to-device copy of another particle set of the same size (enqueued)."""
+        L.check(self.ctx._lib.tp_particles_copy(self.handle, other.handle))
 
     def device_ptrs(self):
         pc, pt, pe, n = _PD(), _PI32(), _PI32(), C.c_int64()
