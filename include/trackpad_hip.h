@@ -62,6 +62,8 @@ typedef struct tp_timing_t {
   int32_t block;       /* work-group size used */
   int32_t ppl;         /* particles per lane used */
   int32_t turns_per_launch;
+  double forward_ms;   /* tp_adjoint: forward pass with checkpoints */
+  double backward_ms;  /* tp_adjoint: reverse pass and gradient reduction */
 } tp_timing_t;
 
 /* Library version (TP_ABI_VERSION). No reference counterpart. */
@@ -140,6 +142,24 @@ int tp_sync(tp_ctx* ctx);
 int tp_track_host(tp_ctx* ctx, const tp_table* t, double* coords, int64_t n, int64_t* lost_flags,
                   int32_t* lost_turn, int32_t* lost_elem, int32_t turn0, int32_t nturns,
                   uint32_t flags);
+
+/* Reverse-mode gradient of  L = mean over surviving particles of (x^2 + y^2)  after nturns
+ * turns (no reference counterpart: the reference's Enzyme extension is an empty stub,
+ * ext/TrackPadEnzymeExt.jl:14-17; its only AD use is forward-mode Enzyme on a single
+ * particle, test/verify_enzyme_compat.jl:13-63).  p holds the initial beam on entry and
+ * the tracked beam on return (as tp_track).  The forward pass stores the beam every
+ * ckpt_stride turns (0 = every turn); the reverse pass starts each segment from the
+ * stored state at its end and runs every sub-step's inverse map with its vector-Jacobian
+ * product.  Outputs (any may be NULL): *loss, *n_alive, dL_dk[n_seq] = dL/dk of each
+ * lattice position's main strength (k1 of a Quadrupole, k2 of a Sextupole, k3 of an
+ * Octupole; 0 for other positions), dL_dx0 (N x 6 column-major host array) = dL/d initial
+ * coordinates.  Particles lost during tracking do not enter L and get zero gradient.
+ * Supported: drifts, thick multipoles, thin multipoles, SBend, exact bends with linear
+ * edges, RF cavities, Marker, t1/t2 translations, exact Hamiltonian; other tables return
+ * TP_EUNSUPPORTED.  flags: TP_TRACK_FAST / TP_TRACK_TIMED as for tp_track. */
+int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns,
+               int32_t ckpt_stride, uint32_t flags, double* loss, int64_t* n_alive, double* dL_dk,
+               double* dL_dx0);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,7 @@ def lib():
         L.orc_ringpass.argtypes = [PE, C.c_int, PD, C.c_int64, PI64, C.c_double, C.c_double, C.c_int, PI32, PI32,
                                    C.c_int32, C.c_int32, C.c_int]
         L.orc_ringpass_ld.argtypes = L.orc_ringpass.argtypes
+        L.orc_ringpass_q.argtypes = L.orc_ringpass.argtypes + [PD, PD]
         L.orc_jacobian.argtypes = [PE, C.c_int, PD, C.c_int32, C.c_double, C.c_int, PD, PD]
         L.orc_loss_dir.argtypes = [PE, C.c_int, PD, C.c_int64, PD, PD, C.c_int32, C.c_double, C.c_double, C.c_int,
                                    PD, PD, PI64]
@@ -144,6 +145,23 @@ def ringpass(coords: np.ndarray, lat, beam, lost_flags: np.ndarray, nturns: int,
                        lt.ctypes.data_as(C.POINTER(C.c_int32)), le.ctypes.data_as(C.POINTER(C.c_int32)),
                        int(turn), int(nturns), int(nthreads))
     return np.ascontiguousarray(c), lf, lt, le
+
+
+def loss_quad(coords, lat, beam, nturns: int, *, aperture=None, nthreads: int = 1):
+    """ringpass! in quad precision: (sum over survivors of x^2 + y^2 as a double pair
+    (hi, lo), number of survivors)."""
+    c = np.asfortranarray(np.array(coords, dtype=np.float64))
+    n = c.shape[0]
+    lf = np.zeros(n, dtype=np.int64)
+    lt = np.full(n, -1, dtype=np.int32)
+    le = lt.copy()
+    arr, ne = elems(lat)
+    ap2 = 0.0 if aperture is None else float(aperture) ** 2
+    hi, lo = C.c_double(), C.c_double()
+    lib().orc_ringpass_q(arr, ne, _f64(c), n, lf.ctypes.data_as(C.POINTER(C.c_int64)), beti(beam), ap2, 1,
+                         lt.ctypes.data_as(C.POINTER(C.c_int32)), le.ctypes.data_as(C.POINTER(C.c_int32)), 0,
+                         int(nturns), int(nthreads), C.byref(hi), C.byref(lo))
+    return (hi.value, lo.value), int(n - lf.sum())
 
 
 def linepass(coords, lat, beam, lost_flags, *, turn: int = 0, **kw):

@@ -33,6 +33,8 @@
 //   * an optional circular aperture x^2+y^2 > R^2, checked together with check_lost at
 //     element exit (the reference stores r_apertures but never reads them).
 
+#include <quadmath.h>
+
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -90,6 +92,19 @@ inline double val(double a) { return a; }
 inline bool isnan_(long double a) { return std::isnan(a); }
 inline double val(long double a) { return (double)a; }
 using std::sqrt; using std::sin; using std::cos; using std::tan; using std::asin; using std::atan; using std::fabs;
+
+// Quad precision (__float128, libquadmath): the yardstick for finite differences, whose
+// rounding noise must stay far below the differences being resolved.
+typedef __float128 f128;
+inline f128 sqrt(f128 a) { return sqrtq(a); }
+inline f128 sin(f128 a) { return sinq(a); }
+inline f128 cos(f128 a) { return cosq(a); }
+inline f128 tan(f128 a) { return tanq(a); }
+inline f128 asin(f128 a) { return asinq(a); }
+inline f128 atan(f128 a) { return atanq(a); }
+inline f128 fabs(f128 a) { return fabsq(a); }
+inline bool isnan_(f128 a) { return isnanq(a) != 0; }
+inline double val(f128 a) { return (double)a; }
 
 // ---------------------------------------------------------------------------------------
 // Raw element description: the fields of the reference's element structs
@@ -663,6 +678,39 @@ int orc_ringpass(const orc_elem* elems, int n_elem, double* coords, int64_t n, i
                  int32_t turn0, int32_t nturns, int nthreads) {
   ringpass_t<double>(elems, n_elem, coords, n, lost_flags, beti, ap_r2, exact, lost_turn, lost_elem, turn0, nturns,
                      nthreads);
+  return 0;
+}
+
+// The same ringpass! in quad precision; results rounded to double.  loss_out (may be NULL)
+// receives sum over survivors of (x^2 + y^2), accumulated in quad precision.
+int orc_ringpass_q(const orc_elem* elems, int n_elem, double* coords, int64_t n, int64_t* lost_flags,
+                   double beti, double ap_r2, int exact, int32_t* lost_turn, int32_t* lost_elem,
+                   int32_t turn0, int32_t nturns, int nthreads, double* loss_hi, double* loss_lo) {
+  std::vector<Elem<f128>> el = load_all<f128>(elems, n_elem);
+  std::vector<f128> wide((size_t)(6 * n));
+  for (int64_t i = 0; i < 6 * n; ++i) wide[(size_t)i] = f128(coords[i]);
+  if (nthreads < 1) nthreads = 1;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+  {
+    int t = omp_get_thread_num(), nt = omp_get_num_threads();
+    int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
+    for (int32_t k = 0; k < nturns; ++k)
+      linepass_range<f128>(el, coords, n, i0, i1, lost_flags, (f128)beti, ap_r2, exact != 0, lost_turn, lost_elem,
+                           turn0 + k, wide.data());
+  }
+#else
+  for (int32_t k = 0; k < nturns; ++k)
+    linepass_range<f128>(el, coords, n, 0, n, lost_flags, (f128)beti, ap_r2, exact != 0, lost_turn, lost_elem,
+                         turn0 + k, wide.data());
+#endif
+  f128 acc = 0;
+  for (int64_t i = 0; i < n; ++i)
+    if (lost_flags[i] != 1) acc += wide[(size_t)i] * wide[(size_t)i] + wide[(size_t)(2 * n + i)] * wide[(size_t)(2 * n + i)];
+  // the quad sum split into a double pair hi + lo
+  const double hi = (double)acc;
+  if (loss_hi) *loss_hi = hi;
+  if (loss_lo) *loss_lo = (double)(acc - (f128)hi);
   return 0;
 }
 

@@ -44,6 +44,8 @@ struct tp_table {
   std::vector<tp_rec> recs;      // host copy with the kick specialisation filled in
   std::vector<int32_t> seq;      // host copy
   std::vector<double> pool;      // host copy
+  int32_t* d_slots = nullptr;    // gradient slot of each position (adjoint), built on first use
+  int32_t n_slots = 0;
 };
 
 struct tp_particles {
@@ -323,6 +325,7 @@ int tp_table_destroy(tp_table* t) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (t->dev) (void)hipFree(t->dev);
+  if (t->d_slots) (void)hipFree(t->d_slots);
   delete t;
   ctx_release(ctx);
   return TP_OK;
@@ -538,6 +541,152 @@ int tp_track_host(tp_ctx* ctx, const tp_table* t, double* coords, int64_t n, int
     if (lost_turn) lost_turn[i] = lt[(size_t)i];
     if (lost_elem) lost_elem[i] = le[(size_t)i];
   }
+  return TP_OK;
+}
+
+namespace {
+
+// RAII device buffer for the adjoint's scratch arrays.
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, int32_t ckpt_stride,
+               uint32_t flags, double* loss, int64_t* n_alive, double* dL_dk, double* dL_dx0) {
+  if (!ctx || !t || !p) return fail(TP_EINVAL, "tp_adjoint: NULL argument");
+  if (t->ctx != ctx || p->ctx != ctx) return fail(TP_EINVAL, "tp_adjoint: table and particles belong to another context");
+  if (ctx->closed) return fail(TP_EINVAL, "tp_adjoint: context destroyed");
+  if (nturns < 1) return fail(TP_EINVAL, "tp_adjoint: nturns = %d", nturns);
+  if (ckpt_stride < 0) return fail(TP_EINVAL, "tp_adjoint: ckpt_stride = %d", ckpt_stride);
+  if (flags & ~(TP_TRACK_FAST | TP_TRACK_TIMED)) return fail(TP_EINVAL, "tp_adjoint: unsupported flags 0x%x", flags);
+  if (t->features & (FT_ESBEND | FT_LINH))
+    return fail(TP_EUNSUPPORTED, "tp_adjoint: Forest exact bends and the linearised drift have no adjoint here");
+  for (const tp_rec& r : t->recs)
+    if (r.flags & (TP_F_R1 | TP_F_R2)) return fail(TP_EUNSUPPORTED, "tp_adjoint: r1/r2 rotations have no adjoint here");
+  int rc = set_dev(ctx);
+  if (rc) return rc;
+  tp_table* tm = const_cast<tp_table*>(t);
+  if (!tm->d_slots) {
+    std::vector<int32_t> slots(t->hdr.n_seq > 0 ? t->hdr.n_seq : 1, -1);
+    int32_t ns = 0;
+    for (uint32_t e = 0; e < t->hdr.n_seq; ++e)
+      if (t->recs[t->seq[e]].gorder >= 0) slots[e] = ns++;
+    TP_HIP(hipMalloc(&tm->d_slots, slots.size() * sizeof(int32_t)));
+    TP_HIP(hipMemcpyAsync(tm->d_slots, slots.data(), slots.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    tm->n_slots = ns;
+  }
+  const int32_t n_slots = tm->n_slots;
+  if ((size_t)4 * n_slots * sizeof(double) > 64 * 1024)
+    return fail(TP_EUNSUPPORTED, "tp_adjoint: %d strength slots exceed the 64 KiB LDS budget (max 2048)", n_slots);
+  const int64_t n = p->n;
+  const int32_t K = ckpt_stride > 0 ? ckpt_stride : 1;
+  const int32_t nseg = (nturns + K - 1) / K;
+  const size_t plane = (size_t)(n > 0 ? n : 1);
+  const int nblocks = (int)((n + 255) / 256);
+  DevBuf ckpt, lam, partial, grad;
+  hipError_t e = hipMalloc(&ckpt.p, (size_t)nseg * 6 * plane * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&lam.p, 6 * plane * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&partial.p, (size_t)(nblocks > 0 ? nblocks : 1) * (n_slots > 0 ? n_slots : 1) * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&grad.p, (size_t)(n_slots > 0 ? n_slots : 1) * sizeof(double));
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? TP_ENOMEM : TP_EHIP, "tp_adjoint: checkpoint allocation (%d x %lld particles): %s",
+                nseg, (long long)n, hipGetErrorName(e));
+  const bool fast = (flags & TP_TRACK_FAST) != 0;
+  hipStream_t st = ctx->stream;
+  TP_HIP(hipEventRecord(ctx->ev0, st));
+  // forward pass, storing the beam at the end of every segment
+  for (int32_t j = 0; j < nseg; ++j) {
+    const int32_t kt = (j == nseg - 1) ? (nturns - j * K) : K;
+    rc = tp_track(ctx, t, p, turn0 + j * K, kt, (fast ? TP_TRACK_FAST : 0) | TP_TRACK_ASYNC);
+    if (rc) return rc;
+    if (n > 0) TP_HIP(hipMemcpyAsync((double*)ckpt.p + (size_t)j * 6 * plane, p->d_coords, 6 * plane * sizeof(double),
+                                     hipMemcpyDeviceToDevice, st));
+  }
+  TP_HIP(hipEventRecord(ctx->ev1, st));
+  TP_HIP(hipEventSynchronize(ctx->ev1));
+  float fwd_ms = 0.f;
+  TP_HIP(hipEventElapsedTime(&fwd_ms, ctx->ev0, ctx->ev1));
+  // survivors and loss (host side, from the final beam)
+  std::vector<int32_t> lt((size_t)plane);
+  std::vector<double> fin(6 * plane);
+  if (n > 0) {
+    TP_HIP(hipMemcpyAsync(lt.data(), p->d_lt, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    TP_HIP(hipMemcpyAsync(fin.data(), p->d_coords, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+    TP_HIP(hipStreamSynchronize(st));
+  }
+  int64_t na = 0;
+  double lsum = 0.0;
+  for (int64_t i = 0; i < n; ++i)
+    if (lt[(size_t)i] == -1) {
+      ++na;
+      lsum += fin[(size_t)i] * fin[(size_t)i] + fin[(size_t)(2 * n + i)] * fin[(size_t)(2 * n + i)];
+    }
+  if (loss) *loss = na > 0 ? lsum / (double)na : 0.0;
+  if (n_alive) *n_alive = na;
+  // reverse pass
+  TP_HIP(hipEventRecord(ctx->ev0, st));
+  if (n > 0 && n_slots > 0)
+    TP_HIP(hipMemsetAsync(partial.p, 0, (size_t)nblocks * n_slots * sizeof(double), st));
+  if (n > 0) TP_HIP(hipMemsetAsync(lam.p, 0, 6 * plane * sizeof(double), st));
+  TpAdjArgs a{};
+  a.seq = t->d_seq;
+  a.recs = t->d_recs;
+  a.pool = t->d_pool;
+  a.slot_of_pos = t->d_slots;
+  a.lam = (double*)lam.p;
+  a.lost_turn = p->d_lt;
+  a.partial = (double*)partial.p;
+  a.n = n;
+  a.ld = n;
+  a.n_seq = (int32_t)t->hdr.n_seq;
+  a.n_slots = n_slots;
+  a.variant = t->variant;
+  a.beti = t->hdr.beti;
+  a.inv_na = na > 0 ? 1.0 / (double)na : 0.0;
+  if (n > 0 && na > 0 && a.n_seq > 0) {
+    for (int32_t j = nseg - 1; j >= 0; --j) {
+      a.end_state = (const double*)ckpt.p + (size_t)j * 6 * plane;
+      a.nturns = (j == nseg - 1) ? (nturns - j * K) : K;
+      a.init_lambda = (j == nseg - 1) ? 1 : 0;
+      hipError_t le = fast ? tp_launch_adjoint_fast(a, st) : tp_launch_adjoint_exact(a, st);
+      if (le != hipSuccess) return fail(TP_EHIP, "tp_adjoint: reverse kernel launch failed: %s", hipGetErrorName(le));
+    }
+    hipError_t le = tp_launch_grad_reduce((const double*)partial.p, nblocks, n_slots, (double*)grad.p, st);
+    if (le != hipSuccess) return fail(TP_EHIP, "tp_adjoint: reduction launch failed: %s", hipGetErrorName(le));
+  } else if (n_slots > 0) {
+    TP_HIP(hipMemsetAsync(grad.p, 0, (size_t)n_slots * sizeof(double), st));
+  }
+  TP_HIP(hipEventRecord(ctx->ev1, st));
+  TP_HIP(hipEventSynchronize(ctx->ev1));
+  float bwd_ms = 0.f;
+  TP_HIP(hipEventElapsedTime(&bwd_ms, ctx->ev0, ctx->ev1));
+  if (flags & TP_TRACK_TIMED) {
+    ctx->timing.forward_ms = fwd_ms;
+    ctx->timing.backward_ms = bwd_ms;
+  }
+  if (dL_dk) {
+    std::vector<double> g((size_t)(n_slots > 0 ? n_slots : 1), 0.0);
+    if (n_slots > 0) TP_HIP(hipMemcpy(g.data(), grad.p, (size_t)n_slots * sizeof(double), hipMemcpyDeviceToHost));
+    int32_t k = 0;
+    for (uint32_t e2 = 0; e2 < t->hdr.n_seq; ++e2) {
+      const tp_rec& r = t->recs[t->seq[e2]];
+      if (r.gorder >= 0) {
+        // d b_g / d k: b1 = k1, b2 = k2/2, b3 = k3/6 (src/tracking.jl:395, 446, 478)
+        const double sc = r.gorder == 1 ? 1.0 : (r.gorder == 2 ? 0.5 : (r.gorder == 3 ? 1.0 / 6.0 : 1.0));
+        dL_dk[e2] = g[(size_t)k++] * sc;
+      } else {
+        dL_dk[e2] = 0.0;
+      }
+    }
+  }
+  if (dL_dx0 && n > 0)
+    TP_HIP(hipMemcpy(dL_dx0, lam.p, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
   return TP_OK;
 }
 

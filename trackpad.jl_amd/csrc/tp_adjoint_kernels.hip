@@ -1,0 +1,275 @@
+// tp_adjoint_kernels.hip -- reverse-mode adjoint of multi-turn tracking for gfx950.
+//
+// Loss: L = (1/N_alive) sum over particles alive after the last turn of (x^2 + y^2).
+// The forward pass (the tracking kernel, run segment by segment) stores the beam at
+// segment boundaries.  adjoint_kernel processes one segment backwards: each lane starts
+// from the stored state at the segment's end, undoes every sub-step with its inverse map
+// and applies the sub-step's VJP to its adjoint (tp_adjoint.h), so no intermediate state
+// is stored.  dL/d(strength) of every lattice position with a gradient slot is reduced
+// across the wave with DPP shuffles, accumulated per wave in LDS and added once per launch
+// to the block's row of a [blocks x slots] partial array; reduce_kernel sums the rows in a
+// fixed order, so gradients are bit-reproducible.
+//
+// Compiled twice like the forward kernels (TP_FAST = 0 / 1).
+#include "tp_adjoint.h"
+
+#if TP_FAST
+#define TP_ADJ_LAUNCH tp_launch_adjoint_fast
+#define TPK_BUILD adj_fast
+#else
+#define TP_ADJ_LAUNCH tp_launch_adjoint_exact
+#define TPK_BUILD adj_exact
+#endif
+
+namespace tpk {
+namespace TPK_BUILD {
+
+#define TP_KICK_SWITCH(FEAT, CALL)                                         \
+  switch (rec.kick) {                                                      \
+    case TP_KICK_NONE: CALL(TP_KICK_NONE); break;                          \
+    case TP_KICK_QUAD: CALL(TP_KICK_QUAD); break;                          \
+    case TP_KICK_SEXT: CALL(TP_KICK_SEXT); break;                          \
+    default:                                                               \
+      if constexpr ((FEAT) & FT_GENKICK) { CALL(TP_KICK_GEN); }            \
+      break;                                                               \
+  }
+
+template <int KK>
+TP_DEV void mpole_back(St& s, Ad& a, const Cx& c, const double* __restrict__ p, int ns, int m, int g, double& gacc) {
+  const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3], L1b = p[4], L2b = p[5];
+  const double* poly = p + 6;
+  // the Yoshida step D1 K1 D2 K2 D2 K1 D1 is a palindrome: undo it in the same order
+  for (int st = 0; st < ns; ++st) {
+    drift_back(s, a, c, L1, L1b);
+    kick_back<KK>(s, a, poly, m, K1, g, gacc);
+    drift_back(s, a, c, L2, L2b);
+    kick_back<KK>(s, a, poly, m, K2, g, gacc);
+    drift_back(s, a, c, L2, L2b);
+    kick_back<KK>(s, a, poly, m, K1, g, gacc);
+    drift_back(s, a, c, L1, L1b);
+  }
+}
+
+template <int KK>
+TP_DEV void sbend_back(St& s, Ad& a, const Cx& c, const double* __restrict__ p, int ns, int m, double beti) {
+  const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3], L1b = p[4], L2b = p[5];
+  const double irho = p[6], K1i = p[7], K2i = p[8];
+  const double* poly = p + 17;
+  for (int st = 0; st < ns; ++st) {
+    drift_back(s, a, c, L1, L1b);
+    bkick_back<KK>(s, a, poly, m, K1, K1i, irho, beti);
+    drift_back(s, a, c, L2, L2b);
+    bkick_back<KK>(s, a, poly, m, K2, K2i, irho, beti);
+    drift_back(s, a, c, L2, L2b);
+    bkick_back<KK>(s, a, poly, m, K1, K1i, irho, beti);
+    drift_back(s, a, c, L1, L1b);
+  }
+}
+
+template <int KK>
+TP_DEV void esbend_body_back(St& s, Ad& a, const Cx& c, const double* __restrict__ p, int ns, int m) {
+  const double irho = p[0];
+  double gdummy = 0.0;
+  double d2 = pxyz(c, 0.0, s.py);
+  if (ns == 0) {
+    ebody_back(s, a, c, irho, p[1], p[5], p[7], p[8], d2);
+    return;
+  }
+  const double L1 = p[1], L2 = p[2], K1 = p[3], K2 = p[4], L1b = p[5], L2b = p[6];
+  const double c1 = p[7], s1 = p[8], c2 = p[9], s2 = p[10];
+  const double* poly = p + 24;
+  for (int st = 0; st < ns; ++st) {
+    ebody_back(s, a, c, irho, L1, L1b, c1, s1, d2);
+    kick_back<KK>(s, a, poly, m, K1, -1, gdummy);
+    if (KK != TP_KICK_NONE) d2 = pxyz(c, 0.0, s.py);
+    ebody_back(s, a, c, irho, L2, L2b, c2, s2, d2);
+    kick_back<KK>(s, a, poly, m, K2, -1, gdummy);
+    if (KK != TP_KICK_NONE) d2 = pxyz(c, 0.0, s.py);
+    ebody_back(s, a, c, irho, L2, L2b, c2, s2, d2);
+    kick_back<KK>(s, a, poly, m, K1, -1, gdummy);
+    if (KK != TP_KICK_NONE) d2 = pxyz(c, 0.0, s.py);
+    ebody_back(s, a, c, irho, L1, L1b, c1, s1, d2);
+  }
+}
+
+TP_DEV void sub6(St& s, const double* __restrict__ t) {
+  s.x = s.x - t[0];
+  s.px = s.px - t[1];
+  s.y = s.y - t[2];
+  s.py = s.py - t[3];
+  s.z = s.z - t[4];
+  s.d = s.d - t[5];
+}
+
+// One element backwards: s is the element's output on entry, its input on return.
+template <unsigned FEAT>
+TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double* __restrict__ pool, double beti,
+                            double& gacc) {
+  const double* __restrict__ p = pool + rec.off;
+  const uint32_t fl = rec.flags;
+  const int m = rec.max_order, ns = rec.nsteps, g = rec.gorder;
+  if (fl & TP_F_T2) {
+    sub6(s, pool + rec.moff + 6);
+    mkctx(s, c, beti);
+  }
+  switch (rec.tag) {
+    case TP_DRIFT:
+      drift_back(s, a, c, p[0], p[1]);
+      break;
+    case TP_MPOLE: {
+#define TP_CALL(KK) mpole_back<KK>(s, a, c, p, ns, m, g, gacc)
+      TP_KICK_SWITCH(FEAT, TP_CALL)
+#undef TP_CALL
+      break;
+    }
+    case TP_THINMP: {
+#define TP_CALL(KK) kick_back<KK>(s, a, p, m, 1.0, -1, gacc)
+      TP_KICK_SWITCH(FEAT, TP_CALL)
+#undef TP_CALL
+      break;
+    }
+    case TP_SBEND:
+      if constexpr ((FEAT & FT_SBEND) != 0) {
+        const int fin = (fl >> TP_F_FIN_SHIFT) & 0xF, fout = (fl >> TP_F_FOUT_SHIFT) & 0xF;
+        const double irho = p[6];
+        if (fout != 0) edge_back(s, a, fout, p + 13, irho, true);
+#define TP_CALL(KK) sbend_back<KK>(s, a, c, p, ns, m, beti)
+        TP_KICK_SWITCH(FEAT, TP_CALL)
+#undef TP_CALL
+        if (fin != 0) edge_back(s, a, fin, p + 9, irho, false);
+      }
+      break;
+    case TP_ESBEND_LE:
+      if constexpr ((FEAT & FT_ESBEND_LE) != 0) {
+        const int fin = (fl >> TP_F_FIN_SHIFT) & 0xF, fout = (fl >> TP_F_FOUT_SHIFT) & 0xF;
+        const double irho = p[0];
+        if (fout != 0) edge_back(s, a, fout, p + 15, irho, true);
+#define TP_CALL(KK) esbend_body_back<KK>(s, a, c, p, ns, m)
+        TP_KICK_SWITCH(FEAT, TP_CALL)
+#undef TP_CALL
+        if (fin != 0) edge_back(s, a, fin, p + 11, irho, false);
+      }
+      break;
+    case TP_RFCAV:
+      if constexpr ((FEAT & FT_RF) != 0) {
+        const bool dr = (fl & TP_F_RF_DRIFT) != 0;
+        if (dr) drift_back(s, a, c, p[0], p[1]);
+        if (fl & TP_F_RF_KICK) {
+          rf_kick_back(s, a, p);
+          mkctx(s, c, beti);
+        }
+        if (dr) drift_back(s, a, c, p[0], p[1]);
+      }
+      break;
+    default:  // TP_MARKER
+      break;
+  }
+  if (fl & TP_F_T1) {
+    sub6(s, pool + rec.moff);
+    mkctx(s, c, beti);
+  }
+}
+
+TP_DEV double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <unsigned FEAT>
+__global__ __launch_bounds__(256) void adjoint_kernel(const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
+                                                      const double* __restrict__ pool,
+                                                      const int32_t* __restrict__ slot_of_pos,
+                                                      const double* __restrict__ end_state, double* __restrict__ lam,
+                                                      const int32_t* __restrict__ lost_turn,
+                                                      double* __restrict__ partial, TpAdjArgs a) {
+  extern __shared__ double lds[];  // [waves per block][n_slots]
+  const int nw = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  for (int k = threadIdx.x; k < nw * a.n_slots; k += blockDim.x) lds[k] = 0.0;
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool alive = (i < a.n) && (lost_turn[i] == -1);
+  St s{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  Ad l{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (alive) {
+    const double* q = end_state + i;
+    s = St{q[0], q[a.ld], q[2 * a.ld], q[3 * a.ld], q[4 * a.ld], q[5 * a.ld]};
+    if (a.init_lambda) {
+      l.x = 2.0 * s.x * a.inv_na;
+      l.y = 2.0 * s.y * a.inv_na;
+    } else {
+      const double* r = lam + i;
+      l = Ad{r[0], r[a.ld], r[2 * a.ld], r[3 * a.ld], r[4 * a.ld], r[5 * a.ld]};
+    }
+  }
+  Cx c;
+  mkctx(s, c, a.beti);
+  for (int32_t t = 0; t < a.nturns; ++t) {
+    for (int32_t e = a.n_seq - 1; e >= 0; --e) {
+      const tp_rec rec = recs[seq[e]];
+      double gacc = 0.0;
+      apply_elem_back<FEAT>(s, l, c, rec, pool, a.beti, gacc);
+      const int32_t slot = slot_of_pos[e];
+      if (slot >= 0) {
+        const double w = wave_sum(gacc);
+        if (lane == 0) lds[wave * a.n_slots + slot] += w;
+      }
+    }
+  }
+  if (i < a.n) {
+    double* r = lam + i;
+    r[0] = l.x;
+    r[a.ld] = l.px;
+    r[2 * a.ld] = l.y;
+    r[3 * a.ld] = l.py;
+    r[4 * a.ld] = l.z;
+    r[5 * a.ld] = l.d;
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < a.n_slots; k += blockDim.x) {
+    double acc = 0.0;
+    for (int w = 0; w < nw; ++w) acc += lds[w * a.n_slots + k];
+    partial[(int64_t)blockIdx.x * a.n_slots + k] += acc;
+  }
+}
+
+// grad[k] = sum over blocks b of partial[b][k], in block order.
+__global__ void reduce_kernel(const double* __restrict__ partial, int nblocks, int n_slots, double* __restrict__ grad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_slots) return;
+  double acc = 0.0;
+  for (int b = 0; b < nblocks; ++b) acc += partial[(int64_t)b * n_slots + k];
+  grad[k] = acc;
+}
+
+template <unsigned FEAT>
+static void launch_v(const TpAdjArgs& a, unsigned grid, hipStream_t stream) {
+  const size_t lds = (size_t)4 * (size_t)(a.n_slots > 0 ? a.n_slots : 1) * sizeof(double);
+  hipLaunchKernelGGL((adjoint_kernel<FEAT>), dim3(grid), dim3(256), lds, stream, a.seq, a.recs, a.pool,
+                     a.slot_of_pos, a.end_state, a.lam, a.lost_turn, a.partial, a);
+}
+
+}  // namespace TPK_BUILD
+}  // namespace tpk
+
+hipError_t TP_ADJ_LAUNCH(const TpAdjArgs& a, hipStream_t stream) {
+  if (a.n <= 0) return hipSuccess;
+  const int64_t grid64 = (a.n + 255) / 256;
+  if (grid64 > 0x7fffffff) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)grid64;
+  switch (a.variant) {
+    case TP_VARIANT_A: tpk::TPK_BUILD::launch_v<TP_FEAT_A>(a, grid, stream); break;
+    case TP_VARIANT_B: tpk::TPK_BUILD::launch_v<TP_FEAT_B>(a, grid, stream); break;
+    default: tpk::TPK_BUILD::launch_v<TP_FEAT_C>(a, grid, stream); break;
+  }
+  return hipGetLastError();
+}
+
+#if TP_FAST
+hipError_t tp_launch_grad_reduce(const double* partial, int nblocks, int n_slots, double* grad, hipStream_t stream) {
+  if (n_slots <= 0) return hipSuccess;
+  hipLaunchKernelGGL(tpk::TPK_BUILD::reduce_kernel, dim3((n_slots + 255) / 256), dim3(256), 0, stream, partial,
+                     nblocks, n_slots, grad);
+  return hipGetLastError();
+}
+#endif

@@ -56,8 +56,33 @@ struct TpKArgs {
   int32_t variant;  // TP_VARIANT_*
 };
 
+// Arguments of one backward segment of the adjoint (tp_adjoint_kernels.hip).
+struct TpAdjArgs {
+  const int32_t* seq;
+  const tp_rec* recs;
+  const double* pool;
+  const int32_t* slot_of_pos;  // gradient slot of each lattice position, -1 for none
+  const double* end_state;     // beam at the end of the segment (6 planes of ld)
+  double* lam;                 // adjoint: at the segment end on entry, at its start on return
+  const int32_t* lost_turn;    // loss record after the last turn (alive == -1)
+  double* partial;             // [blocks][n_slots] strength-gradient partial sums (accumulated)
+  int64_t n;
+  int64_t ld;
+  int32_t n_seq;
+  int32_t n_slots;
+  int32_t nturns;              // turns in this segment
+  int32_t init_lambda;         // 1: start from dL/dstate of the loss at end_state
+  int32_t variant;
+  int32_t pad;
+  double beti;
+  double inv_na;               // 1 / number of surviving particles
+};
+
 // Host launchers exported by the kernel translation units (one per build flavour).
 hipError_t tp_launch_track_exact(const TpKArgs& a, int block, int ppl, hipStream_t stream);
 hipError_t tp_launch_track_fast(const TpKArgs& a, int block, int ppl, hipStream_t stream);
+hipError_t tp_launch_adjoint_exact(const TpAdjArgs& a, hipStream_t stream);
+hipError_t tp_launch_adjoint_fast(const TpAdjArgs& a, hipStream_t stream);
+hipError_t tp_launch_grad_reduce(const double* partial, int nblocks, int n_slots, double* grad, hipStream_t stream);
 
 #endif  // TP_DEVICE_H

@@ -75,6 +75,7 @@ TP_DEV double rcp_nr(double v) {
 __device__ __noinline__ double asin_lib(double u) { return asin(u); }
 __device__ __noinline__ double tan_lib(double u) { return tan(u); }
 __device__ __noinline__ double sin_lib(double u) { return sin(u); }
+__device__ __noinline__ double cos_lib(double u) { return cos(u); }
 
 // asin(u) for |u| <= 1/8 by its Maclaurin series
 //   asin(u) = sum_k C(2k,k) / (4^k (2k+1)) u^(2k+1),

@@ -20,4 +20,14 @@ from .tracking import (  # noqa: F401
     DeviceTable, HIPContext, HIPParticles, default_context, device_table, linepass_, ringpass_, track_,
 )
 
+from .adjoint import Gradients, ringpass_grad, set_strengths, strength_positions  # noqa: F401
+
 __version__ = "0.1.0"
+
+
+def __getattr__(name):
+    if name == "TrackLoss":
+        from . import adjoint
+
+        return adjoint.TrackLoss
+    raise AttributeError(name)

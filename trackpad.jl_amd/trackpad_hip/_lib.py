@@ -35,7 +35,8 @@ class TrackPadHIPError(RuntimeError):
 
 class TimingT(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("launches", C.c_int32), ("block", C.c_int32),
-                ("ppl", C.c_int32), ("turns_per_launch", C.c_int32)]
+                ("ppl", C.c_int32), ("turns_per_launch", C.c_int32), ("forward_ms", C.c_double),
+                ("backward_ms", C.c_double)]
 
 
 _lock = threading.Lock()
@@ -66,6 +67,7 @@ _SIGS = {
     "tp_particles_device_ptrs": ([_P, C.POINTER(_PD), C.POINTER(_PI32), C.POINTER(_PI32), _PI64], C.c_int),
     "tp_track": ([_P, _P, _P, C.c_int32, C.c_int32, C.c_uint32], C.c_int),
     "tp_sync": ([_P], C.c_int),
+    "tp_adjoint": ([_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _PD, _PI64, _PD, _PD], C.c_int),
     "tp_track_host": ([_P, _P, _PD, C.c_int64, _PI64, _PI32, _PI32, C.c_int32, C.c_int32, C.c_uint32], C.c_int),
 }
 

@@ -80,7 +80,7 @@ class HIPContext:
         t = L.TimingT()
         L.check(self._lib.tp_timing(self.handle, C.byref(t)))
         return {"kernel_ms": t.kernel_ms, "launches": t.launches, "block": t.block, "ppl": t.ppl,
-                "turns_per_launch": t.turns_per_launch}
+                "turns_per_launch": t.turns_per_launch, "forward_ms": t.forward_ms, "backward_ms": t.backward_ms}
 
     def sync(self) -> None:
         L.check(self._lib.tp_sync(self.handle))
