@@ -1,0 +1,241 @@
+"""
+    TrackPadHIP
+
+MI355X backend for TrackPad.jl's multi-particle tracking path.  A static `TrackPad.Lattice`
+is compiled into the flat device element table of `include/trackpad_table.h`
+(`compile_table`), uploaded once, and tracked by `libtrackpad_hip` through its C ABI
+(`include/trackpad_hip.h`).  Device-resident particles (`HIPParticles`) get methods of the
+reference's `linepass!` / `ringpass!` (src/lattice.jl:260-317) and a `track!`; host arrays go
+through `linepass_hip!` / `ringpass_hip!` (no type piracy on `Matrix`).  Gradients of
+`mean(x^2 + y^2)` come from the hand-written adjoint kernel (`ringpass_grad`), which the
+ChainRulesCore extension registers as an `rrule`.
+"""
+module TrackPadHIP
+
+using Libdl
+using TrackPad
+using TrackPad: AbstractElement, Beam, Lattice, Drift, Marker, Quadrupole, Sextupole, Octupole,
+                ThinMultipole, SBend, ExactSBend, RFCavity, TimeVaryingElement
+
+export HIPContext, HIPTable, HIPParticles, compile_table, track!, linepass_hip!, ringpass_hip!,
+       ringpass_grad, ExactSBendLinearEdge, download
+
+const LIBPATH = Ref{String}(get(ENV, "TRACKPAD_HIP_LIB",
+                                joinpath(@__DIR__, "..", "..", "..", "lib", "libtrackpad_hip.so")))
+libtp() = LIBPATH[]
+
+const TP_TRACK_FAST = UInt32(1)
+const TP_TRACK_ASYNC = UInt32(2)
+const TP_TRACK_TIMED = UInt32(4)
+
+struct TrackPadHIPError <: Exception
+    code::Cint
+    msg::String
+end
+Base.showerror(io::IO, e::TrackPadHIPError) = print(io, "libtrackpad_hip error ", e.code, ": ", e.msg)
+
+function check(rc::Cint)
+    if rc != 0
+        msg = unsafe_string(ccall((:tp_last_error, libtp()), Cstring, ()))
+        throw(TrackPadHIPError(rc, msg))
+    end
+    return nothing
+end
+
+# ----------------------------------------------------------------------------------------
+# Handles (finalizers may run in any order: the library reference-counts the context)
+# ----------------------------------------------------------------------------------------
+mutable struct HIPContext
+    handle::Ptr{Cvoid}
+    device::Int
+    function HIPContext(device::Integer = 0)
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:tp_ctx_create, libtp()), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, h))
+        c = new(h[], Int(device))
+        finalizer(c) do x
+            ccall((:tp_ctx_destroy, libtp()), Cint, (Ptr{Cvoid},), x.handle)
+        end
+        return c
+    end
+end
+
+const DEFAULT_CTX = Ref{Union{Nothing, HIPContext}}(nothing)
+default_context() = (DEFAULT_CTX[] === nothing && (DEFAULT_CTX[] = HIPContext(0)); DEFAULT_CTX[])
+
+set_launch!(ctx::HIPContext; block = 0, ppl = 0, turns_per_launch = 0) =
+    check(ccall((:tp_set_launch, libtp()), Cint, (Ptr{Cvoid}, Cint, Cint, Cint), ctx.handle, block, ppl,
+                turns_per_launch))
+
+mutable struct HIPTable
+    handle::Ptr{Cvoid}
+    ctx::HIPContext
+    n_seq::Int
+    function HIPTable(ctx::HIPContext, blob::Vector{UInt8})
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        GC.@preserve blob check(ccall((:tp_table_create, libtp()), Cint,
+                                      (Ptr{Cvoid}, Ptr{UInt8}, Csize_t, Ptr{Ptr{Cvoid}}),
+                                      ctx.handle, blob, length(blob), h))
+        ns = Ref{Int32}(0)
+        check(ccall((:tp_table_info, libtp()), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ptr{Int32}, Ptr{Float64}),
+                    h[], ns, C_NULL, C_NULL))
+        t = new(h[], ctx, Int(ns[]))
+        finalizer(t) do x
+            ccall((:tp_table_destroy, libtp()), Cint, (Ptr{Cvoid},), x.handle)
+        end
+        return t
+    end
+end
+HIPTable(lat::Lattice, beam::Beam; ctx::HIPContext = default_context(), kw...) =
+    HIPTable(ctx, compile_table(lat, beam; kw...))
+
+mutable struct HIPParticles
+    handle::Ptr{Cvoid}
+    ctx::HIPContext
+    n::Int
+    function HIPParticles(n::Integer; ctx::HIPContext = default_context())
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:tp_particles_create, libtp()), Cint, (Ptr{Cvoid}, Int64, Ptr{Ptr{Cvoid}}), ctx.handle, n, h))
+        p = new(h[], ctx, Int(n))
+        finalizer(p) do x
+            ccall((:tp_particles_destroy, libtp()), Cint, (Ptr{Cvoid},), x.handle)
+        end
+        return p
+    end
+end
+
+"""
+    HIPParticles(coords::Matrix{Float64}, lost_flags = nothing)
+
+Upload an N x 6 beam (the reference's `coords` layout, src/lattice.jl:274-275: the Julia
+column-major matrix is exactly the device's six planes).  `lost_flags[i] == 1` marks a
+particle as already lost (src/lattice.jl:269-271).
+"""
+function HIPParticles(coords::Matrix{Float64}, lost_flags::Union{Nothing, Vector{Int}} = nothing;
+                      ctx::HIPContext = default_context())
+    size(coords, 2) == 6 || throw(ArgumentError("coords must be N x 6"))
+    p = HIPParticles(size(coords, 1); ctx = ctx)
+    upload!(p, coords, lost_flags)
+    return p
+end
+
+function upload!(p::HIPParticles, coords::Matrix{Float64}, lost_flags = nothing)
+    size(coords) == (p.n, 6) || throw(DimensionMismatch("coords must be $(p.n) x 6"))
+    lf = lost_flags === nothing ? C_NULL : pointer(lost_flags)
+    GC.@preserve coords lost_flags check(ccall((:tp_particles_upload, libtp()), Cint,
+                                                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Int64}, Ptr{Int32}, Ptr{Int32}),
+                                                p.handle, coords, lf, C_NULL, C_NULL))
+    return p
+end
+
+"""(coords, lost_flags, lost_turn, lost_elem) on the host; lost_turn / lost_elem are 0-based, -1 alive."""
+function download(p::HIPParticles)
+    coords = Matrix{Float64}(undef, p.n, 6)
+    lf = Vector{Int64}(undef, p.n)
+    lt = Vector{Int32}(undef, p.n)
+    le = Vector{Int32}(undef, p.n)
+    check(ccall((:tp_particles_download, libtp()), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Ptr{Int64}, Ptr{Int32}, Ptr{Int32}), p.handle, coords, lf, lt, le))
+    return coords, Vector{Int}(lf), lt, le
+end
+
+# ----------------------------------------------------------------------------------------
+# Tracking entry points
+# ----------------------------------------------------------------------------------------
+"""
+    track!(p::HIPParticles, lat, beam; nturns = 1, turn = 0, fast = false)
+
+Track device-resident particles for `nturns` turns (the fused whole-turn kernel).
+"""
+function track!(p::HIPParticles, tab::HIPTable; nturns::Integer = 1, turn::Integer = 0, fast::Bool = false)
+    tab.ctx === p.ctx || throw(ArgumentError("table and particles belong to different contexts"))
+    check(ccall((:tp_track, libtp()), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Int32, UInt32),
+                p.ctx.handle, tab.handle, p.handle, turn, nturns, fast ? TP_TRACK_FAST : UInt32(0)))
+    return p
+end
+track!(p::HIPParticles, lat::Lattice, beam::Beam; aperture = nothing, kw...) =
+    track!(p, HIPTable(p.ctx, compile_table(lat, beam; aperture = aperture)); kw...)
+
+# Methods of the reference's functions for device particles (src/lattice.jl:260, 305).
+TrackPad.linepass!(p::HIPParticles, lat::Lattice, beam::Beam; time::Real = 0, turn::Integer = 0, kw...) =
+    (track!(p, lat, beam; nturns = 1, turn = turn, kw...); nothing)
+TrackPad.ringpass!(p::HIPParticles, lat::Lattice, beam::Beam, nturns::Int; time::Real = 0,
+                   dt_turn::Real = 0, turn::Integer = 0, kw...) =
+    (track!(p, lat, beam; nturns = nturns, turn = turn, kw...); nothing)
+
+"""
+    ringpass_hip!(coords, lat, beam, lost_flags, nturns; turn = 0, fast = false, aperture = nothing)
+
+`ringpass!` on host arrays through the GPU: mutates `coords` and `lost_flags` in place with the
+reference's semantics (flags equal to 1 are skipped; newly lost particles get 1 and keep their
+element-entry coordinates).
+"""
+function ringpass_hip!(coords::Matrix{Float64}, lat::Lattice, beam::Beam, lost_flags::Vector{Int},
+                       nturns::Integer; turn::Integer = 0, fast::Bool = false, aperture = nothing,
+                       ctx::HIPContext = default_context())
+    n = size(coords, 1)
+    size(coords, 2) == 6 || throw(ArgumentError("coords must be N x 6"))
+    length(lost_flags) == n || throw(DimensionMismatch("lost_flags must have N entries"))
+    tab = HIPTable(ctx, compile_table(lat, beam; aperture = aperture))
+    GC.@preserve coords lost_flags check(ccall((:tp_track_host, libtp()), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Int64}, Ptr{Int32}, Ptr{Int32}, Int32, Int32, UInt32),
+        ctx.handle, tab.handle, coords, n, lost_flags, C_NULL, C_NULL, turn, nturns,
+        fast ? TP_TRACK_FAST : UInt32(0)))
+    return nothing
+end
+linepass_hip!(coords::Matrix{Float64}, lat::Lattice, beam::Beam, lost_flags::Vector{Int}; kw...) =
+    ringpass_hip!(coords, lat, beam, lost_flags, 1; kw...)
+
+"""
+    ringpass_grad(coords0, lat, beam, nturns; ckpt_stride = 1, fast = false)
+        -> (loss, dL_dk::Vector{Float64}, dL_dx0::Matrix{Float64}, n_alive)
+
+`loss = mean over surviving particles of (x^2 + y^2)` after `nturns` turns; `dL_dk[i]` is the
+derivative with respect to the main strength of lattice position `i` (k1 of a Quadrupole, k2
+of a Sextupole, k3 of an Octupole, 0 elsewhere); `dL_dx0` with respect to the initial beam.
+"""
+function ringpass_grad(coords0::Matrix{Float64}, lat::Lattice, beam::Beam, nturns::Integer;
+                       ckpt_stride::Integer = 1, fast::Bool = false, ctx::HIPContext = default_context())
+    tab = HIPTable(ctx, compile_table(lat, beam))
+    p = HIPParticles(coords0; ctx = ctx)
+    loss = Ref{Float64}(0)
+    na = Ref{Int64}(0)
+    dk = zeros(Float64, max(tab.n_seq, 1))
+    dx0 = zeros(Float64, p.n, 6)
+    check(ccall((:tp_adjoint, libtp()), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Int32, Int32, UInt32, Ptr{Float64}, Ptr{Int64},
+                 Ptr{Float64}, Ptr{Float64}),
+                ctx.handle, tab.handle, p.handle, 0, nturns, ckpt_stride, fast ? TP_TRACK_FAST : UInt32(0),
+                loss, na, dk, dx0))
+    return loss[], dk[1:tab.n_seq], dx0, Int(na[])
+end
+
+"""A copy of `lat` whose main strengths (k1 / k2 / k3) at 1-based `positions` are `k`."""
+function set_strengths(lat::Lattice, positions, k)
+    els = Any[e for e in lat.elements]
+    for (p, v) in zip(positions, k)
+        e = els[p]
+        f = e isa Quadrupole ? :k1 : e isa Sextupole ? :k2 : e isa Octupole ? :k3 :
+            throw(ArgumentError("position $p has no strength"))
+        els[p] = typeof(e)((n === f ? oftype(getfield(e, n), v) : getfield(e, n) for n in fieldnames(typeof(e)))...)
+    end
+    return Lattice(els; name = lat.name)
+end
+
+"""
+    track_loss(k, coords0, lat, beam, nturns, positions)
+
+`mean(x^2 + y^2)` over surviving particles after `nturns` turns of `lat` with strengths `k` at
+`positions`; differentiable through the ChainRulesCore extension (hand-written adjoint).
+"""
+function track_loss(k::Vector{Float64}, coords0::Matrix{Float64}, lat, beam, nturns::Integer, positions)
+    ctx = default_context()
+    p = HIPParticles(coords0; ctx = ctx)
+    track!(p, set_strengths(lat, positions, k), beam; nturns = nturns)
+    c, lf, _, _ = download(p)
+    alive = lf .== 0
+    return sum(c[alive, 1] .^ 2 .+ c[alive, 3] .^ 2) / count(alive)
+end
+
+include("table.jl")
+
+end # module TrackPadHIP
