@@ -55,6 +55,7 @@ enum tp_status {
 typedef struct tp_ctx tp_ctx;
 typedef struct tp_table tp_table;
 typedef struct tp_particles tp_particles;
+typedef struct tp_batch tp_batch;
 
 typedef struct tp_timing_t {
   double kernel_ms;    /* summed duration of the timed launches of the last timed call */
@@ -160,6 +161,45 @@ int tp_track_host(tp_ctx* ctx, const tp_table* t, double* coords, int64_t n, int
 int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns,
                int32_t ckpt_stride, uint32_t flags, double* loss, int64_t* n_alive, double* dL_dk,
                double* dL_dx0);
+
+/* ---- Batched lattice variants ----------------------------------------------------------
+ * No reference counterpart: the reference tracks one lattice per linepass!/ringpass! call
+ * (src/lattice.jl:260-317); a parameter sweep calls it once per lattice.  A batch binds
+ * n_var (table, particle set) pairs -- lattice variants with their own beams, of uneven
+ * sizes -- and runs all of them in one persistent launch: waves take (variant, 64-particle
+ * tile) work items, ordered by decreasing cost n_seq x nturns, from a device queue.  The
+ * launch sequence is captured once in a hipGraph and replayed while the turn counts (and,
+ * for the adjoint, the checkpoint stride and flags) stay the same.
+ *
+ * The tables are copied into the batch (they may be destroyed afterwards); the particle
+ * sets are referenced and stay alive until the batch is destroyed.  All variants must use
+ * the same Hamiltonian (exact or expanded) and sit on ctx. */
+int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_particles* const* parts,
+                    tp_batch** out);
+int tp_batch_destroy(tp_batch* b);
+
+/* Work-group size of the persistent kernels (64..256, multiple of 64; 0 = 256) and waves
+ * per SIMD to keep resident (0 = as many as the kernel's registers allow). */
+int tp_batch_set_launch(tp_batch* b, int block, int waves_per_simd);
+
+/* Track variant v from turn turn0[v] (NULL = all 0) for nturns[v] turns, as tp_track on
+ * each (table, particle set) pair would.  flags: TP_TRACK_FAST, TP_TRACK_ASYNC,
+ * TP_TRACK_TIMED (tp_timing: kernel_ms of the graph launch). */
+int tp_batch_track(tp_ctx* ctx, tp_batch* b, const int32_t* turn0, const int32_t* nturns, uint32_t flags);
+
+/* tp_adjoint for every variant in one graph launch: a persistent forward pass that stores
+ * each tile's state every ckpt_stride turns, a persistent reverse pass, and a per-variant
+ * reduction in tile order.  Per variant v (outputs may be NULL): loss[v], n_alive[v];
+ * dL_dk gets n_seq(v) entries per variant, concatenated in variant order; dL_dx0 gets the
+ * N(v) x 6 column-major block of each variant, concatenated in variant order.  Supported
+ * tables as for tp_adjoint.  flags: TP_TRACK_FAST, TP_TRACK_TIMED. */
+int tp_batch_adjoint(tp_ctx* ctx, tp_batch* b, const int32_t* turn0, const int32_t* nturns, int32_t ckpt_stride,
+                     uint32_t flags, double* loss, int64_t* n_alive, double* dL_dk, double* dL_dx0);
+
+/* Shape of the last planned batch launch: work items, persistent grids (blocks) of the
+ * forward and reverse kernels, checkpoint bytes. */
+int tp_batch_info(const tp_batch* b, int32_t* n_items, int32_t* grid_fwd, int32_t* grid_back,
+                  int64_t* ckpt_bytes);
 
 #ifdef __cplusplus
 }

@@ -5,11 +5,13 @@
 // failure returns a TP_E* code and leaves a message in tp_last_error().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -48,7 +50,9 @@ struct tp_table {
   int32_t n_slots = 0;
 };
 
+// A particle set is reference counted too: a batch that binds it holds a reference.
 struct tp_particles {
+  std::atomic<int> refs{1};
   tp_ctx* ctx = nullptr;
   int64_t n = 0;
   double* d_coords = nullptr;
@@ -135,6 +139,18 @@ void ctx_release(tp_ctx* ctx) {
   (void)hipEventDestroy(ctx->ev1);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+
+void particles_release(tp_particles* p) {
+  if (p->refs.fetch_sub(1) != 1) return;
+  tp_ctx* ctx = p->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(p->d_coords);
+  (void)hipFree(p->d_lt);
+  (void)hipFree(p->d_le);
+  delete p;
+  ctx_release(ctx);
 }
 
 }  // namespace
@@ -368,14 +384,7 @@ int tp_particles_create(tp_ctx* ctx, int64_t n, tp_particles** out) {
 
 int tp_particles_destroy(tp_particles* p) {
   if (!p) return TP_OK;
-  tp_ctx* ctx = p->ctx;
-  (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  (void)hipFree(p->d_coords);
-  (void)hipFree(p->d_lt);
-  (void)hipFree(p->d_le);
-  delete p;
-  ctx_release(ctx);
+  particles_release(p);
   return TP_OK;
 }
 
@@ -648,8 +657,10 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
   a.n_slots = n_slots;
   a.variant = t->variant;
   a.beti = t->hdr.beti;
-  a.inv_na = na > 0 ? 1.0 / (double)na : 0.0;
-  if (n > 0 && na > 0 && a.n_seq > 0) {
+  // the reverse pass seeds dL/dx = 2x (the gradient of the loss sum); the 1/N_alive of the
+  // mean is applied to the results below
+  const double inv_na = na > 0 ? 1.0 / (double)na : 0.0;
+  if (n > 0 && na > 0) {
     for (int32_t j = nseg - 1; j >= 0; --j) {
       a.end_state = (const double*)ckpt.p + (size_t)j * 6 * plane;
       a.nturns = (j == nseg - 1) ? (nturns - j * K) : K;
@@ -679,14 +690,479 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
       if (r.gorder >= 0) {
         // d b_g / d k: b1 = k1, b2 = k2/2, b3 = k3/6 (src/tracking.jl:395, 446, 478)
         const double sc = r.gorder == 1 ? 1.0 : (r.gorder == 2 ? 0.5 : (r.gorder == 3 ? 1.0 / 6.0 : 1.0));
-        dL_dk[e2] = g[(size_t)k++] * sc;
+        dL_dk[e2] = g[(size_t)k++] * inv_na * sc;
       } else {
         dL_dk[e2] = 0.0;
       }
     }
   }
-  if (dL_dx0 && n > 0)
+  if (dL_dx0 && n > 0) {
     TP_HIP(hipMemcpy(dL_dx0, lam.p, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < 6 * (size_t)n; ++i) dL_dx0[i] *= inv_na;
+  }
+  return TP_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Batches of lattice variants
+// ---------------------------------------------------------------------------------------
+struct tp_batch {
+  tp_ctx* ctx = nullptr;
+  int32_t n_var = 0;
+  std::vector<tp_particles*> parts;
+  std::vector<int32_t> n_seq, n_slots;
+  std::vector<int64_t> seq_off, rec_off, pool_off, slot_off;
+  std::vector<tp_table_header> hdr;
+  std::vector<int32_t> gorder;  // per concatenated position: gorder of its record
+  unsigned features = 0;
+  int variant = TP_VARIANT_ALL;
+  int exact_h = 1;
+  std::string no_adjoint;  // why the adjoint is unsupported ("" = supported)
+  int32_t total_slots = 0, max_slots = 0;
+  int32_t* d_seq = nullptr;
+  tp_rec* d_recs = nullptr;
+  double* d_pool = nullptr;
+  int32_t* d_slots = nullptr;
+  int block = 256, waves_per_simd = 0, cus = 0;
+  // the planned launch (rebuilt when its key changes)
+  int mode = 0;  // 0 none, 1 track, 2 adjoint
+  uint32_t pflags = 0;
+  int32_t pstride = 0;
+  int pblock = 0, pwps = 0;
+  std::vector<int32_t> pturn0, pnturns;
+  int32_t n_items = 0, grid_f = 0, grid_b = 0;
+  int64_t ckpt_bytes = 0;
+  std::vector<int64_t> lam_off;
+  void *d_desc = nullptr, *d_items = nullptr, *d_counter = nullptr, *d_vfirst = nullptr, *d_vtiles = nullptr;
+  void *d_ckpt = nullptr, *d_item_loss = nullptr, *d_item_partial = nullptr, *d_grad = nullptr, *d_vloss = nullptr;
+  void* d_lam = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void dfree(void*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void batch_free_plan(tp_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  for (void** p : {&b->d_desc, &b->d_items, &b->d_counter, &b->d_vfirst, &b->d_vtiles, &b->d_ckpt, &b->d_item_loss,
+                   &b->d_item_partial, &b->d_grad, &b->d_vloss, &b->d_lam})
+    dfree(*p);
+  b->mode = 0;
+  b->n_items = b->grid_f = b->grid_b = 0;
+  b->ckpt_bytes = 0;
+}
+
+hipError_t dalloc(void** p, size_t bytes) { return hipMalloc(p, bytes > 0 ? bytes : 8); }
+
+int pick_grid(int occ_per_cu, int cus, int wpb, int waves_per_simd, int32_t n_items) {
+  int per_cu = occ_per_cu;
+  if (waves_per_simd > 0) per_cu = std::min(per_cu, std::max(1, waves_per_simd * 4 / wpb));
+  if (per_cu < 1) per_cu = 1;
+  const int64_t need = ((int64_t)n_items + wpb - 1) / wpb;
+  return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * cus, need));
+}
+
+// Build the work items, descriptors and buffers for one launch shape and capture the
+// launch sequence in a graph: counter reset (a fill kernel: no memset node), the batched
+// forward kernel and, for the adjoint, the reverse kernel and the per-variant reduction.
+int batch_plan(tp_batch* b, int mode, const int32_t* turn0, const int32_t* nturns, int32_t stride, uint32_t flags) {
+  const int32_t nv = b->n_var;
+  std::vector<int32_t> t0(nv, 0), nt(nturns, nturns + nv);
+  if (turn0) t0.assign(turn0, turn0 + nv);
+  if (b->mode == mode && b->pflags == flags && b->pstride == stride && b->pblock == b->block &&
+      b->pwps == b->waves_per_simd && b->pturn0 == t0 && b->pnturns == nt)
+    return TP_OK;
+  batch_free_plan(b);
+  struct Item {
+    int64_t cost;
+    int32_t v, tile;
+  };
+  std::vector<Item> items;
+  for (int32_t v = 0; v < nv; ++v) {
+    const int64_t n = b->parts[v]->n;
+    if (n <= 0 || nt[v] <= 0) continue;
+    const int64_t tiles = (n + 63) / 64;
+    if ((int64_t)items.size() + tiles > 0x7fffffffLL) return fail(TP_EINVAL, "batch: more than 2^31 work items");
+    for (int64_t k = 0; k < tiles; ++k) items.push_back(Item{(int64_t)b->n_seq[v] * nt[v], v, (int32_t)k});
+  }
+  // longest items first (LPT order); ties by variant then tile, so the tiles of a
+  // variant are contiguous and in tile order
+  std::sort(items.begin(), items.end(), [](const Item& x, const Item& y) {
+    if (x.cost != y.cost) return x.cost > y.cost;
+    if (x.v != y.v) return x.v < y.v;
+    return x.tile < y.tile;
+  });
+  const int32_t n_items = (int32_t)items.size();
+  std::vector<int32_t> vfirst(nv > 0 ? nv : 1, 0), vtiles(nv > 0 ? nv : 1, 0);
+  std::vector<int32_t> hitems(2 * (size_t)(n_items > 0 ? n_items : 1));
+  for (int32_t i = 0; i < n_items; ++i) {
+    const Item& it = items[(size_t)i];
+    if (vtiles[it.v] == 0) vfirst[it.v] = i;
+    ++vtiles[it.v];
+    hitems[2 * (size_t)i] = it.v;
+    hitems[2 * (size_t)i + 1] = it.tile;
+  }
+  const bool adj = (mode == 2);
+  const int32_t K = stride > 0 ? stride : 1;
+  std::vector<TpVarDesc> desc(nv > 0 ? nv : 1);
+  std::vector<int64_t> lam_off(nv > 0 ? nv : 1, 0);
+  int64_t ck = 0, lam = 0;
+  for (int32_t v = 0; v < nv; ++v) {
+    TpVarDesc& d = desc[v];
+    d = TpVarDesc{};
+    tp_particles* p = b->parts[v];
+    d.seq_off = b->seq_off[v];
+    d.rec_off = b->rec_off[v];
+    d.pool_off = b->pool_off[v];
+    d.slot_off = b->slot_off[v];
+    d.coords = p->d_coords;
+    d.lost_turn = p->d_lt;
+    d.lost_elem = p->d_le;
+    d.n = p->n;
+    d.n_seq = b->n_seq[v];
+    d.nturns = nt[v];
+    d.turn0 = t0[v];
+    d.n_slots = b->n_slots[v];
+    d.nseg = adj ? (nt[v] + K - 1) / K : 0;
+    d.ckpt_stride = K;
+    d.beti = b->hdr[v].beti;
+    d.coord_limit = b->hdr[v].coord_limit;
+    d.angle_limit = b->hdr[v].angle_limit;
+    d.ap_r2 = b->hdr[v].aperture_r2;
+    d.ck_off = ck;
+    if (adj) ck += (int64_t)vtiles[v] * d.nseg * 384;
+    lam_off[v] = lam;
+    lam += 6 * p->n;
+  }
+  hipError_t e = dalloc(&b->d_desc, desc.size() * sizeof(TpVarDesc));
+  if (e == hipSuccess) e = dalloc(&b->d_items, hitems.size() * sizeof(int32_t));
+  if (e == hipSuccess) e = dalloc(&b->d_counter, 2 * sizeof(int32_t));
+  if (adj) {
+    if (e == hipSuccess) e = dalloc(&b->d_vfirst, vfirst.size() * sizeof(int32_t));
+    if (e == hipSuccess) e = dalloc(&b->d_vtiles, vtiles.size() * sizeof(int32_t));
+    if (e == hipSuccess) e = dalloc(&b->d_ckpt, (size_t)ck * sizeof(double));
+    if (e == hipSuccess) e = dalloc(&b->d_item_loss, 2 * (size_t)n_items * sizeof(double));
+    if (e == hipSuccess) e = dalloc(&b->d_item_partial, (size_t)n_items * (size_t)b->max_slots * sizeof(double));
+    if (e == hipSuccess) e = dalloc(&b->d_grad, (size_t)b->total_slots * sizeof(double));
+    if (e == hipSuccess) e = dalloc(&b->d_vloss, 2 * (size_t)(nv > 0 ? nv : 1) * sizeof(double));
+    if (e == hipSuccess) e = dalloc(&b->d_lam, (size_t)lam * sizeof(double));
+  }
+  if (e != hipSuccess) {
+    batch_free_plan(b);
+    return fail(e == hipErrorOutOfMemory ? TP_ENOMEM : TP_EHIP, "batch: allocation of the launch plan (%lld B of checkpoints): %s",
+                (long long)(ck * 8), hipGetErrorName(e));
+  }
+  if (adj)
+    for (int32_t v = 0; v < nv; ++v) desc[v].lam = (double*)b->d_lam + lam_off[v];
+  hipStream_t st = b->ctx->stream;
+  TP_HIP(hipMemcpyAsync(b->d_desc, desc.data(), desc.size() * sizeof(TpVarDesc), hipMemcpyHostToDevice, st));
+  TP_HIP(hipMemcpyAsync(b->d_items, hitems.data(), hitems.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  if (adj) {
+    TP_HIP(hipMemcpyAsync(b->d_vfirst, vfirst.data(), vfirst.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    TP_HIP(hipMemcpyAsync(b->d_vtiles, vtiles.data(), vtiles.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  }
+  TP_HIP(hipStreamSynchronize(st));
+  const bool fast = (flags & TP_TRACK_FAST) != 0;
+  const int wpb = b->block / 64;
+  TpBatchArgs a{};
+  a.desc = (const TpVarDesc*)b->d_desc;
+  a.items = (const int2*)b->d_items;
+  a.n_items = n_items;
+  a.seq = b->d_seq;
+  a.recs = b->d_recs;
+  a.pool = b->d_pool;
+  a.slots = b->d_slots;
+  a.ckpt = (double*)b->d_ckpt;
+  a.item_loss = (double*)b->d_item_loss;
+  a.item_partial = (double*)b->d_item_partial;
+  a.max_slots = b->max_slots;
+  a.variant = b->variant;
+  a.exact_h = b->exact_h;
+  a.block = b->block;
+  const int occ_f = fast ? tp_batch_occupancy_fast(b->variant, b->exact_h, adj, b->block)
+                         : tp_batch_occupancy_exact(b->variant, b->exact_h, adj, b->block);
+  if (occ_f < 1) return fail(TP_EHIP, "batch: the forward kernel does not fit a CU with %d threads", b->block);
+  b->grid_f = pick_grid(occ_f, b->cus, wpb, b->waves_per_simd, n_items);
+  if (adj) {
+    const int occ_b = fast ? tp_batch_back_occupancy_fast(b->variant, b->block, b->max_slots)
+                           : tp_batch_back_occupancy_exact(b->variant, b->block, b->max_slots);
+    if (occ_b < 1) return fail(TP_EHIP, "batch: the reverse kernel does not fit a CU with %d threads", b->block);
+    b->grid_b = pick_grid(occ_b, b->cus, wpb, b->waves_per_simd, n_items);
+  }
+  if (n_items > 0) {
+    TP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    hipError_t le = tp_launch_fill_i32((int32_t*)b->d_counter, 0, 2, st);
+    a.counter = (int32_t*)b->d_counter;
+    a.grid = b->grid_f;
+    if (le == hipSuccess) le = fast ? tp_launch_batch_fast(a, adj, st) : tp_launch_batch_exact(a, adj, st);
+    if (adj) {
+      a.counter = (int32_t*)b->d_counter + 1;
+      a.grid = b->grid_b;
+      if (le == hipSuccess) le = fast ? tp_launch_batch_back_fast(a, st) : tp_launch_batch_back_exact(a, st);
+      if (le == hipSuccess)
+        le = tp_launch_batch_reduce(a.desc, (const int32_t*)b->d_vfirst, (const int32_t*)b->d_vtiles, nv,
+                                    a.item_partial, a.max_slots, a.item_loss, (double*)b->d_grad, (double*)b->d_vloss, st);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(st, &g);
+    if (le != hipSuccess || ee != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      batch_free_plan(b);
+      return fail(TP_EHIP, "batch: graph capture failed: %s", hipGetErrorName(le != hipSuccess ? le : ee));
+    }
+    b->graph = g;
+    e = hipGraphInstantiateWithFlags(&b->exec, g, 0);
+    if (e != hipSuccess) {
+      batch_free_plan(b);
+      return fail(TP_EHIP, "batch: graph instantiation failed: %s", hipGetErrorName(e));
+    }
+  }
+  b->n_items = n_items;
+  b->ckpt_bytes = ck * 8;
+  b->lam_off = lam_off;
+  b->mode = mode;
+  b->pflags = flags;
+  b->pstride = stride;
+  b->pblock = b->block;
+  b->pwps = b->waves_per_simd;
+  b->pturn0 = t0;
+  b->pnturns = nt;
+  return TP_OK;
+}
+
+int batch_check(tp_ctx* ctx, tp_batch* b, const int32_t* turn0, const int32_t* nturns, int32_t min_turns, const char* who) {
+  if (!ctx || !b || (!nturns && b->n_var > 0)) return fail(TP_EINVAL, "%s: NULL argument", who);
+  if (b->ctx != ctx) return fail(TP_EINVAL, "%s: batch belongs to another context", who);
+  if (ctx->closed) return fail(TP_EINVAL, "%s: context destroyed", who);
+  for (int32_t v = 0; v < b->n_var; ++v) {
+    const int64_t t0 = turn0 ? turn0[v] : 0;
+    if (nturns[v] < min_turns) return fail(TP_EINVAL, "%s: nturns[%d] = %d", who, v, nturns[v]);
+    if (t0 < 0 || t0 + nturns[v] > 0x7fffffffLL) return fail(TP_EINVAL, "%s: turn range of variant %d overflows int32", who, v);
+  }
+  return set_dev(ctx);
+}
+
+int batch_launch(tp_batch* b, uint32_t flags) {
+  tp_ctx* ctx = b->ctx;
+  hipStream_t st = ctx->stream;
+  const bool timed = (flags & TP_TRACK_TIMED) != 0;
+  if (timed) TP_HIP(hipEventRecord(ctx->ev0, st));
+  if (b->exec) TP_HIP(hipGraphLaunch(b->exec, st));
+  if (timed) {
+    TP_HIP(hipEventRecord(ctx->ev1, st));
+    TP_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.f;
+    TP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->timing.kernel_ms = ms;
+    ctx->timing.launches = b->exec ? 1 : 0;
+    ctx->timing.block = b->block;
+    ctx->timing.ppl = 1;
+    ctx->timing.turns_per_launch = 0;
+  }
+  return TP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_particles* const* parts, tp_batch** out) {
+  if (!ctx || !out || n_var < 0 || (n_var > 0 && (!tables || !parts))) return fail(TP_EINVAL, "tp_batch_create: bad argument");
+  if (ctx->closed) return fail(TP_EINVAL, "tp_batch_create: context destroyed");
+  *out = nullptr;
+  if (n_var > (1 << 20)) return fail(TP_EINVAL, "tp_batch_create: n_var = %d", n_var);
+  for (int32_t v = 0; v < n_var; ++v) {
+    if (!tables[v] || !parts[v]) return fail(TP_EINVAL, "tp_batch_create: variant %d: NULL table or particle set", v);
+    if (tables[v]->ctx != ctx || parts[v]->ctx != ctx)
+      return fail(TP_EINVAL, "tp_batch_create: variant %d belongs to another context", v);
+    if (((tables[v]->features ^ tables[0]->features) & FT_LINH) != 0)
+      return fail(TP_EUNSUPPORTED, "tp_batch_create: variants mix the exact and the expanded Hamiltonian");
+  }
+  int rc = set_dev(ctx);
+  if (rc) return rc;
+  std::unique_ptr<tp_batch> b(new (std::nothrow) tp_batch());
+  if (!b) return fail(TP_ENOMEM, "tp_batch_create: out of host memory");
+  std::vector<int32_t> seq, slots;
+  std::vector<tp_rec> recs;
+  std::vector<double> pool;
+  for (int32_t v = 0; v < n_var; ++v) {
+    const tp_table* t = tables[v];
+    b->features |= t->features;
+    b->seq_off.push_back((int64_t)seq.size());
+    b->rec_off.push_back((int64_t)recs.size());
+    b->pool_off.push_back((int64_t)pool.size());
+    b->slot_off.push_back(b->total_slots);
+    b->n_seq.push_back((int32_t)t->hdr.n_seq);
+    b->hdr.push_back(t->hdr);
+    seq.insert(seq.end(), t->seq.begin(), t->seq.end());
+    recs.insert(recs.end(), t->recs.begin(), t->recs.end());
+    pool.insert(pool.end(), t->pool.begin(), t->pool.end());
+    int32_t ns = 0;
+    for (uint32_t e = 0; e < t->hdr.n_seq; ++e) {
+      const int32_t g = t->recs[t->seq[e]].gorder;
+      slots.push_back(g >= 0 ? ns++ : -1);
+      b->gorder.push_back(g);
+    }
+    b->n_slots.push_back(ns);
+    b->total_slots += ns;
+    b->max_slots = std::max(b->max_slots, ns);
+    if (b->no_adjoint.empty()) {
+      if (t->features & (FT_ESBEND | FT_LINH))
+        b->no_adjoint = "Forest exact bends and the linearised drift have no adjoint here";
+      for (const tp_rec& r : t->recs)
+        if (r.flags & (TP_F_R1 | TP_F_R2)) b->no_adjoint = "r1/r2 rotations have no adjoint here";
+    }
+  }
+  const unsigned feat = b->features;
+  if ((feat & ~TP_FEAT_A) == 0) b->variant = TP_VARIANT_A;
+  else if ((feat & ~TP_FEAT_B) == 0) b->variant = TP_VARIANT_B;
+  else if ((feat & ~TP_FEAT_C) == 0) b->variant = TP_VARIANT_C;
+  else b->variant = TP_VARIANT_ALL;
+  b->exact_h = (feat & FT_LINH) ? 0 : 1;
+  TP_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  hipError_t e = dalloc((void**)&b->d_seq, seq.size() * sizeof(int32_t));
+  if (e == hipSuccess) e = dalloc((void**)&b->d_recs, recs.size() * sizeof(tp_rec));
+  if (e == hipSuccess) e = dalloc((void**)&b->d_pool, pool.size() * sizeof(double));
+  if (e == hipSuccess) e = dalloc((void**)&b->d_slots, slots.size() * sizeof(int32_t));
+  hipStream_t st = ctx->stream;
+  if (e == hipSuccess && !seq.empty()) e = hipMemcpyAsync(b->d_seq, seq.data(), seq.size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !recs.empty())
+    e = hipMemcpyAsync(b->d_recs, recs.data(), recs.size() * sizeof(tp_rec), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !pool.empty()) e = hipMemcpyAsync(b->d_pool, pool.data(), pool.size() * 8, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !slots.empty())
+    e = hipMemcpyAsync(b->d_slots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    for (void* p : {(void*)b->d_seq, (void*)b->d_recs, (void*)b->d_pool, (void*)b->d_slots})
+      if (p) (void)hipFree(p);
+    return fail(e == hipErrorOutOfMemory ? TP_ENOMEM : TP_EHIP, "tp_batch_create: %s", hipGetErrorName(e));
+  }
+  b->ctx = ctx;
+  b->n_var = n_var;
+  for (int32_t v = 0; v < n_var; ++v) {
+    parts[v]->refs.fetch_add(1);
+    b->parts.push_back(parts[v]);
+  }
+  ctx->refs.fetch_add(1);
+  *out = b.release();
+  return TP_OK;
+}
+
+int tp_batch_destroy(tp_batch* b) {
+  if (!b) return TP_OK;
+  tp_ctx* ctx = b->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  batch_free_plan(b);
+  for (void* p : {(void*)b->d_seq, (void*)b->d_recs, (void*)b->d_pool, (void*)b->d_slots})
+    if (p) (void)hipFree(p);
+  for (tp_particles* p : b->parts) particles_release(p);
+  delete b;
+  ctx_release(ctx);
+  return TP_OK;
+}
+
+int tp_batch_set_launch(tp_batch* b, int block, int waves_per_simd) {
+  if (!b) return fail(TP_EINVAL, "tp_batch_set_launch: batch is NULL");
+  if (block == 0) block = 256;
+  if (block < 64 || block > 256 || block % 64 != 0)
+    return fail(TP_EINVAL, "tp_batch_set_launch: block %d not in {64, 128, 192, 256}", block);
+  if (waves_per_simd < 0 || waves_per_simd > 8)
+    return fail(TP_EINVAL, "tp_batch_set_launch: waves_per_simd %d not in [0, 8]", waves_per_simd);
+  b->block = block;
+  b->waves_per_simd = waves_per_simd;
+  return TP_OK;
+}
+
+int tp_batch_info(const tp_batch* b, int32_t* n_items, int32_t* grid_fwd, int32_t* grid_back, int64_t* ckpt_bytes) {
+  if (!b) return fail(TP_EINVAL, "tp_batch_info: batch is NULL");
+  if (n_items) *n_items = b->n_items;
+  if (grid_fwd) *grid_fwd = b->grid_f;
+  if (grid_back) *grid_back = b->grid_b;
+  if (ckpt_bytes) *ckpt_bytes = b->ckpt_bytes;
+  return TP_OK;
+}
+
+int tp_batch_track(tp_ctx* ctx, tp_batch* b, const int32_t* turn0, const int32_t* nturns, uint32_t flags) {
+  int rc = batch_check(ctx, b, turn0, nturns, 0, "tp_batch_track");
+  if (rc) return rc;
+  if (flags & ~(TP_TRACK_FAST | TP_TRACK_ASYNC | TP_TRACK_TIMED))
+    return fail(TP_EINVAL, "tp_batch_track: unknown flags 0x%x", flags);
+  rc = batch_plan(b, 1, turn0, nturns, 0, flags & TP_TRACK_FAST);
+  if (rc) return rc;
+  rc = batch_launch(b, flags);
+  if (rc) return rc;
+  if (!(flags & TP_TRACK_ASYNC)) TP_HIP(hipStreamSynchronize(ctx->stream));
+  return TP_OK;
+}
+
+int tp_batch_adjoint(tp_ctx* ctx, tp_batch* b, const int32_t* turn0, const int32_t* nturns, int32_t ckpt_stride,
+                     uint32_t flags, double* loss, int64_t* n_alive, double* dL_dk, double* dL_dx0) {
+  int rc = batch_check(ctx, b, turn0, nturns, 1, "tp_batch_adjoint");
+  if (rc) return rc;
+  if (flags & ~(TP_TRACK_FAST | TP_TRACK_TIMED)) return fail(TP_EINVAL, "tp_batch_adjoint: unsupported flags 0x%x", flags);
+  if (ckpt_stride < 0) return fail(TP_EINVAL, "tp_batch_adjoint: ckpt_stride = %d", ckpt_stride);
+  if (!b->no_adjoint.empty()) return fail(TP_EUNSUPPORTED, "tp_batch_adjoint: %s", b->no_adjoint.c_str());
+  if ((size_t)(b->block / 64) * b->max_slots * sizeof(double) > 64 * 1024)
+    return fail(TP_EUNSUPPORTED, "tp_batch_adjoint: %d strength slots exceed the 64 KiB LDS budget", b->max_slots);
+  const int32_t K = ckpt_stride > 0 ? ckpt_stride : 1;
+  rc = batch_plan(b, 2, turn0, nturns, K, flags & TP_TRACK_FAST);
+  if (rc) return rc;
+  rc = batch_launch(b, flags);
+  if (rc) return rc;
+  hipStream_t st = ctx->stream;
+  const int32_t nv = b->n_var;
+  std::vector<double> vloss(2 * (size_t)(nv > 0 ? nv : 1), 0.0), grad((size_t)(b->total_slots > 0 ? b->total_slots : 1), 0.0);
+  if (b->exec) {
+    TP_HIP(hipMemcpyAsync(vloss.data(), b->d_vloss, 2 * (size_t)nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (b->total_slots > 0)
+      TP_HIP(hipMemcpyAsync(grad.data(), b->d_grad, (size_t)b->total_slots * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (dL_dx0) {
+      const int64_t tot = nv > 0 ? b->lam_off[(size_t)nv - 1] + 6 * b->parts[(size_t)nv - 1]->n : 0;
+      if (tot > 0) TP_HIP(hipMemcpyAsync(dL_dx0, b->d_lam, (size_t)tot * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+  }
+  TP_HIP(hipStreamSynchronize(st));
+  int64_t pos = 0;
+  for (int32_t v = 0; v < nv; ++v) {
+    const double na = vloss[2 * (size_t)v + 1];
+    const double inv_na = na > 0.0 ? 1.0 / na : 0.0;
+    if (loss) loss[v] = na > 0.0 ? vloss[2 * (size_t)v] / na : 0.0;
+    if (n_alive) n_alive[v] = (int64_t)na;
+    if (dL_dk) {
+      int32_t k = 0;
+      for (int32_t e = 0; e < b->n_seq[v]; ++e) {
+        const int32_t g = b->gorder[(size_t)(b->seq_off[v] + e)];
+        double val = 0.0;
+        if (g >= 0) {
+          // d b_g / d k: b1 = k1, b2 = k2/2, b3 = k3/6 (src/tracking.jl:395, 446, 478)
+          const double sc = g == 1 ? 1.0 : (g == 2 ? 0.5 : (g == 3 ? 1.0 / 6.0 : 1.0));
+          val = na > 0.0 ? grad[(size_t)(b->slot_off[v] + k)] * inv_na * sc : 0.0;
+          ++k;
+        }
+        dL_dk[pos + e] = val;
+      }
+      pos += b->n_seq[v];
+    }
+    if (dL_dx0) {
+      double* x = dL_dx0 + b->lam_off[v];
+      const int64_t m = 6 * b->parts[v]->n;
+      if (!b->exec || na <= 0.0)
+        for (int64_t i = 0; i < m; ++i) x[i] = 0.0;
+      else
+        for (int64_t i = 0; i < m; ++i) x[i] *= inv_na;
+    }
+  }
   return TP_OK;
 }
 

@@ -15,9 +15,13 @@
 
 #if TP_FAST
 #define TP_ADJ_LAUNCH tp_launch_adjoint_fast
+#define TP_BATCH_BACK_LAUNCH tp_launch_batch_back_fast
+#define TP_BATCH_BACK_OCC tp_batch_back_occupancy_fast
 #define TPK_BUILD adj_fast
 #else
 #define TP_ADJ_LAUNCH tp_launch_adjoint_exact
+#define TP_BATCH_BACK_LAUNCH tp_launch_batch_back_exact
+#define TP_BATCH_BACK_OCC tp_batch_back_occupancy_exact
 #define TPK_BUILD adj_exact
 #endif
 
@@ -170,10 +174,25 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
   }
 }
 
-TP_DEV double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// nturns whole turns backwards for one lane: s / l enter as the state and adjoint at the
+// end and leave as those at the start.  The strength gradient of every position with a
+// slot is summed over the wave and added to the wave's LDS row by lane 0.
+template <unsigned FEAT>
+TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
+                       const double* __restrict__ pool, const int32_t* __restrict__ slot_of_pos, int32_t n_seq,
+                       int32_t nturns, double beti, double* row, int lane) {
+  for (int32_t t = 0; t < nturns; ++t) {
+    for (int32_t e = n_seq - 1; e >= 0; --e) {
+      const tp_rec rec = recs[seq[e]];
+      double gacc = 0.0;
+      apply_elem_back<FEAT>(s, l, c, rec, pool, beti, gacc);
+      const int32_t slot = slot_of_pos[e];
+      if (slot >= 0) {
+        const double w = wave_sum(gacc);
+        if (lane == 0) row[slot] += w;
+      }
+    }
+  }
 }
 
 template <unsigned FEAT>
@@ -195,8 +214,8 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const int32_t* __restrict_
     const double* q = end_state + i;
     s = St{q[0], q[a.ld], q[2 * a.ld], q[3 * a.ld], q[4 * a.ld], q[5 * a.ld]};
     if (a.init_lambda) {
-      l.x = 2.0 * s.x * a.inv_na;
-      l.y = 2.0 * s.y * a.inv_na;
+      l.x = 2.0 * s.x;
+      l.y = 2.0 * s.y;
     } else {
       const double* r = lam + i;
       l = Ad{r[0], r[a.ld], r[2 * a.ld], r[3 * a.ld], r[4 * a.ld], r[5 * a.ld]};
@@ -204,18 +223,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const int32_t* __restrict_
   }
   Cx c;
   mkctx(s, c, a.beti);
-  for (int32_t t = 0; t < a.nturns; ++t) {
-    for (int32_t e = a.n_seq - 1; e >= 0; --e) {
-      const tp_rec rec = recs[seq[e]];
-      double gacc = 0.0;
-      apply_elem_back<FEAT>(s, l, c, rec, pool, a.beti, gacc);
-      const int32_t slot = slot_of_pos[e];
-      if (slot >= 0) {
-        const double w = wave_sum(gacc);
-        if (lane == 0) lds[wave * a.n_slots + slot] += w;
-      }
-    }
-  }
+  back_turns<FEAT>(s, l, c, seq, recs, pool, slot_of_pos, a.n_seq, a.nturns, a.beti, lds + wave * a.n_slots, lane);
   if (i < a.n) {
     double* r = lam + i;
     r[0] = l.x;
@@ -230,6 +238,90 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const int32_t* __restrict_
     double acc = 0.0;
     for (int w = 0; w < nw; ++w) acc += lds[w * a.n_slots + k];
     partial[(int64_t)blockIdx.x * a.n_slots + k] += acc;
+  }
+}
+
+// Batched persistent reverse pass: waves take (variant, tile) items from the queue, as in
+// the batched forward kernel that stored the checkpoints.  Each item walks its segments
+// backwards from the stored states, in the same per-lane order as adjoint_kernel, keeps
+// its strength gradient in the wave's own LDS row and writes the row to its item slot;
+// the per-variant sums are formed afterwards in tile order (batch_reduce_kernel).
+template <unsigned FEAT>
+__global__ __launch_bounds__(256) void batch_back_kernel(const TpVarDesc* __restrict__ desc,
+                                                         const int2* __restrict__ items, int32_t n_items,
+                                                         int32_t* __restrict__ counter, const int32_t* __restrict__ seq,
+                                                         const tp_rec* __restrict__ recs,
+                                                         const double* __restrict__ pool,
+                                                         const int32_t* __restrict__ slots,
+                                                         const double* __restrict__ ckpt,
+                                                         double* __restrict__ item_partial, int32_t max_slots) {
+  extern __shared__ double lds[];  // [waves per block][max_slots]
+  const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+  double* row = lds + wave * max_slots;
+  for (;;) {
+    int32_t it = 0;
+    if (lane == 0) it = atomicAdd(counter, 1);
+    it = __builtin_amdgcn_readfirstlane(it);
+    if (it >= n_items) break;
+    const int2 item = items[it];
+    const TpVarDesc& d = desc[item.x];
+    for (int k = lane; k < d.n_slots; k += 64) row[k] = 0.0;
+    const int64_t li = (int64_t)item.y * 64 + lane;
+    const bool alive = (li < d.n) && (d.lost_turn[li] == -1);
+    const double* ck = ckpt + d.ck_off + (int64_t)item.y * d.nseg * 384 + lane;
+    St s{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    Ad l{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    Cx c;
+    for (int32_t j = d.nseg - 1; j >= 0; --j) {
+      if (alive) {
+        const double* q = ck + (int64_t)j * 384;
+        s = St{q[0], q[64], q[128], q[192], q[256], q[320]};
+        if (j == d.nseg - 1) {
+          l.x = 2.0 * s.x;
+          l.y = 2.0 * s.y;
+        }
+      }
+      mkctx(s, c, d.beti);
+      const int32_t nt = (j == d.nseg - 1) ? d.nturns - j * d.ckpt_stride : d.ckpt_stride;
+      back_turns<FEAT>(s, l, c, seq + d.seq_off, recs + d.rec_off, pool + d.pool_off, slots + d.seq_off, d.n_seq, nt,
+                       d.beti, row, lane);
+    }
+    if (li < d.n) {
+      double* r = d.lam + li;
+      r[0] = l.x;
+      r[d.n] = l.px;
+      r[2 * d.n] = l.y;
+      r[3 * d.n] = l.py;
+      r[4 * d.n] = l.z;
+      r[5 * d.n] = l.d;
+    }
+    for (int k = lane; k < d.n_slots; k += 64) item_partial[(int64_t)it * max_slots + k] = row[k];
+  }
+}
+
+// One block per variant: its gradient slots summed over its tiles in tile order, and its
+// loss sum and survivor count likewise (the items of a variant are contiguous).
+__global__ void batch_reduce_kernel(const TpVarDesc* __restrict__ desc, const int32_t* __restrict__ vfirst,
+                                    const int32_t* __restrict__ vtiles, const double* __restrict__ item_partial,
+                                    int32_t max_slots, const double* __restrict__ item_loss, double* __restrict__ grad,
+                                    double* __restrict__ vloss) {
+  const int v = blockIdx.x;
+  const TpVarDesc& d = desc[v];
+  const int64_t first = vfirst[v];
+  const int32_t nt = vtiles[v];
+  for (int k = threadIdx.x; k < d.n_slots; k += blockDim.x) {
+    double acc = 0.0;
+    for (int32_t i = 0; i < nt; ++i) acc += item_partial[(first + i) * max_slots + k];
+    grad[d.slot_off + k] = acc;
+  }
+  if (threadIdx.x == 0) {
+    double ls = 0.0, na = 0.0;
+    for (int32_t i = 0; i < nt; ++i) {
+      ls += item_loss[2 * (first + i)];
+      na += item_loss[2 * (first + i) + 1];
+    }
+    vloss[2 * v] = ls;
+    vloss[2 * v + 1] = na;
   }
 }
 
@@ -249,8 +341,37 @@ static void launch_v(const TpAdjArgs& a, unsigned grid, hipStream_t stream) {
                      a.slot_of_pos, a.end_state, a.lam, a.lost_turn, a.partial, a);
 }
 
+template <unsigned FEAT>
+static void launch_back_v(const TpBatchArgs& b, hipStream_t stream) {
+  const size_t lds = (size_t)(b.block / 64) * (size_t)(b.max_slots > 0 ? b.max_slots : 1) * sizeof(double);
+  hipLaunchKernelGGL((batch_back_kernel<FEAT>), dim3(b.grid), dim3(b.block), lds, stream, b.desc, b.items, b.n_items,
+                     b.counter, b.seq, b.recs, b.pool, b.slots, b.ckpt, b.item_partial, b.max_slots);
+}
+
 }  // namespace TPK_BUILD
 }  // namespace tpk
+
+hipError_t TP_BATCH_BACK_LAUNCH(const TpBatchArgs& b, hipStream_t stream) {
+  if (b.n_items <= 0) return hipSuccess;
+  switch (b.variant) {
+    case TP_VARIANT_A: tpk::TPK_BUILD::launch_back_v<TP_FEAT_A>(b, stream); break;
+    case TP_VARIANT_B: tpk::TPK_BUILD::launch_back_v<TP_FEAT_B>(b, stream); break;
+    default: tpk::TPK_BUILD::launch_back_v<TP_FEAT_C>(b, stream); break;
+  }
+  return hipGetLastError();
+}
+
+int TP_BATCH_BACK_OCC(int variant, int block, int max_slots) {
+  const size_t lds = (size_t)(block / 64) * (size_t)(max_slots > 0 ? max_slots : 1) * sizeof(double);
+  const void* fn = nullptr;
+  switch (variant) {
+    case TP_VARIANT_A: fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::batch_back_kernel<TP_FEAT_A>); break;
+    case TP_VARIANT_B: fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::batch_back_kernel<TP_FEAT_B>); break;
+    default: fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::batch_back_kernel<TP_FEAT_C>); break;
+  }
+  int nb = 0;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, lds) == hipSuccess ? nb : 0;
+}
 
 hipError_t TP_ADJ_LAUNCH(const TpAdjArgs& a, hipStream_t stream) {
   if (a.n <= 0) return hipSuccess;
@@ -270,6 +391,15 @@ hipError_t tp_launch_grad_reduce(const double* partial, int nblocks, int n_slots
   if (n_slots <= 0) return hipSuccess;
   hipLaunchKernelGGL(tpk::TPK_BUILD::reduce_kernel, dim3((n_slots + 255) / 256), dim3(256), 0, stream, partial,
                      nblocks, n_slots, grad);
+  return hipGetLastError();
+}
+
+hipError_t tp_launch_batch_reduce(const TpVarDesc* desc, const int32_t* vfirst, const int32_t* vtiles, int32_t n_var,
+                                  const double* item_partial, int32_t max_slots, const double* item_loss, double* grad,
+                                  double* vloss, hipStream_t stream) {
+  if (n_var <= 0) return hipSuccess;
+  hipLaunchKernelGGL(tpk::TPK_BUILD::batch_reduce_kernel, dim3(n_var), dim3(256), 0, stream, desc, vfirst, vtiles,
+                     item_partial, max_slots, item_loss, grad, vloss);
   return hipGetLastError();
 }
 #endif

@@ -56,6 +56,50 @@ struct TpKArgs {
   int32_t variant;  // TP_VARIANT_*
 };
 
+// One lattice variant of a batch: where its element table sits in the concatenated
+// tables, its particle set, and its turn range.
+struct TpVarDesc {
+  int64_t seq_off;   // first sequence entry (also the first entry of slot_of_pos)
+  int64_t rec_off;   // first record
+  int64_t pool_off;  // first pool double
+  int64_t slot_off;  // first gradient slot of the variant in the concatenated gradient
+  double* coords;    // the variant's particle set: 6 planes of n doubles
+  int32_t* lost_turn;
+  int32_t* lost_elem;
+  double* lam;       // adjoint output (dL/dx0 before the 1/N_alive scale), 6 planes of n
+  int64_t n;
+  int64_t ck_off;    // first double of the variant's checkpoint slab (nseg x 6 x 64 per tile)
+  int32_t n_seq;
+  int32_t nturns;
+  int32_t turn0;
+  int32_t n_slots;
+  int32_t nseg;         // ceil(nturns / ckpt_stride)
+  int32_t ckpt_stride;
+  double beti;
+  double coord_limit;
+  double angle_limit;
+  double ap_r2;
+};
+
+struct TpBatchArgs {
+  const TpVarDesc* desc;
+  const int2* items;  // (variant, tile of 64 particles), sorted by decreasing cost
+  int32_t n_items;
+  int32_t* counter;   // work-queue head, zeroed before the launch
+  const int32_t* seq;  // concatenated tables of all variants
+  const tp_rec* recs;
+  const double* pool;
+  const int32_t* slots;  // concatenated slot_of_pos
+  double* ckpt;          // checkpoint slabs (adjoint)
+  double* item_loss;     // per item: loss sum, survivors (adjoint)
+  double* item_partial;  // per item: max_slots gradient sums (adjoint)
+  int32_t max_slots;
+  int32_t variant;
+  int32_t exact_h;
+  int32_t grid;
+  int32_t block;
+};
+
 // Arguments of one backward segment of the adjoint (tp_adjoint_kernels.hip).
 struct TpAdjArgs {
   const int32_t* seq;
@@ -71,16 +115,28 @@ struct TpAdjArgs {
   int32_t n_seq;
   int32_t n_slots;
   int32_t nturns;              // turns in this segment
-  int32_t init_lambda;         // 1: start from dL/dstate of the loss at end_state
+  int32_t init_lambda;         // 1: start from the gradient of the loss sum, 2x / 2y
   int32_t variant;
   int32_t pad;
   double beti;
-  double inv_na;               // 1 / number of surviving particles
 };
 
 // Host launchers exported by the kernel translation units (one per build flavour).
 hipError_t tp_launch_track_exact(const TpKArgs& a, int block, int ppl, hipStream_t stream);
 hipError_t tp_launch_track_fast(const TpKArgs& a, int block, int ppl, hipStream_t stream);
+hipError_t tp_launch_batch_exact(const TpBatchArgs& b, bool ckpt, hipStream_t stream);
+hipError_t tp_launch_batch_fast(const TpBatchArgs& b, bool ckpt, hipStream_t stream);
+int tp_batch_occupancy_exact(int variant, int exact_h, bool ckpt, int block);
+int tp_batch_occupancy_fast(int variant, int exact_h, bool ckpt, int block);
+hipError_t tp_launch_batch_back_exact(const TpBatchArgs& b, hipStream_t stream);
+hipError_t tp_launch_batch_back_fast(const TpBatchArgs& b, hipStream_t stream);
+int tp_batch_back_occupancy_exact(int variant, int block, int max_slots);
+int tp_batch_back_occupancy_fast(int variant, int block, int max_slots);
+// Per variant: gradient sums over its tiles in tile order, loss sum and survivors.
+hipError_t tp_launch_batch_reduce(const TpVarDesc* desc, const int32_t* vfirst, const int32_t* vtiles, int32_t n_var,
+                                  const double* item_partial, int32_t max_slots, const double* item_loss, double* grad,
+                                  double* vloss, hipStream_t stream);
+hipError_t tp_launch_fill_i32(int32_t* p, int32_t v, int32_t n, hipStream_t stream);
 hipError_t tp_launch_adjoint_exact(const TpAdjArgs& a, hipStream_t stream);
 hipError_t tp_launch_adjoint_fast(const TpAdjArgs& a, hipStream_t stream);
 hipError_t tp_launch_grad_reduce(const double* partial, int nblocks, int n_slots, double* grad, hipStream_t stream);

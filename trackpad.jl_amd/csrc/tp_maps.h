@@ -341,6 +341,14 @@ TP_DEV void ebody_driftlimit(St& s, const Cx& c, double L, double Lb) {
   s.z = s.z + NormL * c.B - Lb;
 }
 
+// Sum over the 64 lanes of a wave (butterfly; every lane gets the same total, formed in
+// the same order whatever the wave's position in the grid).
+TP_DEV double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 }  // namespace tpk
 
 #endif  // TP_MAPS_H

@@ -18,9 +18,13 @@
 
 #if TP_FAST
 #define TP_LAUNCH tp_launch_track_fast
+#define TP_BATCH_LAUNCH tp_launch_batch_fast
+#define TP_BATCH_OCC tp_batch_occupancy_fast
 #define TPK_BUILD fast
 #else
 #define TP_LAUNCH tp_launch_track_exact
+#define TP_BATCH_LAUNCH tp_launch_batch_exact
+#define TP_BATCH_OCC tp_batch_occupancy_exact
 #define TPK_BUILD exact
 #endif
 
@@ -331,31 +335,45 @@ TP_DEV bool is_lost(const St& s, const TpKArgs& a) {
   return l;
 }
 
-template <bool EXH, unsigned FEAT, int PPL>
-__global__ __launch_bounds__(256) void track_kernel(const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
-                                                    const double* __restrict__ pool, double* __restrict__ coords,
-                                                    int32_t* __restrict__ lost_turn, int32_t* __restrict__ lost_elem,
-                                                    TpKArgs a) {
-  const int64_t base = (int64_t)blockIdx.x * blockDim.x * PPL + threadIdx.x;
-  St s[PPL];
-  Cx c[PPL];
-  bool alive[PPL];
-  int64_t idx[PPL];
+// Loads PPL particles per lane (idx / valid per particle); lanes past the beam's end and
+// particles already lost start dead with a zero state.
+template <int PPL>
+TP_DEV void load_lanes(St* s, Cx* c, bool* alive, const double* __restrict__ coords,
+                       const int32_t* __restrict__ lost_turn, int64_t ld, double beti, const int64_t* idx,
+                       const bool* valid) {
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
-    idx[k] = base + (int64_t)k * blockDim.x;
     alive[k] = false;
     s[k] = St{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (idx[k] < a.n) {
+    if (valid[k]) {
       alive[k] = (lost_turn[idx[k]] == -1);
       if (alive[k]) {
         const double* q = coords + idx[k];
-        s[k] = St{q[0], q[a.ld], q[2 * a.ld], q[3 * a.ld], q[4 * a.ld], q[5 * a.ld]};
+        s[k] = St{q[0], q[ld], q[2 * ld], q[3 * ld], q[4 * ld], q[5 * ld]};
       }
     }
-    mkctx(s[k], c[k], a.beti);
+    mkctx(s[k], c[k], beti);
   }
-  for (int32_t t = 0; t < a.nturns; ++t) {
+}
+
+TP_DEV void store_state(double* q, int64_t ld, const St& s) {
+  q[0] = s.x;
+  q[ld] = s.px;
+  q[2 * ld] = s.y;
+  q[3 * ld] = s.py;
+  q[4 * ld] = s.z;
+  q[5 * ld] = s.d;
+}
+
+// Turns [t_begin, t_end) of the ring for PPL particles per lane.  A particle found lost
+// after an element has its entry state, turn (a.turn0 + t) and position written out and
+// is dead from then on; the wave leaves early once every particle in it is dead.
+template <bool EXH, unsigned FEAT, int PPL>
+TP_DEV void run_turns(St* s, Cx* c, bool* alive, const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
+                      const double* __restrict__ pool, double* __restrict__ coords, int32_t* __restrict__ lost_turn,
+                      int32_t* __restrict__ lost_elem, const TpKArgs& a, const int64_t* idx, int32_t t_begin,
+                      int32_t t_end) {
+  for (int32_t t = t_begin; t < t_end; ++t) {
     bool anyl = false;
 #pragma unroll
     for (int k = 0; k < PPL; ++k) anyl = anyl || alive[k];
@@ -377,13 +395,7 @@ __global__ __launch_bounds__(256) void track_kernel(const int32_t* __restrict__ 
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
           if (lostnow[k]) {
-            double* q = coords + idx[k];
-            q[0] = s0[k].x;
-            q[a.ld] = s0[k].px;
-            q[2 * a.ld] = s0[k].y;
-            q[3 * a.ld] = s0[k].py;
-            q[4 * a.ld] = s0[k].z;
-            q[5 * a.ld] = s0[k].d;
+            store_state(coords + idx[k], a.ld, s0[k]);
             lost_turn[idx[k]] = a.turn0 + t;
             lost_elem[idx[k]] = e;
             alive[k] = false;
@@ -392,18 +404,98 @@ __global__ __launch_bounds__(256) void track_kernel(const int32_t* __restrict__ 
       }
     }
   }
+}
+
+template <bool EXH, unsigned FEAT, int PPL>
+__global__ __launch_bounds__(256) void track_kernel(const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
+                                                    const double* __restrict__ pool, double* __restrict__ coords,
+                                                    int32_t* __restrict__ lost_turn, int32_t* __restrict__ lost_elem,
+                                                    TpKArgs a) {
+  const int64_t base = (int64_t)blockIdx.x * blockDim.x * PPL + threadIdx.x;
+  int64_t idx[PPL];
+  bool valid[PPL];
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
-    if (alive[k]) {
-      double* q = coords + idx[k];
-      q[0] = s[k].x;
-      q[a.ld] = s[k].px;
-      q[2 * a.ld] = s[k].y;
-      q[3 * a.ld] = s[k].py;
-      q[4 * a.ld] = s[k].z;
-      q[5 * a.ld] = s[k].d;
-    }
+    idx[k] = base + (int64_t)k * blockDim.x;
+    valid[k] = idx[k] < a.n;
   }
+  St s[PPL];
+  Cx c[PPL];
+  bool alive[PPL];
+  load_lanes<PPL>(s, c, alive, coords, lost_turn, a.ld, a.beti, idx, valid);
+  run_turns<EXH, FEAT, PPL>(s, c, alive, seq, recs, pool, coords, lost_turn, lost_elem, a, idx, 0, a.nturns);
+#pragma unroll
+  for (int k = 0; k < PPL; ++k)
+    if (alive[k]) store_state(coords + idx[k], a.ld, s[k]);
+}
+
+// Batched persistent kernel: every wave takes work items -- (lattice variant, tile of 64
+// particles), sorted by decreasing cost -- from an atomic queue until it is empty.  The
+// item index is made wave-uniform with readfirstlane, so the variant's descriptor and
+// its element table (a slice of the concatenated tables) are read with scalar loads, as
+// in the one-beam kernel.  With CKPT the wave also stores its tile's state every
+// ckpt_stride turns (and at the last turn) into the variant's checkpoint slab, and the
+// tile's loss sum and survivor count, for the batched adjoint.
+template <bool EXH, unsigned FEAT, bool CKPT>
+__global__ __launch_bounds__(256) void batch_kernel(const TpVarDesc* __restrict__ desc, const int2* __restrict__ items,
+                                                    int32_t n_items, int32_t* __restrict__ counter,
+                                                    const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
+                                                    const double* __restrict__ pool, double* __restrict__ ckpt,
+                                                    double* __restrict__ item_loss) {
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int32_t it = 0;
+    if (lane == 0) it = atomicAdd(counter, 1);
+    it = __builtin_amdgcn_readfirstlane(it);
+    if (it >= n_items) break;
+    const int2 item = items[it];
+    const TpVarDesc& d = desc[item.x];
+    TpKArgs a{};
+    a.n_seq = d.n_seq;
+    a.beti = d.beti;
+    a.coord_limit = d.coord_limit;
+    a.angle_limit = d.angle_limit;
+    a.ap_r2 = d.ap_r2;
+    a.ld = d.n;
+    a.turn0 = d.turn0;
+    a.nturns = d.nturns;
+    const int32_t* vseq = seq + d.seq_off;
+    const tp_rec* vrecs = recs + d.rec_off;
+    const double* vpool = pool + d.pool_off;
+    const int64_t li = (int64_t)item.y * 64 + lane;
+    int64_t idx[1] = {li};
+    bool valid[1] = {li < d.n};
+    St s[1];
+    Cx c[1];
+    bool alive[1];
+    load_lanes<1>(s, c, alive, d.coords, d.lost_turn, d.n, d.beti, idx, valid);
+    if constexpr (CKPT) {
+      double* ck = ckpt + d.ck_off + (int64_t)item.y * d.nseg * 384 + lane;
+      for (int32_t j = 0; j < d.nseg; ++j) {
+        const int32_t tb = j * d.ckpt_stride;
+        const int32_t te = (j == d.nseg - 1) ? d.nturns : tb + d.ckpt_stride;
+        run_turns<EXH, FEAT, 1>(s, c, alive, vseq, vrecs, vpool, d.coords, d.lost_turn, d.lost_elem, a, idx, tb, te);
+        store_state(ck + (int64_t)j * 384, 64, s[0]);
+      }
+      const double l2 = alive[0] ? s[0].x * s[0].x + s[0].y * s[0].y : 0.0;
+      const double lsum = wave_sum(l2);
+      const double cnt = (double)__popcll(__ballot(alive[0]));
+      if (lane == 0) {
+        item_loss[2 * (int64_t)it] = lsum;
+        item_loss[2 * (int64_t)it + 1] = cnt;
+      }
+    } else {
+      run_turns<EXH, FEAT, 1>(s, c, alive, vseq, vrecs, vpool, d.coords, d.lost_turn, d.lost_elem, a, idx, 0,
+                              d.nturns);
+    }
+    if (alive[0]) store_state(d.coords + li, d.n, s[0]);
+  }
+}
+
+// Clears the work-queue counters inside a captured graph (no memset nodes in graphs).
+__global__ void fill_i32(int32_t* __restrict__ p, int32_t v, int32_t n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 }  // namespace TPK_BUILD
@@ -417,6 +509,59 @@ static void launch_v(const TpKArgs& a, int ppl, unsigned grid, int block, hipStr
     hipLaunchKernelGGL((tpk::TPK_BUILD::track_kernel<EXH, FEAT, 2>), dim3(grid), dim3(block), 0, stream, TP_KPTRS(a), a);
   else
     hipLaunchKernelGGL((tpk::TPK_BUILD::track_kernel<EXH, FEAT, 1>), dim3(grid), dim3(block), 0, stream, TP_KPTRS(a), a);
+}
+
+template <bool EXH, unsigned FEAT, bool CKPT>
+static const void* batch_fn() {
+  return reinterpret_cast<const void*>(tpk::TPK_BUILD::batch_kernel<EXH, FEAT, CKPT>);
+}
+
+template <bool EXH, unsigned FEAT, bool CKPT>
+static void launch_batch_v(const TpBatchArgs& b, hipStream_t stream) {
+  hipLaunchKernelGGL((tpk::TPK_BUILD::batch_kernel<EXH, FEAT, CKPT>), dim3(b.grid), dim3(b.block), 0, stream, b.desc,
+                     b.items, b.n_items, b.counter, b.seq, b.recs, b.pool, b.ckpt, b.item_loss);
+}
+
+#define TP_BATCH_DISPATCH(CALL)                               \
+  switch (variant) {                                          \
+    case TP_VARIANT_A: CALL(true, TP_FEAT_A); break;          \
+    case TP_VARIANT_B: CALL(true, TP_FEAT_B); break;          \
+    case TP_VARIANT_C: CALL(true, TP_FEAT_C); break;          \
+    default:                                                  \
+      if (exact_h) { CALL(true, TP_FEAT_ALL); }               \
+      else { CALL(false, TP_FEAT_ALL); }                      \
+      break;                                                  \
+  }
+
+hipError_t TP_BATCH_LAUNCH(const TpBatchArgs& b, bool ckpt, hipStream_t stream) {
+  if (b.n_items <= 0) return hipSuccess;
+  const int variant = b.variant, exact_h = b.exact_h;
+  if (ckpt) {
+#define TP_CALL(E, F) launch_batch_v<E, F, true>(b, stream)
+    TP_BATCH_DISPATCH(TP_CALL)
+#undef TP_CALL
+  } else {
+#define TP_CALL(E, F) launch_batch_v<E, F, false>(b, stream)
+    TP_BATCH_DISPATCH(TP_CALL)
+#undef TP_CALL
+  }
+  return hipGetLastError();
+}
+
+// Resident blocks per CU of the batched kernel, for sizing the persistent grid.
+int TP_BATCH_OCC(int variant, int exact_h, bool ckpt, int block) {
+  const void* fn = nullptr;
+  if (ckpt) {
+#define TP_CALL(E, F) fn = batch_fn<E, F, true>()
+    TP_BATCH_DISPATCH(TP_CALL)
+#undef TP_CALL
+  } else {
+#define TP_CALL(E, F) fn = batch_fn<E, F, false>()
+    TP_BATCH_DISPATCH(TP_CALL)
+#undef TP_CALL
+  }
+  int nb = 0;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) == hipSuccess ? nb : 0;
 }
 
 hipError_t TP_LAUNCH(const TpKArgs& a, int block, int ppl, hipStream_t stream) {
@@ -436,3 +581,11 @@ hipError_t TP_LAUNCH(const TpKArgs& a, int block, int ppl, hipStream_t stream) {
   }
   return hipGetLastError();
 }
+
+#if TP_FAST
+hipError_t tp_launch_fill_i32(int32_t* p, int32_t v, int32_t n, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(tpk::TPK_BUILD::fill_i32, dim3((n + 255) / 256), dim3(256), 0, stream, p, v, n);
+  return hipGetLastError();
+}
+#endif

@@ -21,6 +21,7 @@ from .tracking import (  # noqa: F401
 )
 
 from .adjoint import Gradients, ringpass_grad, set_strengths, strength_positions  # noqa: F401
+from .batch import BatchGradients, HIPBatch  # noqa: F401
 
 __version__ = "0.1.0"
 
