@@ -16,7 +16,14 @@ The JSON line carries:
   cpu_baseline   the oracle (the C++ restatement of the reference's CPU path, same loop
                  order) on this host's cores, on a bounded sample of the same workload
 
-Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--workload dba|fodo]
+Workload ``batched``: 256 variants of the DBA ring (strengths x (1 + N(0, 1%)), seeded),
+N_v log-uniform in [1k, 64k], T_v uniform in [100, 2000]; one step is one graph launch of
+the persistent batched kernel over all variants.  The line adds the batched adjoint (one
+graph launch: checkpointed forward, reverse pass, per-variant reduction) and its cost
+relative to the forward, and both against 256 plain per-variant launches on the same GPU
+(measured on every variant with the turn counts scaled by --compare-turn-scale).
+
+Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--workload dba|fodo|batched]
                        [--kernel fast|exact] [--turns-per-launch 100]
 """
 
@@ -80,18 +87,197 @@ def cpu_baseline(lat, beam, mk, aperture, n_sample, t_sample, threads):
     return n_sample * t_sample / dt, dt
 
 
+def live_turns_(lt, le, n_seq, T):
+    """Particle-turns actually tracked: a particle lost at position e of turn t ran t turns
+    and e + 1 elements of the next."""
+    return float(np.where(lt >= 0, lt + (le + 1) / n_seq, T).sum())
+
+
+def bench_batched(args, fast):
+    from trackpad_hip import Beam, HIPBatch, HIPContext, HIPParticles, compile_table
+    from trackpad_hip import workloads as W
+    from trackpad_hip.flops import flops_per_turn
+    import trackpad_hip._lib as L
+
+    beam = Beam(W.ENERGY)
+    nvar = args.variants
+    rings, ns, ts = W.variant_rings(nvar=nvar, seed=11)
+    if args.turn_scale != 1.0:
+        ts = np.maximum(1, np.round(ts * args.turn_scale)).astype(np.int64)
+    ctx = HIPContext(0)
+    ctx.set_waves(args.fwd_waves, args.back_waves)
+    tabs = [ctx.table(compile_table(r, beam)) for r in rings]
+    c0 = [W.dba_particles(int(n), seed=1000 + v) for v, n in enumerate(ns)]
+    parts = [HIPParticles.from_coords(c, ctx=ctx) for c in c0]
+    init = [HIPParticles.from_coords(c, ctx=ctx) for c in c0]
+    b = HIPBatch(tabs, parts, ctx=ctx)
+    b.set_launch(args.block, args.fwd_waves, args.back_waves)
+    n_seq = len(rings[0])
+    fpt = flops_per_turn(rings[0])
+    sched = float(np.sum(ns * ts))
+    print(f"[bench] batched: {nvar} variants, sum N = {int(ns.sum())}, T in [{ts.min()}, {ts.max()}], "
+          f"sum N*T = {sched:.3e}, kernel={args.kernel}", file=sys.stderr, flush=True)
+    K = args.ckpt_stride
+
+    def reset():
+        for p, q in zip(parts, init):
+            p.copy_from(q)
+        ctx.sync()
+
+    # preparation: load the kernels (a 1-turn pass of every variant), then restore the beams
+    b.track(np.ones(nvar, np.int32), fast=fast)
+    reset()
+
+    def fwd_step():
+        b.track(ts.astype(np.int32), fast=fast, timed=True)
+        return ctx.timing()["kernel_ms"]
+
+    for i in range(args.warmup):
+        fwd_step()
+        reset()
+        print(f"[bench] warmup step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    kernel_ms = 0.0
+    wall = 0.0
+    for i in range(args.steps):
+        reset()
+        t0 = time.perf_counter()
+        kernel_ms += fwd_step()
+        wall += time.perf_counter() - t0
+        print(f"[bench] batched forward step {i + 1}/{args.steps}: {wall * 1e3 / (i + 1):.0f} ms", file=sys.stderr,
+              flush=True)
+    info_f = b.info()
+    live = 0.0
+    for p, t in zip(parts, ts):
+        _, _, lt, le = p.download()
+        live += live_turns_(lt, le, n_seq, int(t))
+    avg_s = kernel_ms / 1e3 / args.steps
+    achieved = fpt * live / avg_s
+
+    # batched adjoint, one graph launch
+    reset()
+    t0 = time.perf_counter()
+    g = b.adjoint(ts.astype(np.int32), ckpt_stride=K, fast=fast, timed=True)
+    adj_wall = time.perf_counter() - t0
+    adj_ms = ctx.timing()["kernel_ms"]
+    info_a = b.info()
+    print(f"[bench] batched adjoint: {adj_ms:.0f} ms (graph), {adj_wall * 1e3:.0f} ms wall, "
+          f"checkpoints {info_a['ckpt_bytes'] / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+
+    if args.compare_turn_scale <= 0:
+        bf = pf = ba = pa = float("nan")
+    else:
+        bf, pf, ba, pa = compare_per_variant(args, fast, b, ctx, parts, tabs, ts, ns, n_seq, reset, K, L)
+    out = batched_line(args, nvar, ns, ts, sched, wall, n_seq, info_f, fpt, live, avg_s, achieved, adj_ms,
+                       adj_wall, K, info_a, g, bf, pf, ba, pa)
+    if not args.no_cpu_baseline:
+        ns_, ts_ = (int(v) for v in args.cpu_sample.lower().split("x"))
+        cores = host_cores()
+        mk = lambda n_: W.dba_particles(n_, seed=1000)
+        v_all, dt_all = cpu_baseline(rings[0], beam, mk, None, ns_, ts_, cores)
+        out["cpu_baseline"] = {"value": v_all, "unit": "particle-turns/s", "cores": cores, "kind": "port",
+                               "sample": f"variant 0, {ns_} particles x {ts_} turns ({dt_all:.1f} s), oracle C++ "
+                                         "restatement, OpenMP over particles"}
+    print(json.dumps(out), flush=True)
+
+
+def compare_per_variant(args, fast, b, ctx, parts, tabs, ts, ns, n_seq, reset, K, L):
+    """The same variants as 256 plain launches (tp_track / tp_adjoint one variant after the
+    other on the library's stream), with every turn count scaled by --compare-turn-scale."""
+    import ctypes as C
+
+    sc = args.compare_turn_scale
+    tc = np.maximum(1, np.round(ts * sc)).astype(np.int32)
+    reset()
+    b.track(tc, fast=fast)  # plan + capture outside the timed region
+    reset()
+    t0 = time.perf_counter()
+    b.track(tc, fast=fast)
+    bf = time.perf_counter() - t0
+    reset()
+    t0 = time.perf_counter()
+    for p, tab, t in zip(parts, tabs, tc):
+        L.check(ctx._lib.tp_track(ctx.handle, tab.handle, p.handle, 0, int(t),
+                                  (L.TP_TRACK_FAST if fast else 0) | L.TP_TRACK_ASYNC))
+    ctx.sync()
+    pf = time.perf_counter() - t0
+    print(f"[bench] compare forward (turns x {sc}): batched {bf * 1e3:.0f} ms, per-variant {pf * 1e3:.0f} ms",
+          file=sys.stderr, flush=True)
+    reset()
+    b.adjoint(tc, ckpt_stride=K, fast=fast)  # plan + capture
+    reset()
+    t0 = time.perf_counter()
+    b.adjoint(tc, ckpt_stride=K, fast=fast)
+    ba = time.perf_counter() - t0
+    reset()
+    loss, na = C.c_double(), C.c_int64()
+    dk = np.zeros(n_seq)
+    dx = np.zeros(6 * int(ns.max()))
+    pd = C.POINTER(C.c_double)
+    t0 = time.perf_counter()
+    for p, tab, t in zip(parts, tabs, tc):
+        L.check(ctx._lib.tp_adjoint(ctx.handle, tab.handle, p.handle, 0, int(t), K, L.TP_TRACK_FAST if fast else 0,
+                                    C.byref(loss), C.byref(na), dk.ctypes.data_as(pd), dx.ctypes.data_as(pd)))
+    pa = time.perf_counter() - t0
+    print(f"[bench] compare adjoint (turns x {sc}): batched {ba * 1e3:.0f} ms, per-variant {pa * 1e3:.0f} ms",
+          file=sys.stderr, flush=True)
+    return bf, pf, ba, pa
+
+
+def batched_line(args, nvar, ns, ts, sched, wall, n_seq, info_f, fpt, live, avg_s, achieved, adj_ms, adj_wall, K,
+                 info_a, g, bf, pf, ba, pa):
+    return {
+        "metric": "particle-turns/s FP64 (1 GPU, % FP64 peak); batched adjoint/forward cost ratio",
+        "value": sched * args.steps / wall,
+        "unit": "particle-turns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian beams, seeded strength perturbations)",
+        "config": {"workload": f"batched_dba_{nvar}_variants", "variants": nvar, "particles_total": int(ns.sum()),
+                   "turns_min": int(ts.min()), "turns_max": int(ts.max()), "particle_turns": sched,
+                   "elements_per_turn": n_seq, "kernel": args.kernel, "block": args.block,
+                   "fwd_waves": args.fwd_waves, "back_waves": args.back_waves,
+                   "work_items": info_f["n_items"], "grid_blocks": info_f["grid_fwd"],
+                   "flops_per_particle_turn": fpt, "turn_scale": args.turn_scale},
+        "live_particle_turns_per_s": live / avg_s,
+        "roofline": {"bound": "mfma", "pipe": "vector FP64 VALU (no MFMA: not a contraction); FP64 peak 78.6 TFLOP/s",
+                     "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK, "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": args.steps},
+        "batched_adjoint": {"graph_ms": adj_ms, "wall_ms": adj_wall * 1e3, "forward_graph_ms": avg_s * 1e3,
+                            "adjoint_over_forward": adj_ms / (avg_s * 1e3), "ckpt_stride": K,
+                            "ckpt_bytes": info_a["ckpt_bytes"], "grid_back_blocks": info_a["grid_back"],
+                            "mean_loss": float(np.mean(g.loss)), "survivors": int(g.n_alive.sum())},
+        "vs_per_variant_launches": {"turn_scale": args.compare_turn_scale, "forward_batched_ms": bf * 1e3,
+                                    "forward_per_variant_ms": pf * 1e3, "forward_speedup": pf / bf,
+                                    "adjoint_batched_ms": ba * 1e3, "adjoint_per_variant_ms": pa * 1e3,
+                                    "adjoint_speedup": pa / ba},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="dba", choices=["dba", "fodo"])
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 2; batched 1)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 1; batched 0)")
+    ap.add_argument("--workload", default="dba", choices=["dba", "fodo", "batched"])
+    ap.add_argument("--variants", type=int, default=256, help="batched: number of lattice variants")
+    ap.add_argument("--turn-scale", type=float, default=1.0, help="batched: scale every T_v (1 = the workload)")
+    ap.add_argument("--compare-turn-scale", type=float, default=0.05,
+                    help="batched: turn scale of the comparison with per-variant launches")
+    ap.add_argument("--ckpt-stride", type=int, default=10, help="batched adjoint: turns between checkpoints")
     ap.add_argument("--kernel", default="fast", choices=["fast", "exact"])
     ap.add_argument("--particles", type=int, default=0, help="override N (default: the workload's)")
     ap.add_argument("--turns", type=int, default=0, help="override turns per step (default: the workload's)")
     ap.add_argument("--turns-per-launch", type=int, default=100)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--ppl", type=int, default=1)
+    ap.add_argument("--fwd-waves", type=int, default=0, help="register budget of the tracking kernels (0, 5, 6)")
+    ap.add_argument("--back-waves", type=int, default=0, help="register budget of the reverse kernels (0, 3, 4)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", default="16384x20", help="particles x turns of the CPU baseline sample")
     args = ap.parse_args()
@@ -100,6 +286,14 @@ def main():
         print("bench.py: this build runs on one GPU only (the tracking path shards by particles; "
               "multi-GPU runs are not part of it). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    batched = args.workload == "batched"
+    if args.steps is None:
+        args.steps = 1 if batched else 2
+    if args.warmup is None:
+        args.warmup = 0 if batched else 1
+    if batched:
+        bench_batched(args, args.kernel == "fast")
+        return
 
     from trackpad_hip import HIPContext, HIPParticles, compile_table, track_
     from trackpad_hip.flops import flops_per_turn
@@ -108,6 +302,7 @@ def main():
     fast = args.kernel == "fast"
     ctx = HIPContext(0)
     ctx.set_launch(args.block, args.ppl, args.turns_per_launch)
+    ctx.set_waves(args.fwd_waves, args.back_waves)
     tab = ctx.table(compile_table(lat, beam, aperture=aperture))
     c0 = mk(n)
     init = HIPParticles.from_coords(c0, ctx=ctx)
@@ -141,7 +336,7 @@ def main():
     fpt = flops_per_turn(lat)
     n_seq = len(lat)
     lost = lt >= 0
-    live_turns = np.where(lost, lt + (le + 1) / n_seq, T).sum()  # particle-turns actually tracked
+    live_turns = live_turns_(lt, le, n_seq, T)
     sched = float(n) * T * args.steps
     value = sched / wall
     live = float(live_turns) * args.steps / wall
@@ -164,7 +359,8 @@ def main():
         "data": "synthetic (seeded Gaussian beam)",
         "config": {"workload": f"{args.workload}_{n}x{T}", "particles": n, "turns": T, "elements_per_turn": n_seq,
                    "kernel": args.kernel, "block": args.block, "particles_per_lane": args.ppl,
-                   "turns_per_launch": args.turns_per_launch, "flops_per_particle_turn": fpt},
+                   "turns_per_launch": args.turns_per_launch, "fwd_waves": args.fwd_waves,
+                   "flops_per_particle_turn": fpt},
         "live_particle_turns_per_s": live,
         "lost_particles": int(lost.sum()),
         "roofline": {"bound": "mfma", "pipe": "vector FP64 VALU (no MFMA: not a contraction); FP64 peak 78.6 TFLOP/s",

@@ -89,6 +89,14 @@ int tp_ctx_destroy(tp_ctx* ctx);
  * (0 = all turns in one launch). */
 int tp_set_launch(tp_ctx* ctx, int block, int ppl, int turns_per_launch);
 
+/* Register budget of the kernels used by later calls, as resident waves per SIMD the
+ * register allocator must allow (it spills to scratch if it must): fwd_waves 0 (the
+ * compiler's choice), 5 or 6 for the tracking kernels; back_waves 0, 3 or 4 for the
+ * reverse (adjoint) kernels.  Budgets are compiled for the one-particle-per-lane kernels
+ * of lattices made of drifts, multipoles, bends and RF cavities; other kernels keep the
+ * compiler's budget.  Results do not depend on the budget.  No reference counterpart. */
+int tp_set_waves(tp_ctx* ctx, int fwd_waves, int back_waves);
+
 /* Kernel timing of the last call made with TP_TRACK_TIMED. */
 int tp_timing(tp_ctx* ctx, tp_timing_t* out);
 
@@ -178,9 +186,10 @@ int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_part
                     tp_batch** out);
 int tp_batch_destroy(tp_batch* b);
 
-/* Work-group size of the persistent kernels (64..256, multiple of 64; 0 = 256) and waves
- * per SIMD to keep resident (0 = as many as the kernel's registers allow). */
-int tp_batch_set_launch(tp_batch* b, int block, int waves_per_simd);
+/* Work-group size of the persistent kernels (64..256, multiple of 64; 0 = 256) and their
+ * register budgets, as for tp_set_waves.  The persistent grids hold as many blocks as stay
+ * resident on every CU. */
+int tp_batch_set_launch(tp_batch* b, int block, int fwd_waves, int back_waves);
 
 /* Track variant v from turn turn0[v] (NULL = all 0) for nturns[v] turns, as tp_track on
  * each (table, particle set) pair would.  flags: TP_TRACK_FAST, TP_TRACK_ASYNC,

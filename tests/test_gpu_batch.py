@@ -147,7 +147,7 @@ def test_batch_many_ragged_variants(ctx):
     tabs = [ctx.table(compile_table(l, beam)) for l in lats]
     parts = [HIPParticles.from_coords(c, None, ctx) for c in coords]
     b = HIPBatch(tabs, parts)
-    b.set_launch(128, 2)
+    b.set_launch(128, 5, 3)
     b.track(ts)
     ref = _per_variant_track(ctx, tabs, coords, [None] * nv, ts, np.zeros(nv, np.int32), False)
     for v, (p, r) in enumerate(zip(parts, ref)):

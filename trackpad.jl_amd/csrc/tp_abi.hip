@@ -30,6 +30,7 @@ struct tp_ctx {
   int block = 256;
   int ppl = 1;
   int tpl = 0;  // turns per launch, 0 = all
+  int fwd_waves = 0, back_waves = 0;  // register budgets (0 = the compiler's)
   tp_timing_t timing{};
 };
 
@@ -208,6 +209,17 @@ int tp_set_launch(tp_ctx* ctx, int block, int ppl, int turns_per_launch) {
   ctx->block = block;
   ctx->ppl = ppl;
   ctx->tpl = turns_per_launch;
+  return TP_OK;
+}
+
+int tp_set_waves(tp_ctx* ctx, int fwd_waves, int back_waves) {
+  if (!ctx || ctx->closed) return fail(TP_EINVAL, "tp_set_waves: ctx is NULL or destroyed");
+  if (fwd_waves != 0 && fwd_waves != 5 && fwd_waves != 6)
+    return fail(TP_EINVAL, "tp_set_waves: fwd_waves %d not in {0, 5, 6}", fwd_waves);
+  if (back_waves != 0 && back_waves != 3 && back_waves != 4)
+    return fail(TP_EINVAL, "tp_set_waves: back_waves %d not in {0, 3, 4}", back_waves);
+  ctx->fwd_waves = fwd_waves;
+  ctx->back_waves = back_waves;
   return TP_OK;
 }
 
@@ -490,6 +502,7 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
   a.lost_turn = p->d_lt;
   a.lost_elem = p->d_le;
   a.variant = t->variant;
+  a.waves = ctx->fwd_waves;
   const bool timed = (flags & TP_TRACK_TIMED) != 0;
   const int chunk = ctx->tpl > 0 ? ctx->tpl : (nturns > 0 ? nturns : 1);
   double ms_total = 0.0;
@@ -656,6 +669,7 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
   a.n_seq = (int32_t)t->hdr.n_seq;
   a.n_slots = n_slots;
   a.variant = t->variant;
+  a.waves = ctx->back_waves;
   a.beti = t->hdr.beti;
   // the reverse pass seeds dL/dx = 2x (the gradient of the loss sum); the 1/N_alive of the
   // mean is applied to the results below
@@ -725,12 +739,12 @@ struct tp_batch {
   tp_rec* d_recs = nullptr;
   double* d_pool = nullptr;
   int32_t* d_slots = nullptr;
-  int block = 256, waves_per_simd = 0, cus = 0;
+  int block = 256, fwd_waves = 0, back_waves = 0, cus = 0;
   // the planned launch (rebuilt when its key changes)
   int mode = 0;  // 0 none, 1 track, 2 adjoint
   uint32_t pflags = 0;
   int32_t pstride = 0;
-  int pblock = 0, pwps = 0;
+  int pblock = 0, pfw = 0, pbw = 0;
   std::vector<int32_t> pturn0, pnturns;
   int32_t n_items = 0, grid_f = 0, grid_b = 0;
   int64_t ckpt_bytes = 0;
@@ -764,10 +778,10 @@ void batch_free_plan(tp_batch* b) {
 
 hipError_t dalloc(void** p, size_t bytes) { return hipMalloc(p, bytes > 0 ? bytes : 8); }
 
-int pick_grid(int occ_per_cu, int cus, int wpb, int waves_per_simd, int32_t n_items) {
-  int per_cu = occ_per_cu;
-  if (waves_per_simd > 0) per_cu = std::min(per_cu, std::max(1, waves_per_simd * 4 / wpb));
-  if (per_cu < 1) per_cu = 1;
+// Persistent grid: as many blocks as stay resident (occupancy x CUs), but no more than
+// the items can feed.
+int pick_grid(int occ_per_cu, int cus, int wpb, int32_t n_items) {
+  const int per_cu = occ_per_cu < 1 ? 1 : occ_per_cu;
   const int64_t need = ((int64_t)n_items + wpb - 1) / wpb;
   return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * cus, need));
 }
@@ -780,7 +794,7 @@ int batch_plan(tp_batch* b, int mode, const int32_t* turn0, const int32_t* nturn
   std::vector<int32_t> t0(nv, 0), nt(nturns, nturns + nv);
   if (turn0) t0.assign(turn0, turn0 + nv);
   if (b->mode == mode && b->pflags == flags && b->pstride == stride && b->pblock == b->block &&
-      b->pwps == b->waves_per_simd && b->pturn0 == t0 && b->pnturns == nt)
+      b->pfw == b->fwd_waves && b->pbw == b->back_waves && b->pturn0 == t0 && b->pnturns == nt)
     return TP_OK;
   batch_free_plan(b);
   struct Item {
@@ -889,25 +903,27 @@ int batch_plan(tp_batch* b, int mode, const int32_t* turn0, const int32_t* nturn
   a.variant = b->variant;
   a.exact_h = b->exact_h;
   a.block = b->block;
-  const int occ_f = fast ? tp_batch_occupancy_fast(b->variant, b->exact_h, adj, b->block)
-                         : tp_batch_occupancy_exact(b->variant, b->exact_h, adj, b->block);
+  const int occ_f = fast ? tp_batch_occupancy_fast(b->variant, b->exact_h, adj, b->block, b->fwd_waves)
+                         : tp_batch_occupancy_exact(b->variant, b->exact_h, adj, b->block, b->fwd_waves);
   if (occ_f < 1) return fail(TP_EHIP, "batch: the forward kernel does not fit a CU with %d threads", b->block);
-  b->grid_f = pick_grid(occ_f, b->cus, wpb, b->waves_per_simd, n_items);
+  b->grid_f = pick_grid(occ_f, b->cus, wpb, n_items);
   if (adj) {
-    const int occ_b = fast ? tp_batch_back_occupancy_fast(b->variant, b->block, b->max_slots)
-                           : tp_batch_back_occupancy_exact(b->variant, b->block, b->max_slots);
+    const int occ_b = fast ? tp_batch_back_occupancy_fast(b->variant, b->block, b->max_slots, b->back_waves)
+                           : tp_batch_back_occupancy_exact(b->variant, b->block, b->max_slots, b->back_waves);
     if (occ_b < 1) return fail(TP_EHIP, "batch: the reverse kernel does not fit a CU with %d threads", b->block);
-    b->grid_b = pick_grid(occ_b, b->cus, wpb, b->waves_per_simd, n_items);
+    b->grid_b = pick_grid(occ_b, b->cus, wpb, n_items);
   }
   if (n_items > 0) {
     TP_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
     hipError_t le = tp_launch_fill_i32((int32_t*)b->d_counter, 0, 2, st);
     a.counter = (int32_t*)b->d_counter;
     a.grid = b->grid_f;
+    a.waves = b->fwd_waves;
     if (le == hipSuccess) le = fast ? tp_launch_batch_fast(a, adj, st) : tp_launch_batch_exact(a, adj, st);
     if (adj) {
       a.counter = (int32_t*)b->d_counter + 1;
       a.grid = b->grid_b;
+      a.waves = b->back_waves;
       if (le == hipSuccess) le = fast ? tp_launch_batch_back_fast(a, st) : tp_launch_batch_back_exact(a, st);
       if (le == hipSuccess)
         le = tp_launch_batch_reduce(a.desc, (const int32_t*)b->d_vfirst, (const int32_t*)b->d_vtiles, nv,
@@ -934,7 +950,8 @@ int batch_plan(tp_batch* b, int mode, const int32_t* turn0, const int32_t* nturn
   b->pflags = flags;
   b->pstride = stride;
   b->pblock = b->block;
-  b->pwps = b->waves_per_simd;
+  b->pfw = b->fwd_waves;
+  b->pbw = b->back_waves;
   b->pturn0 = t0;
   b->pnturns = nt;
   return TP_OK;
@@ -1072,15 +1089,18 @@ int tp_batch_destroy(tp_batch* b) {
   return TP_OK;
 }
 
-int tp_batch_set_launch(tp_batch* b, int block, int waves_per_simd) {
+int tp_batch_set_launch(tp_batch* b, int block, int fwd_waves, int back_waves) {
   if (!b) return fail(TP_EINVAL, "tp_batch_set_launch: batch is NULL");
   if (block == 0) block = 256;
   if (block < 64 || block > 256 || block % 64 != 0)
     return fail(TP_EINVAL, "tp_batch_set_launch: block %d not in {64, 128, 192, 256}", block);
-  if (waves_per_simd < 0 || waves_per_simd > 8)
-    return fail(TP_EINVAL, "tp_batch_set_launch: waves_per_simd %d not in [0, 8]", waves_per_simd);
+  if (fwd_waves != 0 && fwd_waves != 5 && fwd_waves != 6)
+    return fail(TP_EINVAL, "tp_batch_set_launch: fwd_waves %d not in {0, 5, 6}", fwd_waves);
+  if (back_waves != 0 && back_waves != 3 && back_waves != 4)
+    return fail(TP_EINVAL, "tp_batch_set_launch: back_waves %d not in {0, 3, 4}", back_waves);
   b->block = block;
-  b->waves_per_simd = waves_per_simd;
+  b->fwd_waves = fwd_waves;
+  b->back_waves = back_waves;
   return TP_OK;
 }
 

@@ -195,8 +195,12 @@ TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq, con
   }
 }
 
-template <unsigned FEAT>
-__global__ __launch_bounds__(256) void adjoint_kernel(const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
+// WPE > 0 asks the register allocator for at least WPE resident waves per SIMD (spilling
+// if it must); WPE = 0 leaves the budget to the compiler.
+#define TP_WAVES(WPE) __attribute__((amdgpu_waves_per_eu((WPE) > 0 ? (WPE) : 1)))
+
+template <unsigned FEAT, int WPE>
+__global__ __launch_bounds__(256) TP_WAVES(WPE) void adjoint_kernel(const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
                                                       const double* __restrict__ pool,
                                                       const int32_t* __restrict__ slot_of_pos,
                                                       const double* __restrict__ end_state, double* __restrict__ lam,
@@ -246,8 +250,8 @@ __global__ __launch_bounds__(256) void adjoint_kernel(const int32_t* __restrict_
 // backwards from the stored states, in the same per-lane order as adjoint_kernel, keeps
 // its strength gradient in the wave's own LDS row and writes the row to its item slot;
 // the per-variant sums are formed afterwards in tile order (batch_reduce_kernel).
-template <unsigned FEAT>
-__global__ __launch_bounds__(256) void batch_back_kernel(const TpVarDesc* __restrict__ desc,
+template <unsigned FEAT, int WPE>
+__global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_back_kernel(const TpVarDesc* __restrict__ desc,
                                                          const int2* __restrict__ items, int32_t n_items,
                                                          int32_t* __restrict__ counter, const int32_t* __restrict__ seq,
                                                          const tp_rec* __restrict__ recs,
@@ -334,18 +338,44 @@ __global__ void reduce_kernel(const double* __restrict__ partial, int nblocks, i
   grad[k] = acc;
 }
 
+// Register budgets compiled: the default, and at least 3 or 4 waves per SIMD for variants
+// B and C (the default leaves their reverse kernels at 2).
+#define TP_WPE_SWITCH(waves, CALL)                            \
+  if constexpr (FEAT == TP_FEAT_B || FEAT == TP_FEAT_C) {     \
+    if ((waves) == 3) { CALL(3); }                            \
+    else if ((waves) == 4) { CALL(4); }                       \
+    else { CALL(0); }                                         \
+  } else {                                                    \
+    CALL(0);                                                  \
+  }
+
 template <unsigned FEAT>
 static void launch_v(const TpAdjArgs& a, unsigned grid, hipStream_t stream) {
   const size_t lds = (size_t)4 * (size_t)(a.n_slots > 0 ? a.n_slots : 1) * sizeof(double);
-  hipLaunchKernelGGL((adjoint_kernel<FEAT>), dim3(grid), dim3(256), lds, stream, a.seq, a.recs, a.pool,
-                     a.slot_of_pos, a.end_state, a.lam, a.lost_turn, a.partial, a);
+#define TP_CALL(W)                                                                                          \
+  hipLaunchKernelGGL((adjoint_kernel<FEAT, W>), dim3(grid), dim3(256), lds, stream, a.seq, a.recs, a.pool, \
+                     a.slot_of_pos, a.end_state, a.lam, a.lost_turn, a.partial, a)
+  TP_WPE_SWITCH(a.waves, TP_CALL)
+#undef TP_CALL
 }
 
 template <unsigned FEAT>
 static void launch_back_v(const TpBatchArgs& b, hipStream_t stream) {
   const size_t lds = (size_t)(b.block / 64) * (size_t)(b.max_slots > 0 ? b.max_slots : 1) * sizeof(double);
-  hipLaunchKernelGGL((batch_back_kernel<FEAT>), dim3(b.grid), dim3(b.block), lds, stream, b.desc, b.items, b.n_items,
-                     b.counter, b.seq, b.recs, b.pool, b.slots, b.ckpt, b.item_partial, b.max_slots);
+#define TP_CALL(W)                                                                                             \
+  hipLaunchKernelGGL((batch_back_kernel<FEAT, W>), dim3(b.grid), dim3(b.block), lds, stream, b.desc, b.items, \
+                     b.n_items, b.counter, b.seq, b.recs, b.pool, b.slots, b.ckpt, b.item_partial, b.max_slots)
+  TP_WPE_SWITCH(b.waves, TP_CALL)
+#undef TP_CALL
+}
+
+template <unsigned FEAT>
+static const void* back_fn(int waves) {
+  const void* fn = nullptr;
+#define TP_CALL(W) fn = reinterpret_cast<const void*>(batch_back_kernel<FEAT, W>)
+  TP_WPE_SWITCH(waves, TP_CALL)
+#undef TP_CALL
+  return fn;
 }
 
 }  // namespace TPK_BUILD
@@ -361,13 +391,13 @@ hipError_t TP_BATCH_BACK_LAUNCH(const TpBatchArgs& b, hipStream_t stream) {
   return hipGetLastError();
 }
 
-int TP_BATCH_BACK_OCC(int variant, int block, int max_slots) {
+int TP_BATCH_BACK_OCC(int variant, int block, int max_slots, int waves) {
   const size_t lds = (size_t)(block / 64) * (size_t)(max_slots > 0 ? max_slots : 1) * sizeof(double);
   const void* fn = nullptr;
   switch (variant) {
-    case TP_VARIANT_A: fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::batch_back_kernel<TP_FEAT_A>); break;
-    case TP_VARIANT_B: fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::batch_back_kernel<TP_FEAT_B>); break;
-    default: fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::batch_back_kernel<TP_FEAT_C>); break;
+    case TP_VARIANT_A: fn = tpk::TPK_BUILD::back_fn<TP_FEAT_A>(waves); break;
+    case TP_VARIANT_B: fn = tpk::TPK_BUILD::back_fn<TP_FEAT_B>(waves); break;
+    default: fn = tpk::TPK_BUILD::back_fn<TP_FEAT_C>(waves); break;
   }
   int nb = 0;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, lds) == hipSuccess ? nb : 0;

@@ -54,6 +54,7 @@ struct TpKArgs {
   int32_t turn0;
   int32_t nturns;
   int32_t variant;  // TP_VARIANT_*
+  int32_t waves;    // register budget of the kernel: 0 (compiler's), 5 or 6 waves per SIMD
 };
 
 // One lattice variant of a batch: where its element table sits in the concatenated
@@ -98,6 +99,7 @@ struct TpBatchArgs {
   int32_t exact_h;
   int32_t grid;
   int32_t block;
+  int32_t waves;  // register budget of the kernel launched (0 = the compiler's)
 };
 
 // Arguments of one backward segment of the adjoint (tp_adjoint_kernels.hip).
@@ -117,7 +119,7 @@ struct TpAdjArgs {
   int32_t nturns;              // turns in this segment
   int32_t init_lambda;         // 1: start from the gradient of the loss sum, 2x / 2y
   int32_t variant;
-  int32_t pad;
+  int32_t waves;               // register budget: 0 (compiler's), 3 or 4 waves per SIMD
   double beti;
 };
 
@@ -126,12 +128,12 @@ hipError_t tp_launch_track_exact(const TpKArgs& a, int block, int ppl, hipStream
 hipError_t tp_launch_track_fast(const TpKArgs& a, int block, int ppl, hipStream_t stream);
 hipError_t tp_launch_batch_exact(const TpBatchArgs& b, bool ckpt, hipStream_t stream);
 hipError_t tp_launch_batch_fast(const TpBatchArgs& b, bool ckpt, hipStream_t stream);
-int tp_batch_occupancy_exact(int variant, int exact_h, bool ckpt, int block);
-int tp_batch_occupancy_fast(int variant, int exact_h, bool ckpt, int block);
+int tp_batch_occupancy_exact(int variant, int exact_h, bool ckpt, int block, int waves);
+int tp_batch_occupancy_fast(int variant, int exact_h, bool ckpt, int block, int waves);
 hipError_t tp_launch_batch_back_exact(const TpBatchArgs& b, hipStream_t stream);
 hipError_t tp_launch_batch_back_fast(const TpBatchArgs& b, hipStream_t stream);
-int tp_batch_back_occupancy_exact(int variant, int block, int max_slots);
-int tp_batch_back_occupancy_fast(int variant, int block, int max_slots);
+int tp_batch_back_occupancy_exact(int variant, int block, int max_slots, int waves);
+int tp_batch_back_occupancy_fast(int variant, int block, int max_slots, int waves);
 // Per variant: gradient sums over its tiles in tile order, loss sum and survivors.
 hipError_t tp_launch_batch_reduce(const TpVarDesc* desc, const int32_t* vfirst, const int32_t* vtiles, int32_t n_var,
                                   const double* item_partial, int32_t max_slots, const double* item_loss, double* grad,

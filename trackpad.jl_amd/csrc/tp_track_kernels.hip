@@ -406,8 +406,12 @@ TP_DEV void run_turns(St* s, Cx* c, bool* alive, const int32_t* __restrict__ seq
   }
 }
 
-template <bool EXH, unsigned FEAT, int PPL>
-__global__ __launch_bounds__(256) void track_kernel(const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
+// WPE > 0 asks the register allocator for at least WPE resident waves per SIMD (spilling
+// if it must); WPE = 0 leaves the budget to the compiler.
+#define TP_WAVES(WPE) __attribute__((amdgpu_waves_per_eu((WPE) > 0 ? (WPE) : 1)))
+
+template <bool EXH, unsigned FEAT, int PPL, int WPE>
+__global__ __launch_bounds__(256) TP_WAVES(WPE) void track_kernel(const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
                                                     const double* __restrict__ pool, double* __restrict__ coords,
                                                     int32_t* __restrict__ lost_turn, int32_t* __restrict__ lost_elem,
                                                     TpKArgs a) {
@@ -436,8 +440,8 @@ __global__ __launch_bounds__(256) void track_kernel(const int32_t* __restrict__ 
 // in the one-beam kernel.  With CKPT the wave also stores its tile's state every
 // ckpt_stride turns (and at the last turn) into the variant's checkpoint slab, and the
 // tile's loss sum and survivor count, for the batched adjoint.
-template <bool EXH, unsigned FEAT, bool CKPT>
-__global__ __launch_bounds__(256) void batch_kernel(const TpVarDesc* __restrict__ desc, const int2* __restrict__ items,
+template <bool EXH, unsigned FEAT, bool CKPT, int WPE>
+__global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_kernel(const TpVarDesc* __restrict__ desc, const int2* __restrict__ items,
                                                     int32_t n_items, int32_t* __restrict__ counter,
                                                     const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
                                                     const double* __restrict__ pool, double* __restrict__ ckpt,
@@ -503,23 +507,52 @@ __global__ void fill_i32(int32_t* __restrict__ p, int32_t v, int32_t n) {
 
 #define TP_KPTRS(a) a.seq, a.recs, a.pool, a.coords, a.lost_turn, a.lost_elem
 
+// Register budgets compiled per kernel: the default, and at least 5 or 6 waves per SIMD
+// for the one-particle-per-lane kernels of variants B and C (the others keep the default).
+template <bool EXH, unsigned FEAT>
+constexpr bool has_budgets() {
+  return EXH && (FEAT == TP_FEAT_B || FEAT == TP_FEAT_C);
+}
+
+#define TP_WPE_SWITCH(waves, CALL)                    \
+  if constexpr (has_budgets<EXH, FEAT>()) {           \
+    if ((waves) == 5) { CALL(5); }                    \
+    else if ((waves) == 6) { CALL(6); }               \
+    else { CALL(0); }                                 \
+  } else {                                            \
+    CALL(0);                                          \
+  }
+
 template <bool EXH, unsigned FEAT>
 static void launch_v(const TpKArgs& a, int ppl, unsigned grid, int block, hipStream_t stream) {
-  if (ppl == 2)
-    hipLaunchKernelGGL((tpk::TPK_BUILD::track_kernel<EXH, FEAT, 2>), dim3(grid), dim3(block), 0, stream, TP_KPTRS(a), a);
-  else
-    hipLaunchKernelGGL((tpk::TPK_BUILD::track_kernel<EXH, FEAT, 1>), dim3(grid), dim3(block), 0, stream, TP_KPTRS(a), a);
+  if (ppl == 2) {
+    hipLaunchKernelGGL((tpk::TPK_BUILD::track_kernel<EXH, FEAT, 2, 0>), dim3(grid), dim3(block), 0, stream,
+                       TP_KPTRS(a), a);
+  } else {
+#define TP_CALL(W)                                                                                          \
+  hipLaunchKernelGGL((tpk::TPK_BUILD::track_kernel<EXH, FEAT, 1, W>), dim3(grid), dim3(block), 0, stream, \
+                     TP_KPTRS(a), a)
+    TP_WPE_SWITCH(a.waves, TP_CALL)
+#undef TP_CALL
+  }
 }
 
 template <bool EXH, unsigned FEAT, bool CKPT>
-static const void* batch_fn() {
-  return reinterpret_cast<const void*>(tpk::TPK_BUILD::batch_kernel<EXH, FEAT, CKPT>);
+static const void* batch_fn(int waves) {
+  const void* fn = nullptr;
+#define TP_CALL(W) fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::batch_kernel<EXH, FEAT, CKPT, W>)
+  TP_WPE_SWITCH(waves, TP_CALL)
+#undef TP_CALL
+  return fn;
 }
 
 template <bool EXH, unsigned FEAT, bool CKPT>
 static void launch_batch_v(const TpBatchArgs& b, hipStream_t stream) {
-  hipLaunchKernelGGL((tpk::TPK_BUILD::batch_kernel<EXH, FEAT, CKPT>), dim3(b.grid), dim3(b.block), 0, stream, b.desc,
-                     b.items, b.n_items, b.counter, b.seq, b.recs, b.pool, b.ckpt, b.item_loss);
+#define TP_CALL(W)                                                                                               \
+  hipLaunchKernelGGL((tpk::TPK_BUILD::batch_kernel<EXH, FEAT, CKPT, W>), dim3(b.grid), dim3(b.block), 0, stream, \
+                     b.desc, b.items, b.n_items, b.counter, b.seq, b.recs, b.pool, b.ckpt, b.item_loss)
+  TP_WPE_SWITCH(b.waves, TP_CALL)
+#undef TP_CALL
 }
 
 #define TP_BATCH_DISPATCH(CALL)                               \
@@ -549,14 +582,14 @@ hipError_t TP_BATCH_LAUNCH(const TpBatchArgs& b, bool ckpt, hipStream_t stream) 
 }
 
 // Resident blocks per CU of the batched kernel, for sizing the persistent grid.
-int TP_BATCH_OCC(int variant, int exact_h, bool ckpt, int block) {
+int TP_BATCH_OCC(int variant, int exact_h, bool ckpt, int block, int waves) {
   const void* fn = nullptr;
   if (ckpt) {
-#define TP_CALL(E, F) fn = batch_fn<E, F, true>()
+#define TP_CALL(E, F) fn = batch_fn<E, F, true>(waves)
     TP_BATCH_DISPATCH(TP_CALL)
 #undef TP_CALL
   } else {
-#define TP_CALL(E, F) fn = batch_fn<E, F, false>()
+#define TP_CALL(E, F) fn = batch_fn<E, F, false>(waves)
     TP_BATCH_DISPATCH(TP_CALL)
 #undef TP_CALL
   }

@@ -63,8 +63,10 @@ class HIPBatch:
         self._fin = weakref.finalize(self, _release, self.ctx._lib.tp_batch_destroy, h)
         self._fin.atexit = False
 
-    def set_launch(self, block: int = 0, waves_per_simd: int = 0) -> None:
-        L.check(self.ctx._lib.tp_batch_set_launch(self.handle, int(block), int(waves_per_simd)))
+    def set_launch(self, block: int = 0, fwd_waves: int = 0, back_waves: int = 0) -> None:
+        """Work-group size and register budgets (resident waves per SIMD) of the forward
+        and reverse persistent kernels; 0 keeps the default."""
+        L.check(self.ctx._lib.tp_batch_set_launch(self.handle, int(block), int(fwd_waves), int(back_waves)))
 
     def info(self) -> dict:
         ni, gf, gb, cb = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64()

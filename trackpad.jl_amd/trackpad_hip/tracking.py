@@ -76,6 +76,10 @@ class HIPContext:
     def set_launch(self, block: int = 0, ppl: int = 0, turns_per_launch: int = 0) -> None:
         L.check(self._lib.tp_set_launch(self.handle, int(block), int(ppl), int(turns_per_launch)))
 
+    def set_waves(self, fwd_waves: int = 0, back_waves: int = 0) -> None:
+        """Register budgets (resident waves per SIMD) of the tracking and reverse kernels."""
+        L.check(self._lib.tp_set_waves(self.handle, int(fwd_waves), int(back_waves)))
+
     def timing(self) -> dict:
         t = L.TimingT()
         L.check(self._lib.tp_timing(self.handle, C.byref(t)))
