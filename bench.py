@@ -112,12 +112,18 @@ def bench_batched(args, fast):
     init = [HIPParticles.from_coords(c, ctx=ctx) for c in c0]
     b = HIPBatch(tabs, parts, ctx=ctx)
     b.set_launch(args.block, args.fwd_waves, args.back_waves)
+    K = args.ckpt_stride
+    tune = None
+    if not args.no_autotune:
+        probe = np.maximum(1, ts // 100).astype(np.int32)
+        tune = b.autotune(probe, ckpt_stride=K, adjoint=True, fast=fast, cache=args.tune_cache)
+        args.block, args.fwd_waves, args.back_waves = tune["block"], tune["fwd_waves"], tune["back_waves"]
+        print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
     n_seq = len(rings[0])
     fpt = flops_per_turn(rings[0])
     sched = float(np.sum(ns * ts))
     print(f"[bench] batched: {nvar} variants, sum N = {int(ns.sum())}, T in [{ts.min()}, {ts.max()}], "
           f"sum N*T = {sched:.3e}, kernel={args.kernel}", file=sys.stderr, flush=True)
-    K = args.ckpt_stride
 
     def reset():
         for p, q in zip(parts, init):
@@ -241,7 +247,7 @@ def batched_line(args, nvar, ns, ts, sched, wall, n_seq, info_f, fpt, live, avg_
         "config": {"workload": f"batched_dba_{nvar}_variants", "variants": nvar, "particles_total": int(ns.sum()),
                    "turns_min": int(ts.min()), "turns_max": int(ts.max()), "particle_turns": sched,
                    "elements_per_turn": n_seq, "kernel": args.kernel, "block": args.block,
-                   "fwd_waves": args.fwd_waves, "back_waves": args.back_waves,
+                   "fwd_waves": args.fwd_waves, "back_waves": args.back_waves, "autotuned": not args.no_autotune,
                    "work_items": info_f["n_items"], "grid_blocks": info_f["grid_fwd"],
                    "flops_per_particle_turn": fpt, "turn_scale": args.turn_scale},
         "live_particle_turns_per_s": live / avg_s,
@@ -273,6 +279,9 @@ def main():
     ap.add_argument("--kernel", default="fast", choices=["fast", "exact"])
     ap.add_argument("--particles", type=int, default=0, help="override N (default: the workload's)")
     ap.add_argument("--turns", type=int, default=0, help="override turns per step (default: the workload's)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use --block/--ppl/--turns-per-launch/--*-waves as given instead of the autotuner's choice")
+    ap.add_argument("--tune-cache", default=None, help="autotuner cache file (default: none, always time)")
     ap.add_argument("--turns-per-launch", type=int, default=100)
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--ppl", type=int, default=1)
@@ -307,6 +316,13 @@ def main():
     c0 = mk(n)
     init = HIPParticles.from_coords(c0, ctx=ctx)
     work = HIPParticles(n, ctx)
+    tune = None
+    if not args.no_autotune:
+        # launch shape from the autotuner: every candidate timed on a copy of the beam
+        tune = ctx.autotune(tab, init, probe_turns=2, fast=fast, cache=args.tune_cache)
+        args.block, args.ppl, args.turns_per_launch = tune["block"], tune["ppl"], tune["turns_per_launch"]
+        args.fwd_waves = tune["fwd_waves"]
+        print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
     print(f"[bench] workload={args.workload} N={n} T={T} elements={len(lat)} kernel={args.kernel} "
           f"block={args.block} ppl={args.ppl} turns/launch={args.turns_per_launch}", file=sys.stderr, flush=True)
 
@@ -360,7 +376,7 @@ def main():
         "config": {"workload": f"{args.workload}_{n}x{T}", "particles": n, "turns": T, "elements_per_turn": n_seq,
                    "kernel": args.kernel, "block": args.block, "particles_per_lane": args.ppl,
                    "turns_per_launch": args.turns_per_launch, "fwd_waves": args.fwd_waves,
-                   "flops_per_particle_turn": fpt},
+                   "autotuned": tune is not None, "flops_per_particle_turn": fpt},
         "live_particle_turns_per_s": live,
         "lost_particles": int(lost.sum()),
         "roofline": {"bound": "mfma", "pipe": "vector FP64 VALU (no MFMA: not a contraction); FP64 peak 78.6 TFLOP/s",

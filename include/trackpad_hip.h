@@ -210,6 +210,39 @@ int tp_batch_adjoint(tp_ctx* ctx, tp_batch* b, const int32_t* turn0, const int32
 int tp_batch_info(const tp_batch* b, int32_t* n_items, int32_t* grid_fwd, int32_t* grid_back,
                   int64_t* ckpt_bytes);
 
+/* ---- Autotuner ----------------------------------------------------------------------------
+ * No reference counterpart.  Chooses launch shapes by timing candidates on the device. */
+typedef struct tp_launch_cfg {
+  int32_t block;             /* work-group size */
+  int32_t ppl;               /* particles per lane (tp_autotune) */
+  int32_t turns_per_launch;  /* tp_autotune: turns per launch (0 = all) */
+  int32_t fwd_waves;         /* register budget of the tracking kernels (tp_set_waves) */
+  int32_t back_waves;        /* register budget of the reverse kernels */
+  int32_t candidates;        /* candidates timed (0: taken from the cache file) */
+  double rate;               /* particle-turns/s of the choice on the probe (forward) */
+} tp_launch_cfg;
+
+/* tp_autotune flags */
+#define TP_TUNE_ADJOINT 0x100u /* tp_batch_autotune: also time the reverse kernel budgets */
+
+/* Time every candidate launch shape -- block {64, 128, 256} x particles per lane {1, 2} x
+ * register budget {0, 5, 6} where compiled -- on a device copy of p for probe_turns turns
+ * (p itself is not changed), apply the fastest to ctx (tp_set_launch, tp_set_waves), and
+ * choose turns per launch so that one launch of N particles lasts about
+ * target_launch_ms at the measured rate (0 = 1000 ms).  flags: TP_TRACK_FAST or 0, the
+ * kernel flavour to tune.  cache_path (may be NULL): a text file of earlier choices keyed
+ * by device, table contents, N, probe_turns and flags; a hit is applied without timing,
+ * a miss is appended. */
+int tp_autotune(tp_ctx* ctx, const tp_table* t, const tp_particles* p, int32_t probe_turns, uint32_t flags,
+                double target_launch_ms, const char* cache_path, tp_launch_cfg* out);
+
+/* The same for a batch: block and forward register budget timed on the batched forward
+ * with turn counts probe_turns[v] (and, with TP_TUNE_ADJOINT, the reverse budget on the
+ * batched adjoint with checkpoint stride ckpt_stride); the particle sets are restored
+ * after every probe.  Applies the choice with tp_batch_set_launch. */
+int tp_batch_autotune(tp_ctx* ctx, tp_batch* b, const int32_t* probe_turns, int32_t ckpt_stride, uint32_t flags,
+                      const char* cache_path, tp_launch_cfg* out);
+
 #ifdef __cplusplus
 }
 #endif

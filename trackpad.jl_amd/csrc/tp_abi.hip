@@ -15,53 +15,9 @@
 #include <string>
 #include <vector>
 
-#include "trackpad_hip.h"
-#include "tp_device.h"
+#include "tp_runtime.h"
 
-// A context is reference counted: every table and particle set created on it holds a
-// reference, so destroy calls may come in any order (Julia and Python finalizers run in
-// arbitrary order) and the stream outlives everything enqueued on it.
-struct tp_ctx {
-  std::atomic<int> refs{1};
-  std::atomic<bool> closed{false};
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int block = 256;
-  int ppl = 1;
-  int tpl = 0;  // turns per launch, 0 = all
-  int fwd_waves = 0, back_waves = 0;  // register budgets (0 = the compiler's)
-  tp_timing_t timing{};
-};
-
-struct tp_table {
-  tp_ctx* ctx = nullptr;
-  void* dev = nullptr;
-  size_t nbytes = 0;
-  tp_table_header hdr{};
-  const int32_t* d_seq = nullptr;
-  const tp_rec* d_recs = nullptr;
-  const double* d_pool = nullptr;
-  unsigned features = 0;
-  int variant = TP_VARIANT_ALL;
-  std::vector<tp_rec> recs;      // host copy with the kick specialisation filled in
-  std::vector<int32_t> seq;      // host copy
-  std::vector<double> pool;      // host copy
-  int32_t* d_slots = nullptr;    // gradient slot of each position (adjoint), built on first use
-  int32_t n_slots = 0;
-};
-
-// A particle set is reference counted too: a batch that binds it holds a reference.
-struct tp_particles {
-  std::atomic<int> refs{1};
-  tp_ctx* ctx = nullptr;
-  int64_t n = 0;
-  double* d_coords = nullptr;
-  int32_t* d_lt = nullptr;
-  int32_t* d_le = nullptr;
-};
-
-namespace {
+namespace tpr {
 
 thread_local std::string g_err;
 
@@ -75,12 +31,6 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define TP_HIP(call)                                                                       \
-  do {                                                                                     \
-    hipError_t e_ = (call);                                                                \
-    if (e_ != hipSuccess) return fail(TP_EHIP, "%s failed: %s (%s)", #call, hipGetErrorName(e_), \
-                                      hipGetErrorString(e_));                             \
-  } while (0)
 
 // Length in doubles of a record's parameter block (trackpad_table.h).
 int64_t block_len(int tag, int m) {
@@ -127,6 +77,15 @@ int classify_kick(const double* poly, int m) {
   return TP_KICK_NONE;
 }
 
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+  const unsigned char* c = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) {
+    h ^= c[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
 int set_dev(tp_ctx* ctx) {
   TP_HIP(hipSetDevice(ctx->device));
   return TP_OK;
@@ -154,7 +113,9 @@ void particles_release(tp_particles* p) {
   ctx_release(ctx);
 }
 
-}  // namespace
+}  // namespace tpr
+
+using namespace tpr;
 
 extern "C" {
 
@@ -719,44 +680,7 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
 
 }  // extern "C"
 
-// ---------------------------------------------------------------------------------------
-// Batches of lattice variants
-// ---------------------------------------------------------------------------------------
-struct tp_batch {
-  tp_ctx* ctx = nullptr;
-  int32_t n_var = 0;
-  std::vector<tp_particles*> parts;
-  std::vector<int32_t> n_seq, n_slots;
-  std::vector<int64_t> seq_off, rec_off, pool_off, slot_off;
-  std::vector<tp_table_header> hdr;
-  std::vector<int32_t> gorder;  // per concatenated position: gorder of its record
-  unsigned features = 0;
-  int variant = TP_VARIANT_ALL;
-  int exact_h = 1;
-  std::string no_adjoint;  // why the adjoint is unsupported ("" = supported)
-  int32_t total_slots = 0, max_slots = 0;
-  int32_t* d_seq = nullptr;
-  tp_rec* d_recs = nullptr;
-  double* d_pool = nullptr;
-  int32_t* d_slots = nullptr;
-  int block = 256, fwd_waves = 0, back_waves = 0, cus = 0;
-  // the planned launch (rebuilt when its key changes)
-  int mode = 0;  // 0 none, 1 track, 2 adjoint
-  uint32_t pflags = 0;
-  int32_t pstride = 0;
-  int pblock = 0, pfw = 0, pbw = 0;
-  std::vector<int32_t> pturn0, pnturns;
-  int32_t n_items = 0, grid_f = 0, grid_b = 0;
-  int64_t ckpt_bytes = 0;
-  std::vector<int64_t> lam_off;
-  void *d_desc = nullptr, *d_items = nullptr, *d_counter = nullptr, *d_vfirst = nullptr, *d_vtiles = nullptr;
-  void *d_ckpt = nullptr, *d_item_loss = nullptr, *d_item_partial = nullptr, *d_grad = nullptr, *d_vloss = nullptr;
-  void* d_lam = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-};
-
-namespace {
+namespace tpr {
 
 void dfree(void*& p) {
   if (p) (void)hipFree(p);
@@ -989,7 +913,7 @@ int batch_launch(tp_batch* b, uint32_t flags) {
   return TP_OK;
 }
 
-}  // namespace
+}  // namespace tpr
 
 extern "C" {
 
@@ -1064,6 +988,14 @@ int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_part
       if (p) (void)hipFree(p);
     return fail(e == hipErrorOutOfMemory ? TP_ENOMEM : TP_EHIP, "tp_batch_create: %s", hipGetErrorName(e));
   }
+  uint64_t h = fnv1a(FNV_OFFSET, seq.data(), seq.size() * sizeof(int32_t));
+  h = fnv1a(h, recs.data(), recs.size() * sizeof(tp_rec));
+  h = fnv1a(h, pool.data(), pool.size() * sizeof(double));
+  for (int32_t v = 0; v < n_var; ++v) {
+    h = fnv1a(h, &parts[v]->n, sizeof(int64_t));
+    h = fnv1a(h, &tables[v]->hdr, sizeof(tp_table_header));
+  }
+  b->hash = h;
   b->ctx = ctx;
   b->n_var = n_var;
   for (int32_t v = 0; v < n_var; ++v) {

@@ -12,7 +12,7 @@ import os
 import threading
 
 __all__ = ["lib", "TrackPadHIPError", "check", "LIB_PATH", "TP_TRACK_FAST", "TP_TRACK_ASYNC",
-           "TP_TRACK_TIMED", "TimingT"]
+           "TP_TRACK_TIMED", "TimingT", "LaunchCfg", "TP_TUNE_ADJOINT"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TRACKPAD_HIP_LIB", os.path.join(os.path.dirname(_HERE), "lib", "libtrackpad_hip.so"))
@@ -31,6 +31,15 @@ class TrackPadHIPError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"{_STATUS.get(code, code)}: {msg}")
         self.code = code
+
+
+class LaunchCfg(C.Structure):
+    _fields_ = [("block", C.c_int32), ("ppl", C.c_int32), ("turns_per_launch", C.c_int32),
+                ("fwd_waves", C.c_int32), ("back_waves", C.c_int32), ("candidates", C.c_int32),
+                ("rate", C.c_double)]
+
+
+TP_TUNE_ADJOINT = 0x100
 
 
 class TimingT(C.Structure):
@@ -73,6 +82,8 @@ _SIGS = {
     "tp_batch_destroy": ([_P], C.c_int),
     "tp_batch_set_launch": ([_P, C.c_int, C.c_int, C.c_int], C.c_int),
     "tp_set_waves": ([_P, C.c_int, C.c_int], C.c_int),
+    "tp_autotune": ([_P, _P, _P, C.c_int32, C.c_uint32, C.c_double, C.c_char_p, C.c_void_p], C.c_int),
+    "tp_batch_autotune": ([_P, _P, _PI32, C.c_int32, C.c_uint32, C.c_char_p, C.c_void_p], C.c_int),
     "tp_batch_track": ([_P, _P, _PI32, _PI32, C.c_uint32], C.c_int),
     "tp_batch_adjoint": ([_P, _P, _PI32, _PI32, C.c_int32, C.c_uint32, _PD, _PI64, _PD, _PD], C.c_int),
     "tp_batch_info": ([_P, _PI32, _PI32, _PI32, _PI64], C.c_int),

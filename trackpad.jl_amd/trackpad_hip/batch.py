@@ -18,7 +18,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _lib as L
-from .tracking import DeviceTable, HIPContext, HIPParticles, _flags, _release, default_context
+from .tracking import DeviceTable, HIPContext, HIPParticles, _cfg_dict, _flags, _release, default_context
 
 __all__ = ["HIPBatch", "BatchGradients"]
 
@@ -67,6 +67,18 @@ class HIPBatch:
         """Work-group size and register budgets (resident waves per SIMD) of the forward
         and reverse persistent kernels; 0 keeps the default."""
         L.check(self.ctx._lib.tp_batch_set_launch(self.handle, int(block), int(fwd_waves), int(back_waves)))
+
+    def autotune(self, probe_turns, *, ckpt_stride: int = 10, adjoint: bool = False, fast: bool = True,
+                 cache: Optional[str] = None) -> dict:
+        """Time block sizes and register budgets of the persistent kernels on the bound
+        beams (restored afterwards) and apply the fastest (tp_batch_autotune)."""
+        nt, _ = self._turns(probe_turns, None)
+        cfg = L.LaunchCfg()
+        fl = (L.TP_TRACK_FAST if fast else 0) | (L.TP_TUNE_ADJOINT if adjoint else 0)
+        L.check(self.ctx._lib.tp_batch_autotune(self.ctx.handle, self.handle, nt.ctypes.data_as(_PI32),
+                                                int(ckpt_stride), fl, None if cache is None else str(cache).encode(),
+                                                C.byref(cfg)))
+        return _cfg_dict(cfg)
 
     def info(self) -> dict:
         ni, gf, gb, cb = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64()

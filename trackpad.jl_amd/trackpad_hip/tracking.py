@@ -80,6 +80,16 @@ class HIPContext:
         """Register budgets (resident waves per SIMD) of the tracking and reverse kernels."""
         L.check(self._lib.tp_set_waves(self.handle, int(fwd_waves), int(back_waves)))
 
+    def autotune(self, table: "DeviceTable", particles: "HIPParticles", probe_turns: int = 2, *, fast: bool = True,
+                 target_launch_ms: float = 0.0, cache: Optional[str] = None) -> dict:
+        """Time the candidate launch shapes on a copy of ``particles`` and apply the fastest
+        (tp_autotune); ``cache`` is a text file of earlier choices."""
+        cfg = L.LaunchCfg()
+        L.check(self._lib.tp_autotune(self.handle, table.handle, particles.handle, int(probe_turns),
+                                      L.TP_TRACK_FAST if fast else 0, float(target_launch_ms),
+                                      None if cache is None else str(cache).encode(), C.byref(cfg)))
+        return _cfg_dict(cfg)
+
     def timing(self) -> dict:
         t = L.TimingT()
         L.check(self._lib.tp_timing(self.handle, C.byref(t)))
@@ -197,6 +207,12 @@ class HIPParticles:
         pc, pt, pe, n = _PD(), _PI32(), _PI32(), C.c_int64()
         L.check(self.ctx._lib.tp_particles_device_ptrs(self.handle, C.byref(pc), C.byref(pt), C.byref(pe), C.byref(n)))
         return C.cast(pc, C.c_void_p).value, C.cast(pt, C.c_void_p).value, C.cast(pe, C.c_void_p).value, n.value
+
+
+def _cfg_dict(cfg) -> dict:
+    return {"block": cfg.block, "ppl": cfg.ppl, "turns_per_launch": cfg.turns_per_launch,
+            "fwd_waves": cfg.fwd_waves, "back_waves": cfg.back_waves, "candidates": cfg.candidates,
+            "rate": cfg.rate}
 
 
 def _as_coords(coords) -> np.ndarray:
