@@ -43,6 +43,12 @@ import numpy as np  # noqa: E402
 FP64_PEAK = 78.6e12
 HBM_PEAK = 8.0e12
 
+# HBM bytes per particle of one tracking-kernel launch, measured with rocprofv3 --pmc on the
+# 1e6-particle DBA launch: FETCH_SIZE 25,553.6 KiB x 1.998 (the counter reads 23,456.9 KiB for a
+# plain 48 MB device copy) + WRITE_SIZE 46,875.0 KiB x 1.00 (exact on the same copy), per 1e6
+# particles.  Independent of the turns per launch (the state stays in registers in between).
+MEASURED_BYTES_PER_PARTICLE_LAUNCH = {"dba": (25553.6 * 1.998 + 46875.0) * 1024 / 1e6}
+
 
 def host_cores() -> int:
     try:
@@ -359,6 +365,8 @@ def main():
     avg_launch_s = kernel_ms / 1e3 / max(launches, 1)
     live_per_launch = float(live_turns) * args.steps / max(launches, 1)
     achieved = fpt * live_per_launch / avg_launch_s
+    bpp = MEASURED_BYTES_PER_PARTICLE_LAUNCH.get(args.workload)
+    traffic = None if bpp is None else bpp * n  # HBM bytes per launch (rocprof PMC, calibrated)
 
     out = {
         "metric": "particle-turns/s FP64 (1 GPU, % FP64 peak); batched adjoint/forward cost ratio",
@@ -381,7 +389,7 @@ def main():
         "lost_particles": int(lost.sum()),
         "roofline": {"bound": "mfma", "pipe": "vector FP64 VALU (no MFMA: not a contraction); FP64 peak 78.6 TFLOP/s",
                      "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK, "traffic": None,
+                     "frac": achieved / FP64_PEAK, "traffic": traffic,
                      "avg_launch_ms": avg_launch_s * 1e3, "launches": launches},
     }
     if not args.no_cpu_baseline:
