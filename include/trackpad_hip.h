@@ -243,6 +243,17 @@ int tp_autotune(tp_ctx* ctx, const tp_table* t, const tp_particles* p, int32_t p
 int tp_batch_autotune(tp_ctx* ctx, tp_batch* b, const int32_t* probe_turns, int32_t ckpt_stride, uint32_t flags,
                       const char* cache_path, tp_launch_cfg* out);
 
+/* Jacobian of nturns turns, d(final coordinates) / d(initial coordinates), of every particle
+ * of p: jac[36 q + 6 i + j] = d out_i / d in_j for particle q (row-major 6 x 6 per particle;
+ * zero for particles lost during tracking).  The exact derivative of the tracked map, by six
+ * reverse passes with the unit seeds e_i through the adjoint machinery of tp_adjoint -- the
+ * GPU replacement of the finite-difference one_turn_map / findm66 (src/optics.jl:110-121,
+ * 184-215), which the host-side optics functions (gettune, getchrom, find_closed_orbit_*)
+ * build on.  p holds the initial beam on entry and the tracked beam on return.
+ * ckpt_stride, flags and the supported tables as for tp_adjoint. */
+int tp_jacobian(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns,
+                int32_t ckpt_stride, uint32_t flags, double* jac);
+
 #ifdef __cplusplus
 }
 #endif

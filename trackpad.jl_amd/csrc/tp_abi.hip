@@ -539,19 +539,50 @@ struct DevBuf {
 
 }  // namespace
 
+}  // extern "C"
+
+namespace {
+
+// Checks shared by the reverse-mode entry points.
+int adj_check(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t nturns, int32_t ckpt_stride, uint32_t flags,
+              const char* who) {
+  if (!ctx || !t || !p) return fail(TP_EINVAL, "%s: NULL argument", who);
+  if (t->ctx != ctx || p->ctx != ctx) return fail(TP_EINVAL, "%s: table and particles belong to another context", who);
+  if (ctx->closed) return fail(TP_EINVAL, "%s: context destroyed", who);
+  if (nturns < 1) return fail(TP_EINVAL, "%s: nturns = %d", who, nturns);
+  if (ckpt_stride < 0) return fail(TP_EINVAL, "%s: ckpt_stride = %d", who, ckpt_stride);
+  if (flags & ~(TP_TRACK_FAST | TP_TRACK_TIMED)) return fail(TP_EINVAL, "%s: unsupported flags 0x%x", who, flags);
+  if (t->features & (FT_ESBEND | FT_LINH))
+    return fail(TP_EUNSUPPORTED, "%s: Forest exact bends and the linearised drift have no adjoint here", who);
+  for (const tp_rec& r : t->recs)
+    if (r.flags & (TP_F_R1 | TP_F_R2)) return fail(TP_EUNSUPPORTED, "%s: r1/r2 rotations have no adjoint here", who);
+  return set_dev(ctx);
+}
+
+// Forward pass of the reverse-mode entry points: tracks p segment by segment (K turns) and
+// stores the beam at the end of every segment into ckpt ([nseg][6][n] doubles).
+int adj_forward(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, int32_t K, bool fast,
+                double* ckpt) {
+  const int32_t nseg = (nturns + K - 1) / K;
+  const size_t plane = (size_t)p->n;
+  for (int32_t j = 0; j < nseg; ++j) {
+    const int32_t kt = (j == nseg - 1) ? (nturns - j * K) : K;
+    int rc = tp_track(ctx, t, p, turn0 + j * K, kt, (fast ? TP_TRACK_FAST : 0) | TP_TRACK_ASYNC);
+    if (rc) return rc;
+    if (plane > 0)
+      TP_HIP(hipMemcpyAsync(ckpt + (size_t)j * 6 * plane, p->d_coords, 6 * plane * sizeof(double),
+                            hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return TP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, int32_t ckpt_stride,
                uint32_t flags, double* loss, int64_t* n_alive, double* dL_dk, double* dL_dx0) {
-  if (!ctx || !t || !p) return fail(TP_EINVAL, "tp_adjoint: NULL argument");
-  if (t->ctx != ctx || p->ctx != ctx) return fail(TP_EINVAL, "tp_adjoint: table and particles belong to another context");
-  if (ctx->closed) return fail(TP_EINVAL, "tp_adjoint: context destroyed");
-  if (nturns < 1) return fail(TP_EINVAL, "tp_adjoint: nturns = %d", nturns);
-  if (ckpt_stride < 0) return fail(TP_EINVAL, "tp_adjoint: ckpt_stride = %d", ckpt_stride);
-  if (flags & ~(TP_TRACK_FAST | TP_TRACK_TIMED)) return fail(TP_EINVAL, "tp_adjoint: unsupported flags 0x%x", flags);
-  if (t->features & (FT_ESBEND | FT_LINH))
-    return fail(TP_EUNSUPPORTED, "tp_adjoint: Forest exact bends and the linearised drift have no adjoint here");
-  for (const tp_rec& r : t->recs)
-    if (r.flags & (TP_F_R1 | TP_F_R2)) return fail(TP_EUNSUPPORTED, "tp_adjoint: r1/r2 rotations have no adjoint here");
-  int rc = set_dev(ctx);
+  int rc = adj_check(ctx, t, p, nturns, ckpt_stride, flags, "tp_adjoint");
   if (rc) return rc;
   tp_table* tm = const_cast<tp_table*>(t);
   if (!tm->d_slots) {
@@ -584,13 +615,8 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
   hipStream_t st = ctx->stream;
   TP_HIP(hipEventRecord(ctx->ev0, st));
   // forward pass, storing the beam at the end of every segment
-  for (int32_t j = 0; j < nseg; ++j) {
-    const int32_t kt = (j == nseg - 1) ? (nturns - j * K) : K;
-    rc = tp_track(ctx, t, p, turn0 + j * K, kt, (fast ? TP_TRACK_FAST : 0) | TP_TRACK_ASYNC);
-    if (rc) return rc;
-    if (n > 0) TP_HIP(hipMemcpyAsync((double*)ckpt.p + (size_t)j * 6 * plane, p->d_coords, 6 * plane * sizeof(double),
-                                     hipMemcpyDeviceToDevice, st));
-  }
+  rc = adj_forward(ctx, t, p, turn0, nturns, K, fast, (double*)ckpt.p);
+  if (rc) return rc;
   TP_HIP(hipEventRecord(ctx->ev1, st));
   TP_HIP(hipEventSynchronize(ctx->ev1));
   float fwd_ms = 0.f;
@@ -674,6 +700,74 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
   if (dL_dx0 && n > 0) {
     TP_HIP(hipMemcpy(dL_dx0, lam.p, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < 6 * (size_t)n; ++i) dL_dx0[i] *= inv_na;
+  }
+  return TP_OK;
+}
+
+int tp_jacobian(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, int32_t ckpt_stride,
+                uint32_t flags, double* jac) {
+  int rc = adj_check(ctx, t, p, nturns, ckpt_stride, flags, "tp_jacobian");
+  if (rc) return rc;
+  const int64_t n = p->n;
+  const int32_t K = ckpt_stride > 0 ? ckpt_stride : 1;
+  const int32_t nseg = (nturns + K - 1) / K;
+  const size_t plane = (size_t)(n > 0 ? n : 1);
+  DevBuf ckpt, lam, partial;
+  hipError_t e = hipMalloc(&ckpt.p, (size_t)nseg * 6 * plane * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&lam.p, 6 * plane * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&partial.p, sizeof(double));
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? TP_ENOMEM : TP_EHIP, "tp_jacobian: checkpoint allocation (%d x %lld particles): %s",
+                nseg, (long long)n, hipGetErrorName(e));
+  const bool fast = (flags & TP_TRACK_FAST) != 0;
+  hipStream_t st = ctx->stream;
+  TP_HIP(hipEventRecord(ctx->ev0, st));
+  rc = adj_forward(ctx, t, p, turn0, nturns, K, fast, (double*)ckpt.p);
+  if (rc) return rc;
+  TP_HIP(hipEventRecord(ctx->ev1, st));
+  TP_HIP(hipEventSynchronize(ctx->ev1));
+  float fwd_ms = 0.f;
+  TP_HIP(hipEventElapsedTime(&fwd_ms, ctx->ev0, ctx->ev1));
+  TpAdjArgs a{};
+  a.seq = t->d_seq;
+  a.recs = t->d_recs;
+  a.pool = t->d_pool;
+  a.slot_of_pos = nullptr;  // no strength gradients
+  a.lam = (double*)lam.p;
+  a.lost_turn = p->d_lt;
+  a.partial = (double*)partial.p;
+  a.n = n;
+  a.ld = n;
+  a.n_seq = (int32_t)t->hdr.n_seq;
+  a.n_slots = 0;
+  a.variant = t->variant;
+  a.waves = ctx->back_waves;
+  a.beti = t->hdr.beti;
+  std::vector<double> row(6 * plane);
+  float bwd_ms = 0.f;
+  // row i of every particle's Jacobian: one reverse pass from the unit seed e_i
+  for (int i = 0; i < 6 && n > 0; ++i) {
+    TP_HIP(hipEventRecord(ctx->ev0, st));
+    for (int32_t j = nseg - 1; j >= 0; --j) {
+      a.end_state = (const double*)ckpt.p + (size_t)j * 6 * plane;
+      a.nturns = (j == nseg - 1) ? (nturns - j * K) : K;
+      a.init_lambda = (j == nseg - 1) ? 2 + i : 0;
+      hipError_t le = fast ? tp_launch_adjoint_fast(a, st) : tp_launch_adjoint_exact(a, st);
+      if (le != hipSuccess) return fail(TP_EHIP, "tp_jacobian: reverse kernel launch failed: %s", hipGetErrorName(le));
+    }
+    TP_HIP(hipEventRecord(ctx->ev1, st));
+    TP_HIP(hipMemcpyAsync(row.data(), lam.p, 6 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+    TP_HIP(hipStreamSynchronize(st));
+    float ms = 0.f;
+    TP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    bwd_ms += ms;
+    if (jac)
+      for (int64_t q = 0; q < n; ++q)
+        for (int c = 0; c < 6; ++c) jac[(size_t)q * 36 + (size_t)i * 6 + c] = row[(size_t)c * n + q];
+  }
+  if (flags & TP_TRACK_TIMED) {
+    ctx->timing.forward_ms = fwd_ms;
+    ctx->timing.backward_ms = bwd_ms;
   }
   return TP_OK;
 }

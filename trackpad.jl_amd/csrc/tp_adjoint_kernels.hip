@@ -176,7 +176,8 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
 
 // nturns whole turns backwards for one lane: s / l enter as the state and adjoint at the
 // end and leave as those at the start.  The strength gradient of every position with a
-// slot is summed over the wave and added to the wave's LDS row by lane 0.
+// slot is summed over the wave and added to the wave's LDS row by lane 0 (no strength
+// gradients when slot_of_pos is null).
 template <unsigned FEAT>
 TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
                        const double* __restrict__ pool, const int32_t* __restrict__ slot_of_pos, int32_t n_seq,
@@ -186,7 +187,7 @@ TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq, con
       const tp_rec rec = recs[seq[e]];
       double gacc = 0.0;
       apply_elem_back<FEAT>(s, l, c, rec, pool, beti, gacc);
-      const int32_t slot = slot_of_pos[e];
+      const int32_t slot = slot_of_pos ? slot_of_pos[e] : -1;
       if (slot >= 0) {
         const double w = wave_sum(gacc);
         if (lane == 0) row[slot] += w;
@@ -217,9 +218,18 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void adjoint_kernel(const int32_
   if (alive) {
     const double* q = end_state + i;
     s = St{q[0], q[a.ld], q[2 * a.ld], q[3 * a.ld], q[4 * a.ld], q[5 * a.ld]};
-    if (a.init_lambda) {
+    if (a.init_lambda == 1) {
       l.x = 2.0 * s.x;
       l.y = 2.0 * s.y;
+    } else if (a.init_lambda >= 2) {  // unit seed: row init_lambda - 2 of the Jacobian
+      switch (a.init_lambda - 2) {
+        case 0: l.x = 1.0; break;
+        case 1: l.px = 1.0; break;
+        case 2: l.y = 1.0; break;
+        case 3: l.py = 1.0; break;
+        case 4: l.z = 1.0; break;
+        default: l.d = 1.0; break;
+      }
     } else {
       const double* r = lam + i;
       l = Ad{r[0], r[a.ld], r[2 * a.ld], r[3 * a.ld], r[4 * a.ld], r[5 * a.ld]};

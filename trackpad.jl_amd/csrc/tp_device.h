@@ -107,7 +107,8 @@ struct TpAdjArgs {
   const int32_t* seq;
   const tp_rec* recs;
   const double* pool;
-  const int32_t* slot_of_pos;  // gradient slot of each lattice position, -1 for none
+  const int32_t* slot_of_pos;  // gradient slot of each lattice position, -1 for none (null: no
+                               // strength gradients)
   const double* end_state;     // beam at the end of the segment (6 planes of ld)
   double* lam;                 // adjoint: at the segment end on entry, at its start on return
   const int32_t* lost_turn;    // loss record after the last turn (alive == -1)
@@ -117,7 +118,8 @@ struct TpAdjArgs {
   int32_t n_seq;
   int32_t n_slots;
   int32_t nturns;              // turns in this segment
-  int32_t init_lambda;         // 1: start from the gradient of the loss sum, 2x / 2y
+  int32_t init_lambda;         // 0: continue from lam; 1: start from the gradient of the
+                               // loss sum, 2x / 2y; 2 + i: start from the unit vector e_i
   int32_t variant;
   int32_t waves;               // register budget: 0 (compiler's), 3 or 4 waves per SIMD
   double beti;

@@ -77,6 +77,7 @@ _SIGS = {
     "tp_track": ([_P, _P, _P, C.c_int32, C.c_int32, C.c_uint32], C.c_int),
     "tp_sync": ([_P], C.c_int),
     "tp_adjoint": ([_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _PD, _PI64, _PD, _PD], C.c_int),
+    "tp_jacobian": ([_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _PD], C.c_int),
     "tp_track_host": ([_P, _P, _PD, C.c_int64, _PI64, _PI32, _PI32, C.c_int32, C.c_int32, C.c_uint32], C.c_int),
     "tp_batch_create": ([_P, C.c_int32, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)], C.c_int),
     "tp_batch_destroy": ([_P], C.c_int),
