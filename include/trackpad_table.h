@@ -5,8 +5,10 @@
  * Julia) turns a static TrackPad Lattice + Beam into one little-endian byte blob:
  *
  *   tp_table_header                      64 bytes
- *   int32_t seq[n_seq]                   record index of each lattice position, padded
- *                                        with zeros to a multiple of 8 bytes
+ *   int32_t seq[n_rows][n_seq]           record index of each lattice position, one row per
+ *                                        turn of a time-varying lattice (n_rows = 1 for a
+ *                                        static one), padded with zeros to a multiple of 8
+ *                                        bytes
  *   tp_rec  recs[n_rec]                  32 bytes each; identical elements share a record
  *   double  pool[pool_len]               per-element parameter blocks (see below)
  *
@@ -93,8 +95,10 @@ typedef struct tp_table_header {
   uint32_t n_rec;    /* distinct element records */
   uint32_t pool_len; /* doubles in the parameter pool */
   uint32_t flags;    /* TP_H_* */
-  uint32_t reserved0;
-  uint32_t reserved1;
+  uint32_t n_rows;   /* sequence rows: 0 or 1 = a static lattice; R > 1 = row r is the
+                        lattice resolved at turn row_turn0 + r (time-varying elements
+                        materialised per turn, src/lattice.jl:267, src/time_dependence.jl) */
+  int32_t row_turn0; /* turn of row 0 (time-varying tables only) */
   double beti;        /* 1/beta of the reference particle (src/lattice.jl:61) */
   double coord_limit; /* |x|,|y| loss limit, COORD_LIMIT (src/tracking.jl:25) */
   double angle_limit; /* |px|,|py| loss limit, ANGLE_LIMIT (src/tracking.jl:26) */

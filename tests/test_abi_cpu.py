@@ -92,12 +92,50 @@ def test_errors_are_codes_without_a_device():
     assert lib.tp_set_launch(None, 256, 1, 0) == -1
 
 
-def test_time_varying_element_is_refused():
-    from trackpad_hip import TimeVaryingElement, materialize_lattice
+def test_time_varying_lattice_rows():
+    """A TimeVaryingElement is resolved per turn into sequence rows (ringpass! resolves it in
+    every linepass! call, src/lattice.jl:267, 305-317); static elements share one record."""
+    import struct
 
-    lat = Lattice([Drift(1.0), TimeVaryingElement(Quadrupole(0.5, 1.0), {"k1": lambda t, n: 1.0 + 0.1 * n})])
-    with pytest.raises(ValueError, match="materialize"):
-        compile_table(lat, Beam(3e9))
-    m = materialize_lattice(lat, turn=3)
-    assert m[1].k1 == pytest.approx(1.3)
-    assert _check(compile_table(m, Beam(3e9)))[0] == 0
+    from trackpad_hip import TimeVaryingElement, materialize_lattice, timed
+
+    seen = []
+
+    def k1(t, n):
+        seen.append((t, n))
+        return 1.0 + 0.1 * n
+
+    q = Quadrupole(0.5, 1.0)
+    lat = Lattice([Drift(1.0), TimeVaryingElement(q, {"k1": k1}), Drift(1.0)])
+    # no turn range: one row, the lattice at (time, turn), the same bytes as materialising
+    one = compile_table(lat, Beam(3e9), time=0.25, turn=3)
+    assert one == compile_table(materialize_lattice(lat, time=0.25, turn=3), Beam(3e9))
+    # four rows from turn 3, time advanced by accumulation as ringpass! does
+    seen.clear()
+    blob = compile_table(lat, Beam(3e9), turn=3, nturns=4, time=0.5, dt_turn=0.1)
+    assert _check(blob)[0] == 0
+    h = struct.unpack_from("<6IIi4d", blob)
+    n_seq, n_rec, n_rows, row_turn0 = h[2], h[3], h[6], h[7]
+    assert (n_seq, n_rows, row_turn0) == (3, 4, 3)
+    t = 0.5
+    want = []
+    for k in range(4):
+        want.append((t, 3 + k))
+        t += 0.1
+    assert seen == want
+    seq = struct.unpack_from("<12i", blob, 64)
+    assert len({seq[0], seq[2], seq[3], seq[5], seq[6], seq[8], seq[9], seq[11]}) == 1  # the drifts
+    assert len({seq[1], seq[4], seq[7], seq[10]}) == 4  # four different quadrupoles
+    assert n_rec == 5
+    # the k1 slot of each row's quadrupole record (poly b1 of the Yoshida block)
+    recs_at = 64 + 48
+    pool_at = recs_at + n_rec * 32
+    for k in range(4):
+        r = struct.unpack_from("<4iI3i", blob, recs_at + 32 * seq[1 + 3 * k])
+        b1 = struct.unpack_from("<d", blob, pool_at + 8 * (r[3] + 6 + 2))[0]
+        assert b1 == 1.0 + 0.1 * (3 + k)
+    # integer fields keep their type
+    tq = timed(Quadrupole(0.4, 0.6), num_int_steps=lambda t, n: 8 if t < 0.5 else 12)
+    assert tq.materialize(0.2, 0).num_int_steps == 8 and tq.materialize(1.0, 5).num_int_steps == 12
+    with pytest.raises(ValueError, match="no field"):
+        timed(Quadrupole(0.4, 0.6), k7=1.0)

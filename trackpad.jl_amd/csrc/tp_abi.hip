@@ -203,7 +203,12 @@ int parse_table(const void* bytes, size_t nbytes, tp_table* t) {
   std::memcpy(&h, bytes, sizeof h);
   if (h.magic != TP_TABLE_MAGIC) return fail(TP_ETABLE, "table: bad magic 0x%08x", h.magic);
   if (h.version != TP_TABLE_VERSION) return fail(TP_ETABLE, "table: version %u, library reads %u", h.version, TP_TABLE_VERSION);
-  const size_t seq_bytes = ((size_t)h.n_seq * 4 + 7) / 8 * 8;
+  if (h.n_rows > 1000000u) return fail(TP_ETABLE, "table: %u sequence rows", h.n_rows);
+  const size_t rows = h.n_rows > 1 ? h.n_rows : 1;
+  if (rows > 1 && (int64_t)h.row_turn0 + (int64_t)rows > 0x7fffffffLL)
+    return fail(TP_ETABLE, "table: sequence rows overflow the turn range");
+  const size_t n_pos = rows * (size_t)h.n_seq;
+  const size_t seq_bytes = (n_pos * 4 + 7) / 8 * 8;
   const size_t need = sizeof h + seq_bytes + (size_t)h.n_rec * sizeof(tp_rec) + (size_t)h.pool_len * 8;
   if (need != nbytes) return fail(TP_ETABLE, "table: size %zu, header implies %zu", nbytes, need);
   if (h.n_seq > 0 && h.n_rec == 0) return fail(TP_ETABLE, "table: %u positions but no records", h.n_seq);
@@ -212,15 +217,15 @@ int parse_table(const void* bytes, size_t nbytes, tp_table* t) {
     return fail(TP_ETABLE, "table: bad loss limits");
   t->hdr = h;
   const char* b = (const char*)bytes;
-  t->seq.resize(h.n_seq);
-  std::memcpy(t->seq.data(), b + sizeof h, (size_t)h.n_seq * 4);
+  t->seq.resize(n_pos);
+  std::memcpy(t->seq.data(), b + sizeof h, n_pos * 4);
   t->recs.resize(h.n_rec);
   std::memcpy(t->recs.data(), b + sizeof h + seq_bytes, (size_t)h.n_rec * sizeof(tp_rec));
   t->pool.resize(h.pool_len);
   std::memcpy(t->pool.data(), b + sizeof h + seq_bytes + (size_t)h.n_rec * sizeof(tp_rec), (size_t)h.pool_len * 8);
-  for (uint32_t i = 0; i < h.n_seq; ++i)
+  for (size_t i = 0; i < n_pos; ++i)
     if (t->seq[i] < 0 || (uint32_t)t->seq[i] >= h.n_rec)
-      return fail(TP_ETABLE, "table: seq[%u] = %d out of range", i, t->seq[i]);
+      return fail(TP_ETABLE, "table: seq[%zu] = %d out of range", i, t->seq[i]);
   unsigned feat = 0;
   if (!(h.flags & TP_H_EXACT_HAMILTONIAN)) feat |= FT_LINH;
   for (uint32_t i = 0; i < h.n_rec; ++i) {
@@ -284,7 +289,7 @@ int tp_table_create(tp_ctx* ctx, const void* bytes, size_t nbytes, tp_table** ou
   }
   t->ctx = ctx;
   const tp_table_header& h = t->hdr;
-  const size_t seq_bytes = ((size_t)h.n_seq * 4 + 7) / 8 * 8;
+  const size_t seq_bytes = (t->seq.size() * 4 + 7) / 8 * 8;
   // Device copy: the same layout with the library's kick field written.
   const char* b = (const char*)bytes;
   std::vector<char> blob(b, b + nbytes);
@@ -464,6 +469,12 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
   a.lost_elem = p->d_le;
   a.variant = t->variant;
   a.waves = ctx->fwd_waves;
+  a.n_rows = t->hdr.n_rows > 1 ? (int32_t)t->hdr.n_rows : 1;
+  a.row_turn0 = t->hdr.row_turn0;
+  if (a.n_rows > 1 && nturns > 0 &&
+      ((int64_t)turn0 < a.row_turn0 || (int64_t)turn0 + nturns > (int64_t)a.row_turn0 + a.n_rows))
+    return fail(TP_EINVAL, "tp_track: turns [%d, %lld) outside the table's time-varying rows [%d, %lld)", turn0,
+                (long long)turn0 + nturns, a.row_turn0, (long long)a.row_turn0 + a.n_rows);
   const bool timed = (flags & TP_TRACK_TIMED) != 0;
   const int chunk = ctx->tpl > 0 ? ctx->tpl : (nturns > 0 ? nturns : 1);
   double ms_total = 0.0;
@@ -556,6 +567,7 @@ int adj_check(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t nturns, i
     return fail(TP_EUNSUPPORTED, "%s: Forest exact bends and the linearised drift have no adjoint here", who);
   for (const tp_rec& r : t->recs)
     if (r.flags & (TP_F_R1 | TP_F_R2)) return fail(TP_EUNSUPPORTED, "%s: r1/r2 rotations have no adjoint here", who);
+  if (t->hdr.n_rows > 1) return fail(TP_EUNSUPPORTED, "%s: time-varying tables have no adjoint here", who);
   return set_dev(ctx);
 }
 
@@ -854,6 +866,9 @@ int batch_plan(tp_batch* b, int mode, const int32_t* turn0, const int32_t* nturn
     d = TpVarDesc{};
     tp_particles* p = b->parts[v];
     d.seq_off = b->seq_off[v];
+    d.pos_off = b->pos_off[v];
+    d.n_rows = b->hdr[v].n_rows > 1 ? (int32_t)b->hdr[v].n_rows : 1;
+    d.row_turn0 = b->hdr[v].row_turn0;
     d.rec_off = b->rec_off[v];
     d.pool_off = b->pool_off[v];
     d.slot_off = b->slot_off[v];
@@ -983,6 +998,9 @@ int batch_check(tp_ctx* ctx, tp_batch* b, const int32_t* turn0, const int32_t* n
     const int64_t t0 = turn0 ? turn0[v] : 0;
     if (nturns[v] < min_turns) return fail(TP_EINVAL, "%s: nturns[%d] = %d", who, v, nturns[v]);
     if (t0 < 0 || t0 + nturns[v] > 0x7fffffffLL) return fail(TP_EINVAL, "%s: turn range of variant %d overflows int32", who, v);
+    const tp_table_header& h = b->hdr[(size_t)v];
+    if (h.n_rows > 1 && nturns[v] > 0 && (t0 < h.row_turn0 || t0 + nturns[v] > (int64_t)h.row_turn0 + h.n_rows))
+      return fail(TP_EINVAL, "%s: turns of variant %d outside its table's time-varying rows", who, v);
   }
   return set_dev(ctx);
 }
@@ -1034,6 +1052,7 @@ int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_part
     const tp_table* t = tables[v];
     b->features |= t->features;
     b->seq_off.push_back((int64_t)seq.size());
+    b->pos_off.push_back((int64_t)slots.size());
     b->rec_off.push_back((int64_t)recs.size());
     b->pool_off.push_back((int64_t)pool.size());
     b->slot_off.push_back(b->total_slots);
@@ -1052,6 +1071,7 @@ int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_part
     b->total_slots += ns;
     b->max_slots = std::max(b->max_slots, ns);
     if (b->no_adjoint.empty()) {
+      if (t->hdr.n_rows > 1) b->no_adjoint = "time-varying tables have no adjoint here";
       if (t->features & (FT_ESBEND | FT_LINH))
         b->no_adjoint = "Forest exact bends and the linearised drift have no adjoint here";
       for (const tp_rec& r : t->recs)
@@ -1188,7 +1208,7 @@ int tp_batch_adjoint(tp_ctx* ctx, tp_batch* b, const int32_t* turn0, const int32
     if (dL_dk) {
       int32_t k = 0;
       for (int32_t e = 0; e < b->n_seq[v]; ++e) {
-        const int32_t g = b->gorder[(size_t)(b->seq_off[v] + e)];
+        const int32_t g = b->gorder[(size_t)(b->pos_off[v] + e)];
         double val = 0.0;
         if (g >= 0) {
           // d b_g / d k: b1 = k1, b2 = k2/2, b3 = k3/6 (src/tracking.jl:395, 446, 478)

@@ -297,7 +297,7 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_back_kernel(const TpV
       }
       mkctx(s, c, d.beti);
       const int32_t nt = (j == d.nseg - 1) ? d.nturns - j * d.ckpt_stride : d.ckpt_stride;
-      back_turns<FEAT>(s, l, c, seq + d.seq_off, recs + d.rec_off, pool + d.pool_off, slots + d.seq_off, d.n_seq, nt,
+      back_turns<FEAT>(s, l, c, seq + d.seq_off, recs + d.rec_off, pool + d.pool_off, slots + d.pos_off, d.n_seq, nt,
                        d.beti, row, lane);
     }
     if (li < d.n) {

@@ -162,6 +162,9 @@ int tp_autotune(tp_ctx* ctx, const tp_table* t, const tp_particles* p, int32_t p
   rc = tp_particles_create(ctx, n, &q);
   if (rc) return rc;
   const int save_block = ctx->block, save_ppl = ctx->ppl, save_tpl = ctx->tpl, save_fwd = ctx->fwd_waves;
+  // a time-varying table is probed on its own turn rows
+  const int32_t pt0 = t->hdr.n_rows > 1 ? t->hdr.row_turn0 : 0;
+  if (t->hdr.n_rows > 1 && (uint32_t)probe_turns > t->hdr.n_rows) probe_turns = (int32_t)t->hdr.n_rows;
   int ncand = 0;
   best.rate = -1.0;
   for (int block : {64, 128, 256}) {
@@ -174,9 +177,9 @@ int tp_autotune(tp_ctx* ctx, const tp_table* t, const tp_particles* p, int32_t p
         ctx->fwd_waves = w;
         // one untimed turn (code-object load, caches), then the timed probe from p's state
         rc = tp_particles_copy(q, p);
-        if (!rc) rc = tp_track(ctx, t, q, 0, 1, flags);
+        if (!rc) rc = tp_track(ctx, t, q, pt0, 1, flags);
         if (!rc) rc = tp_particles_copy(q, p);
-        if (!rc) rc = tp_track(ctx, t, q, 0, probe_turns, flags | TP_TRACK_TIMED);
+        if (!rc) rc = tp_track(ctx, t, q, pt0, probe_turns, flags | TP_TRACK_TIMED);
         if (rc) break;
         ++ncand;
         const double s = ctx->timing.kernel_ms * 1e-3;
@@ -212,7 +215,12 @@ int tp_autotune(tp_ctx* ctx, const tp_table* t, const tp_particles* p, int32_t p
 
 int tp_batch_autotune(tp_ctx* ctx, tp_batch* b, const int32_t* probe_turns, int32_t ckpt_stride, uint32_t flags,
                       const char* cache_path, tp_launch_cfg* out) {
-  int rc = batch_check(ctx, b, nullptr, probe_turns, 1, "tp_batch_autotune");
+  if (!b) return fail(TP_EINVAL, "tp_batch_autotune: batch is NULL");
+  // time-varying variants are probed on their own turn rows
+  std::vector<int32_t> pt0((size_t)(b->n_var > 0 ? b->n_var : 1), 0);
+  for (int32_t v = 0; v < b->n_var; ++v)
+    if (b->hdr[(size_t)v].n_rows > 1) pt0[(size_t)v] = b->hdr[(size_t)v].row_turn0;
+  int rc = batch_check(ctx, b, pt0.data(), probe_turns, 1, "tp_batch_autotune");
   if (rc) return rc;
   if (flags & ~(TP_TRACK_FAST | TP_TUNE_ADJOINT)) return fail(TP_EINVAL, "tp_batch_autotune: unsupported flags 0x%x", flags);
   if (ckpt_stride < 0) return fail(TP_EINVAL, "tp_batch_autotune: ckpt_stride = %d", ckpt_stride);
@@ -242,7 +250,7 @@ int tp_batch_autotune(tp_ctx* ctx, tp_batch* b, const int32_t* probe_turns, int3
   TP_HIP(bk.save());
   int ncand = 0;
   auto probe = [&](int mode, double* ms) -> int {
-    int r = batch_plan(b, mode, nullptr, probe_turns, mode == 2 ? K : 0, kflags);
+    int r = batch_plan(b, mode, pt0.data(), probe_turns, mode == 2 ? K : 0, kflags);
     if (!r) r = batch_launch(b, 0);  // untimed: loads the code object, warms the caches
     if (r) return r;
     hipError_t e = bk.restore();

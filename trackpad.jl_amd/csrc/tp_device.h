@@ -55,12 +55,15 @@ struct TpKArgs {
   int32_t nturns;
   int32_t variant;  // TP_VARIANT_*
   int32_t waves;    // register budget of the kernel: 0 (compiler's), 5 or 6 waves per SIMD
+  int32_t n_rows;   // sequence rows (1: static lattice; > 1: one row per turn from row_turn0)
+  int32_t row_turn0;
 };
 
 // One lattice variant of a batch: where its element table sits in the concatenated
 // tables, its particle set, and its turn range.
 struct TpVarDesc {
-  int64_t seq_off;   // first sequence entry (also the first entry of slot_of_pos)
+  int64_t seq_off;   // first sequence entry (rows x n_seq entries)
+  int64_t pos_off;   // first entry of the variant's slot_of_pos (n_seq entries)
   int64_t rec_off;   // first record
   int64_t pool_off;  // first pool double
   int64_t slot_off;  // first gradient slot of the variant in the concatenated gradient
@@ -76,6 +79,8 @@ struct TpVarDesc {
   int32_t n_slots;
   int32_t nseg;         // ceil(nturns / ckpt_stride)
   int32_t ckpt_stride;
+  int32_t n_rows;        // sequence rows of a time-varying lattice (1 = static)
+  int32_t row_turn0;
   double beti;
   double coord_limit;
   double angle_limit;

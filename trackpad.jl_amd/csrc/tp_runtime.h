@@ -40,7 +40,7 @@ struct tp_table {
   unsigned features = 0;
   int variant = TP_VARIANT_ALL;
   std::vector<tp_rec> recs;      // host copy with the kick specialisation filled in
-  std::vector<int32_t> seq;      // host copy
+  std::vector<int32_t> seq;      // host copy (n_rows x n_seq)
   std::vector<double> pool;      // host copy
   int32_t* d_slots = nullptr;    // gradient slot of each position (adjoint), built on first use
   int32_t n_slots = 0;
@@ -64,7 +64,7 @@ struct tp_batch {
   int32_t n_var = 0;
   std::vector<tp_particles*> parts;
   std::vector<int32_t> n_seq, n_slots;
-  std::vector<int64_t> seq_off, rec_off, pool_off, slot_off;
+  std::vector<int64_t> seq_off, pos_off, rec_off, pool_off, slot_off;  // pos_off: into slots / gorder
   std::vector<tp_table_header> hdr;
   std::vector<int32_t> gorder;  // per concatenated position: gorder of its record
   unsigned features = 0;

@@ -378,8 +378,10 @@ TP_DEV void run_turns(St* s, Cx* c, bool* alive, const int32_t* __restrict__ seq
 #pragma unroll
     for (int k = 0; k < PPL; ++k) anyl = anyl || alive[k];
     if (!__any(anyl)) break;
+    // the lattice of this turn: row (turn - row_turn0) of a time-varying table
+    const int32_t* __restrict__ seq_t = seq + (a.n_rows > 1 ? (int64_t)(a.turn0 + t - a.row_turn0) * a.n_seq : 0);
     for (int32_t e = 0; e < a.n_seq; ++e) {
-      const tp_rec rec = recs[seq[e]];
+      const tp_rec rec = recs[seq_t[e]];
       St s0[PPL];
 #pragma unroll
       for (int k = 0; k < PPL; ++k) s0[k] = s[k];
@@ -463,6 +465,8 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_kernel(const TpVarDes
     a.ld = d.n;
     a.turn0 = d.turn0;
     a.nturns = d.nturns;
+    a.n_rows = d.n_rows;
+    a.row_turn0 = d.row_turn0;
     const int32_t* vseq = seq + d.seq_off;
     const tp_rec* vrecs = recs + d.rec_off;
     const double* vpool = pool + d.pool_off;

@@ -9,7 +9,7 @@ path): element types, Beam, Lattice, ``linepass_`` / ``ringpass_`` (Julia's ``li
 from .elements import (  # noqa: F401
     MAXORD, C_LIGHT, AbstractCavity, AbstractDrift, AbstractElement, AbstractMagnet, Drift, ERBend,
     ExactSBend, ExactSBendLinearEdge, Marker, Octupole, Quadrupole, RBend, RFCavity, SBend, Sextupole,
-    ThinMultipole, TimeVaryingElement, get_length,
+    ThinMultipole, TimeVaryingElement, get_length, timed,
 )
 from .lattice import (  # noqa: F401
     M_ELECTRON, M_PROTON, Beam, Lattice, beti, findelem, materialize_lattice, spos, total_length,

@@ -36,7 +36,7 @@ __all__ = [
     "MAXORD", "AbstractElement", "AbstractMagnet", "AbstractDrift", "AbstractCavity",
     "Drift", "Marker", "Quadrupole", "Sextupole", "Octupole", "ThinMultipole",
     "SBend", "RBend", "ExactSBend", "ERBend", "ExactSBendLinearEdge", "RFCavity",
-    "TimeVaryingElement", "C_LIGHT", "get_length",
+    "TimeVaryingElement", "timed", "C_LIGHT", "get_length",
 ]
 
 # src/elements/rf.jl:349
@@ -308,25 +308,43 @@ class RFCavity(AbstractCavity):
 
 
 class TimeVaryingElement(AbstractElement):
-    """Element with time- or turn-dependent parameters (src/time_dependence.jl:110-155).
+    """Element with time- or turn-dependent parameters (src/time_dependence.jl:127-136).
 
-    Present so that lattices can carry one; the device path refuses it, as the reference's
-    GPU adaptation does (src/time_dependence.jl:137-139): materialise the lattice first.
+    ``params`` maps field names to numbers or to callables ``f(time, turn)`` (the
+    reference's ``TimeDependentParam`` of a ``TimeContext``).  The device table resolves the
+    element once per turn (``compile_table(..., turn=, nturns=, time=, dt_turn=)``), as
+    ``linepass!`` does at the start of every call (src/lattice.jl:267).
     """
 
     def __init__(self, base: AbstractElement, params: dict):
+        for k in params:
+            if not hasattr(base, k):
+                raise ValueError(f"{type(base).__name__} has no field {k!r}")  # src/time_dependence.jl:150
         self.base = base
         self.params = dict(params)
         self.name = getattr(base, "name", "TVE")
 
     def materialize(self, time: float = 0.0, turn: int = 0) -> AbstractElement:
-        """Static copy with every callable parameter evaluated at (time, turn)."""
+        """Static copy with every parameter evaluated at (time, turn) and converted to the
+        field's type (src/time_dependence.jl:163-174)."""
         import copy
 
         e = copy.deepcopy(self.base)
         for k, f in self.params.items():
-            setattr(e, k, float(f(time, turn)) if callable(f) else f)
+            v = f(time, turn) if callable(f) else f
+            old = getattr(self.base, k)
+            if isinstance(old, (bool, int)) and not isinstance(old, float):
+                v = int(v)
+            elif isinstance(old, float):
+                v = float(v)
+            setattr(e, k, v)
         return e
+
+
+def timed(elem: AbstractElement, **params) -> TimeVaryingElement:
+    """timed(elem; kwargs...) (src/time_dependence.jl:141-155): any field of ``elem`` given as
+    a number or as a callable ``f(time, turn)``."""
+    return TimeVaryingElement(elem, params)
 
 
 def get_length(elem: AbstractElement) -> float:

@@ -139,8 +139,9 @@ def _record(e: AbstractElement, bt: float):
     """(tag, nsteps, max_order, flags, gorder, gscale, block, misalign_block) for one element."""
     if isinstance(e, TimeVaryingElement):
         raise ValueError(
-            f"element {e.name!r} is time-varying: call materialize_lattice first "
-            "(the device path needs a static lattice, src/time_dependence.jl:137-139)")
+            f"element {e.name!r} is time-varying: compile_table resolves it per turn when given "
+            "the turn range (turn=, nturns=, time=, dt_turn=), or call materialize_lattice first "
+            "(src/time_dependence.jl:137-139)")
     if isinstance(e, Marker):
         return TAGS["MARKER"], 0, 0, 0, -1, 0.0, [], None
     if isinstance(e, Drift):
@@ -236,21 +237,29 @@ def _record(e: AbstractElement, bt: float):
 
 def compile_table(lat: Lattice, beam: Beam, *, aperture: Optional[float] = None,
                   exact_hamiltonian: bool = True, coord_limit: float = 1.0,
-                  angle_limit: float = 1.0, info: Optional[TableInfo] = None) -> bytes:
-    """Compile a static lattice into the device table blob.
+                  angle_limit: float = 1.0, info: Optional[TableInfo] = None,
+                  turn: int = 0, nturns: Optional[int] = None, time: float = 0.0,
+                  dt_turn: float = 0.0) -> bytes:
+    """Compile a lattice into the device table blob.
 
     ``aperture`` is the radius R of an optional circular aperture (x^2 + y^2 > R^2 at an
     element exit marks the particle lost, in addition to check_lost).  ``exact_hamiltonian``
     mirrors the reference's global ``USE_EXACT_HAMILTONIAN`` (src/tracking.jl:38).
+
+    A lattice with ``TimeVaryingElement``s is resolved per turn, as ``ringpass!`` resolves it
+    in every ``linepass!`` call (src/lattice.jl:267, 305-317): row k of the table is the
+    lattice at turn ``turn + k`` and time ``time`` advanced by ``dt_turn`` k times, for
+    ``nturns`` rows (one row, at (time, turn), when ``nturns`` is None).  Static elements
+    share one record across rows; identical materialised elements share records too.
     """
     bt = beti(beam)
     pool: List[float] = []
     recs: List[Tuple[int, ...]] = []
     rec_index: Dict[tuple, int] = {}
-    seq: List[int] = []
     if info is not None:
         info.seq, info.gorder, info.gscale = [], [], []
-    for e in lat.elements:
+
+    def rec_of(e):
         tag, n, m, flags, gorder, gscale, blk, mblk = _record(e, bt)
         key = (tag, n, m, flags, gorder, tuple(struct.pack("<%dd" % len(blk), *blk)),
                None if mblk is None else tuple(struct.pack("<84d", *mblk)))
@@ -265,14 +274,26 @@ def compile_table(lat: Lattice, beam: Beam, *, aperture: Optional[float] = None,
             idx = len(recs)
             recs.append((tag, n, m, off, flags, moff, gorder, 0))
             rec_index[key] = idx
-        seq.append(idx)
-        if info is not None:
-            info.seq.append(idx)
-            info.gorder.append(gorder)
-            info.gscale.append(gscale)
+        return idx, gorder, gscale
+
+    varying = [isinstance(e, TimeVaryingElement) for e in lat.elements]
+    rows = 1 if (not any(varying) or nturns is None) else max(int(nturns), 1)
+    static = [None if v else rec_of(e) for e, v in zip(lat.elements, varying)]
+    seq: List[int] = []
+    t = float(time)
+    for k in range(rows):
+        for i, (e, st) in enumerate(zip(lat.elements, static)):
+            r = st if st is not None else rec_of(e.materialize(t, int(turn) + k))
+            seq.append(r[0])
+            if info is not None and k == 0:
+                info.seq.append(r[0])
+                info.gorder.append(r[1])
+                info.gscale.append(r[2])
+        t += float(dt_turn)  # accumulated as ringpass! does
+    n_rows, row_turn0 = (rows, int(turn)) if rows > 1 else (0, 0)
     hdr = struct.pack(
-        "<8I4d", TP_TABLE_MAGIC, TP_TABLE_VERSION, len(seq), len(recs), len(pool),
-        H_EXACT_HAMILTONIAN if exact_hamiltonian else 0, 0, 0,
+        "<6IIi4d", TP_TABLE_MAGIC, TP_TABLE_VERSION, len(lat.elements), len(recs), len(pool),
+        H_EXACT_HAMILTONIAN if exact_hamiltonian else 0, n_rows, row_turn0,
         bt, float(coord_limit), float(angle_limit),
         0.0 if aperture is None else float(aperture) * float(aperture))
     seq_b = struct.pack("<%di" % len(seq), *seq)
@@ -281,5 +302,5 @@ def compile_table(lat: Lattice, beam: Beam, *, aperture: Optional[float] = None,
     rec_b = b"".join(struct.pack("<4iI3i", *r) for r in recs)
     pool_b = struct.pack("<%dd" % len(pool), *pool)
     if info is not None:
-        info.n_seq, info.n_rec, info.pool_len = len(seq), len(recs), len(pool)
+        info.n_seq, info.n_rec, info.pool_len = len(lat.elements), len(recs), len(pool)
     return hdr + seq_b + rec_b + pool_b

@@ -232,10 +232,10 @@ def ringpass_(coords: np.ndarray, lat: Lattice, beam: Beam, lost_flags: np.ndarr
               lost_turn: Optional[np.ndarray] = None, lost_elem: Optional[np.ndarray] = None) -> None:
     """In-place multi-turn tracking on host arrays (ringpass!, src/lattice.jl:305-317).
 
-    The lattice must be static; ``time`` and ``dt_turn`` are accepted for signature parity
-    and have no effect on a static lattice (src/time_dependence.jl:177-178).
+    Time-varying elements are resolved at every turn k at time ``time`` advanced by
+    ``dt_turn`` k times and turn ``turn + k``, as ``ringpass!`` passes them to ``linepass!``;
+    on a static lattice ``time`` and ``dt_turn`` have no effect.
     """
-    del time, dt_turn
     if not isinstance(coords, np.ndarray) or coords.dtype != np.float64 or coords.ndim != 2 or coords.shape[1] != 6:
         raise TypeError("coords must be a float64 numpy array of shape (N, 6)")
     if not isinstance(lost_flags, np.ndarray) or lost_flags.dtype != np.int64 or lost_flags.shape != (coords.shape[0],):
@@ -244,7 +244,8 @@ def ringpass_(coords: np.ndarray, lat: Lattice, beam: Beam, lost_flags: np.ndarr
     if nturns < 0:
         raise ValueError(f"nturns = {nturns} < 0")
     ctx = ctx or default_context()
-    tab = ctx.table(compile_table(lat, beam, aperture=aperture))
+    tab = ctx.table(compile_table(lat, beam, aperture=aperture, turn=turn, nturns=nturns, time=time,
+                                  dt_turn=dt_turn))
     n = coords.shape[0]
     work = coords if coords.flags.f_contiguous else np.asfortranarray(coords)
     lt = np.empty(n, dtype=np.int32) if lost_turn is None else lost_turn
@@ -264,7 +265,8 @@ def linepass_(coords: np.ndarray, lat: Lattice, beam: Beam, lost_flags: np.ndarr
 
 
 def track_(particles: HIPParticles, lat, beam: Optional[Beam] = None, nturns: int = 1, *,
-           turn: int = 0, fast: bool = False, timed: bool = False, aperture: Optional[float] = None) -> None:
+           turn: int = 0, fast: bool = False, timed: bool = False, aperture: Optional[float] = None,
+           time: float = 0.0, dt_turn: float = 0.0) -> None:
     """Track device-resident particles for ``nturns`` turns (the north-star ``track!``).
 
     ``lat`` is a :class:`Lattice` (compiled with ``beam``) or an already uploaded
@@ -276,6 +278,7 @@ def track_(particles: HIPParticles, lat, beam: Optional[Beam] = None, nturns: in
     else:
         if beam is None:
             raise ValueError("track_ needs a Beam to compile a Lattice")
-        tab = ctx.table(compile_table(lat, beam, aperture=aperture))
+        tab = ctx.table(compile_table(lat, beam, aperture=aperture, turn=turn, nturns=nturns, time=time,
+                                      dt_turn=dt_turn))
     L.check(ctx._lib.tp_track(ctx.handle, tab.handle, particles.handle, int(turn), int(nturns),
                               _flags(fast, timed)))
