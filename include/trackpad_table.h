@@ -48,6 +48,30 @@
  *                [24..] poly
  *   TP_RFCAV     [0] L/2 [1] (L/2)*beti [2] volt/energy [3] 2pi*freq [4] lag [5] philag
  *                [6] beta*beta
+ *   TP_SOLENOID  [0] L [1] ks  (ks = 0 is written as TP_DRIFT)            src/tracking.jl:529-566
+ *   TP_CORRECTOR [0] L [1] xkick [2] ykick [3] xkick^2/3 + ykick^2/3 [4] xkick/2 [5] ykick/2
+ *                                                                         src/tracking.jl:579-605
+ *   TP_TRANSLATION [0] dx [1] dy [2] ds                                   src/tracking.jl:1182-1192
+ *   TP_YROTATION [0] cos(a) [1] sin(a) [2] tan(a), a = -angle (0 is written as TP_MARKER)
+ *                                                                         src/tracking.jl:1194-1218
+ *   TP_CRAB      [0] L/2 [1] (L/2)*beti [2] volt'/E [3] k [4] phi' [5] k*volt'/E*beti
+ *                (volt' = volt(1+err1), phi' = phi+err2, E = max(|energy|, eps))
+ *                                                                         src/tracking.jl:1531-1547
+ *   TP_ACCEL     [0] volt/(beta^2 E) [1] k [2] phis [3] sin(phis)        src/tracking.jl:1549-1559
+ *   TP_LONGRF    [0] 2pi*h*eta/k (eta = alphac - (1 - beta^2))            src/tracking.jl:1575-1585
+ *   TP_BOOST     [0] tan(angle) [1] 1/cos(angle)                          src/tracking.jl:1587-1595
+ *   TP_INVBOOST  [0] sin(angle) [1] cos(angle)                            src/tracking.jl:1597-1604
+ *   TP_GAUSSBB   [0] sx^2 [1] sy^2 [2] factor, then (weight, xoff, yoff) for each of
+ *                rec.nsteps slices (StrongThinGaussianBeam: one slice, factor 1)
+ *                                                                         src/tracking.jl:1610-1641
+ *   TP_RLCWAKE   [0] scale [1] Rshunt*w0/Q0 [2] w0' [3] 2*Q0' [4] w0 [5] 2*Q0
+ *                (w0 = 2pi*freq, Q0' = sqrt(Q0^2 - 1/4), w0' = w0/Q0*Q0')  src/tracking.jl:1647-1654
+ *   TP_TABWAKE   [0] scale [1] fliphalf [2] times[0]*fliphalf, then times[n] and wakes[n],
+ *                n = rec.nsteps                                           src/tracking.jl:1656-1663
+ *   TP_LBEND     [0] L [1] K [2] angle [3] by_error [4] angle/L, then the entrance and exit
+ *                edge blocks (fx, fy, e, fc) of TP_SBEND at [5] and [9]   src/tracking.jl:1076-1170
+ *   TP_NAN       (none): every coordinate becomes NaN -- the reference's _nan6 for SBendSC and
+ *                LBend of zero length (src/tracking.jl:1047-1049, 1158-1160)
  *
  *   Misalignment block at rec.moff (present when any TP_F_T1/T2/R1/R2 flag is set):
  *                [0..6) t1  [6..12) t2  [12..48) r1  [48..84) r2   (r1, r2 column-major)
@@ -70,8 +94,25 @@ enum tp_tag {
   TP_ESBEND = 5,     /* ExactSBend / ERBend: yrot, Forest fringe, wedge, exact body */
   TP_ESBEND_LE = 6,  /* exact body with the linear edge maps of SBend */
   TP_RFCAV = 7,      /* RFCavity */
-  TP_NTAGS = 8
+  TP_SOLENOID = 8,   /* Solenoid with ks != 0 */
+  TP_CORRECTOR = 9,  /* Corrector, HKicker, VKicker */
+  TP_TRANSLATION = 10,
+  TP_YROTATION = 11,
+  TP_CRAB = 12,      /* CrabCavity */
+  TP_ACCEL = 13,     /* AccelCavity */
+  TP_LONGRF = 14,    /* LongitudinalRFMap */
+  TP_BOOST = 15,     /* LorentzBoost (mode 0) */
+  TP_INVBOOST = 16,  /* InvLorentzBoost (mode 0) */
+  TP_GAUSSBB = 17,   /* StrongThinGaussianBeam, StrongGaussianBeam */
+  TP_RLCWAKE = 18,   /* LongitudinalRLCWake */
+  TP_TABWAKE = 19,   /* LongitudinalWake (tabulated) */
+  TP_LBEND = 20,     /* LBend */
+  TP_NAN = 21,       /* the reference's NaN map */
+  TP_NTAGS = 22
 };
+/* Also written with the tags above: DriftSC (TP_DRIFT), QuadrupoleSC / SextupoleSC /
+ * OctupoleSC (TP_MPOLE), SBendSC / RBendSC (TP_SBEND), SpaceCharge (TP_MARKER: its
+ * single-particle map is the identity, src/tracking.jl:1176-1180). */
 
 /* rec.flags */
 #define TP_F_T1 0x1u

@@ -21,6 +21,10 @@ NP = 11  # MAXORD + 1
 
 ORC_MARKER, ORC_DRIFT, ORC_QUAD, ORC_SEXT, ORC_OCT, ORC_THINMP, ORC_SBEND, ORC_EXACTSBEND, \
     ORC_RFCAVITY, ORC_EXACTSBEND_LE = range(10)
+(ORC_SOLENOID, ORC_CORRECTOR, ORC_TRANSLATION, ORC_YROTATION, ORC_CRAB, ORC_ACCEL, ORC_LONGRF, ORC_BOOST,
+ ORC_INVBOOST, ORC_THINBB, ORC_SGB, ORC_RLCWAKE, ORC_TABWAKE, ORC_DRIFTSC, ORC_QUADSC, ORC_SEXTSC, ORC_OCTSC,
+ ORC_SBENDSC, ORC_LBEND, ORC_SPACECHARGE) = range(10, 30)
+NX = 128
 
 
 class OrcElem(C.Structure):
@@ -35,7 +39,7 @@ class OrcElem(C.Structure):
         ("t1", C.c_double * 6), ("t2", C.c_double * 6),
         ("r1", C.c_double * 36), ("r2", C.c_double * 36),
         ("volt", C.c_double), ("freq", C.c_double), ("h", C.c_double), ("lag", C.c_double),
-        ("philag", C.c_double), ("energy", C.c_double),
+        ("philag", C.c_double), ("energy", C.c_double), ("x", C.c_double * NX),
     ]
 
 
@@ -74,7 +78,61 @@ def _kind(e) -> int:
     return {"Marker": ORC_MARKER, "Drift": ORC_DRIFT, "Quadrupole": ORC_QUAD, "Sextupole": ORC_SEXT,
             "Octupole": ORC_OCT, "ThinMultipole": ORC_THINMP, "SBend": ORC_SBEND,
             "ExactSBend": ORC_EXACTSBEND, "RFCavity": ORC_RFCAVITY,
-            "ExactSBendLinearEdge": ORC_EXACTSBEND_LE}[n]
+            "ExactSBendLinearEdge": ORC_EXACTSBEND_LE, "Solenoid": ORC_SOLENOID, "Corrector": ORC_CORRECTOR,
+            "Translation": ORC_TRANSLATION, "YRotation": ORC_YROTATION, "CrabCavity": ORC_CRAB,
+            "AccelCavity": ORC_ACCEL, "LongitudinalRFMap": ORC_LONGRF, "LorentzBoost": ORC_BOOST,
+            "InvLorentzBoost": ORC_INVBOOST, "StrongThinGaussianBeam": ORC_THINBB, "StrongGaussianBeam": ORC_SGB,
+            "LongitudinalRLCWake": ORC_RLCWAKE, "LongitudinalWake": ORC_TABWAKE, "DriftSC": ORC_DRIFTSC,
+            "QuadrupoleSC": ORC_QUADSC, "SextupoleSC": ORC_SEXTSC, "OctupoleSC": ORC_OCTSC, "SBendSC": ORC_SBENDSC,
+            "LBend": ORC_LBEND, "SpaceCharge": ORC_SPACECHARGE}[n]
+
+
+def _extra_fields(e, o) -> None:
+    """Per-kind extra fields in orc_elem.x (the raw fields of the reference structs)."""
+    n = type(e).__name__
+    x = []
+    if n == "Solenoid":
+        o.k = e.ks
+    elif n == "Corrector":
+        x = [e.xkick, e.ykick]
+    elif n == "Translation":
+        x = [e.dx, e.dy, e.ds]
+    elif n == "CrabCavity":
+        x = [e.k, e.phi, float(e.errors[0]), float(e.errors[1])]
+    elif n == "AccelCavity":
+        x = [e.k, e.phis]
+    elif n == "LongitudinalRFMap":
+        rf = e.rf
+        kind = {"RFCavity": 1, "AccelCavity": 2, "CrabCavity": 3}.get(type(rf).__name__, 0)
+        x = [e.alphac, kind, getattr(rf, "freq", 0.0), getattr(rf, "k", 0.0), getattr(rf, "h", 1.0)]
+    elif n == "LorentzBoost":
+        x = [e.cosang, e.tanang, e.mode]
+    elif n == "InvLorentzBoost":
+        x = [e.sinang, e.cosang, e.mode]
+    elif n == "StrongThinGaussianBeam":
+        x = [e.amplitude, e.rmssizex, e.rmssizey, e.xoffset, e.yoffset]
+    elif n == "StrongGaussianBeam":
+        k = int(e.nzslice)
+        if 3 * k + 8 > NX:
+            raise ValueError("the oracle holds at most 40 beam-beam slices")
+        w = [float(e.zslice_npar[i]) if i < len(e.zslice_npar) else 1.0 / k for i in range(k)]
+        xo = [float(e.xoffsets[i]) if i < len(e.xoffsets) else 0.0 for i in range(k)]
+        yo = [float(e.yoffsets[i]) if i < len(e.yoffsets) else 0.0 for i in range(k)]
+        x = [e.charge, e.total_energy, float(e.beamsize[0]), float(e.beamsize[1]), k, 0.0, 0.0, 0.0] + w + xo + yo
+    elif n == "LongitudinalRLCWake":
+        x = [e.freq, e.Rshunt, e.Q0, e.scale]
+    elif n == "LongitudinalWake":
+        k = len(e.times)
+        if 2 * k + 3 > NX:
+            raise ValueError("the oracle holds at most 62 wake samples")
+        x = [e.scale, e.fliphalf, k] + list(map(float, e.times)) + list(map(float, e.wakefields))
+    elif n in ("QuadrupoleSC", "SextupoleSC", "OctupoleSC"):
+        x = [e.k0, e.k1, e.k2, e.k3]
+    elif n == "LBend":
+        x = [e.K, e.by_error]
+        o.gap = e.full_gap
+    for i, v in enumerate(x):
+        o.x[i] = float(v)
 
 
 def to_orc(e) -> OrcElem:
@@ -110,6 +168,7 @@ def to_orc(e) -> OrcElem:
             arr = getattr(o, f)
             for i, x in enumerate(M.reshape(-1, order="F")):
                 arr[i] = float(x)
+    _extra_fields(e, o)
     return o
 
 

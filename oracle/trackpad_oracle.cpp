@@ -74,6 +74,7 @@ template <int K> inline Dual<K> operator/(double a, const Dual<K>& b) { return D
 template <int K> inline bool operator<(const Dual<K>& a, const Dual<K>& b) { return a.v < b.v; }
 template <int K> inline bool operator>(const Dual<K>& a, const Dual<K>& b) { return a.v > b.v; }
 template <int K> inline bool operator<=(const Dual<K>& a, const Dual<K>& b) { return a.v <= b.v; }
+template <int K> inline bool operator>=(const Dual<K>& a, const Dual<K>& b) { return a.v >= b.v; }
 template <int K> inline bool operator==(const Dual<K>& a, const Dual<K>& b) { return a.v == b.v; }
 template <int K> inline bool operator!=(const Dual<K>& a, const Dual<K>& b) { return a.v != b.v; }
 template <int K> inline Dual<K> chain(const Dual<K>& a, double fv, double dfdx) { Dual<K> r(fv); for (int k = 0; k < K; ++k) r.d[k] = dfdx * a.d[k]; return r; }
@@ -84,6 +85,9 @@ template <int K> inline Dual<K> tan(const Dual<K>& a) { double t = std::tan(a.v)
 template <int K> inline Dual<K> asin(const Dual<K>& a) { return chain(a, std::asin(a.v), 1.0 / std::sqrt(1.0 - a.v * a.v)); }
 template <int K> inline Dual<K> atan(const Dual<K>& a) { return chain(a, std::atan(a.v), 1.0 / (1.0 + a.v * a.v)); }
 template <int K> inline Dual<K> fabs(const Dual<K>& a) { return a.v < 0 ? -a : a; }
+template <int K> inline Dual<K> exp(const Dual<K>& a) { double e = std::exp(a.v); return chain(a, e, e); }
+template <int K> inline Dual<K> sinh(const Dual<K>& a) { return chain(a, std::sinh(a.v), std::cosh(a.v)); }
+template <int K> inline Dual<K> cosh(const Dual<K>& a) { return chain(a, std::cosh(a.v), std::sinh(a.v)); }
 template <int K> inline bool isnan_(const Dual<K>& a) { return std::isnan(a.v); }
 template <int K> inline double val(const Dual<K>& a) { return a.v; }
 
@@ -92,6 +96,7 @@ inline double val(double a) { return a; }
 inline bool isnan_(long double a) { return std::isnan(a); }
 inline double val(long double a) { return (double)a; }
 using std::sqrt; using std::sin; using std::cos; using std::tan; using std::asin; using std::atan; using std::fabs;
+using std::exp; using std::sinh; using std::cosh;
 
 // Quad precision (__float128, libquadmath): the yardstick for finite differences, whose
 // rounding noise must stay far below the differences being resolved.
@@ -103,6 +108,9 @@ inline f128 tan(f128 a) { return tanq(a); }
 inline f128 asin(f128 a) { return asinq(a); }
 inline f128 atan(f128 a) { return atanq(a); }
 inline f128 fabs(f128 a) { return fabsq(a); }
+inline f128 exp(f128 a) { return expq(a); }
+inline f128 sinh(f128 a) { return sinhq(a); }
+inline f128 cosh(f128 a) { return coshq(a); }
 inline bool isnan_(f128 a) { return isnanq(a) != 0; }
 inline double val(f128 a) { return (double)a; }
 
@@ -122,7 +130,31 @@ enum OrcKind {
   ORC_RFCAVITY = 8,       // src/elements/rf.jl:6-21
   ORC_EXACTSBEND_LE = 9,  // exact body (src/tracking.jl:858-894) with the linear edge maps
                           // of src/tracking.jl:160-221 (no reference element; composition)
+  // the other families; their extra fields travel in orc_elem.x (listed per kind below)
+  ORC_SOLENOID = 10,      // src/elements/solenoid.jl:6-26       L, k = ks
+  ORC_CORRECTOR = 11,     // src/elements/corrector.jl:6-27      L, x = xkick, ykick
+  ORC_TRANSLATION = 12,   // src/elements/SCelements.jl:420-433  x = dx, dy, ds
+  ORC_YROTATION = 13,     // src/elements/SCelements.jl:444-455  angle
+  ORC_CRAB = 14,          // src/elements/rf.jl:35-54            L, volt, energy, x = k, phi, err1, err2
+  ORC_ACCEL = 15,         // src/elements/rf.jl:69-87            volt, energy, x = k, phis
+  ORC_LONGRF = 16,        // src/elements/rf.jl:102-105          x = alphac, rf kind, rf freq, rf k, rf h
+  ORC_BOOST = 17,         // src/elements/beambeam.jl:6-19       x = cosang, tanang, mode
+  ORC_INVBOOST = 18,      // src/elements/beambeam.jl:30-43      x = sinang, cosang, mode
+  ORC_THINBB = 19,        // src/elements/beambeam.jl:54-69      x = amplitude, sx, sy, xoff, yoff
+  ORC_SGB = 20,           // src/elements/beambeam.jl:83-119     x = charge, total_energy, sx, sy, n,
+                          //                                     [8..) npar[n], xoff[n], yoff[n]
+  ORC_RLCWAKE = 21,       // src/elements/wake.jl:6-36           x = freq, Rshunt, Q0, scale
+  ORC_TABWAKE = 22,       // src/elements/wake.jl:44-91          x = scale, fliphalf, n, times[n], wakes[n]
+  ORC_DRIFTSC = 23,       // src/elements/SCelements.jl:6-38     L
+  ORC_QUADSC = 24,        // src/elements/SCelements.jl:51-101   L, x = k0..k3, polynom_a, ...
+  ORC_SEXTSC = 25,        // src/elements/SCelements.jl:116-166
+  ORC_OCTSC = 26,         // src/elements/SCelements.jl:181-231
+  ORC_SBENDSC = 27,       // src/elements/SCelements.jl:246-311  as SBend
+  ORC_LBEND = 28,         // src/elements/SCelements.jl:346-379  L, angle, e1, e2, fint1, fint2, gap =
+                          //                                     full_gap, x = K, by_error
+  ORC_SPACECHARGE = 29,   // src/elements/SCelements.jl:394-408
 };
+#define ORC_NX 128
 
 extern "C" {
 typedef struct orc_elem {
@@ -134,6 +166,7 @@ typedef struct orc_elem {
   double t1[6], t2[6];
   double r1[36], r2[36];  // column-major, as SMatrix{6,6}
   double volt, freq, h, lag, philag, energy;
+  double x[ORC_NX];  // per-kind extra fields (OrcKind)
 } orc_elem;
 }
 
@@ -146,6 +179,7 @@ struct Elem {
   T t1[6], t2[6];
   T r1[36], r2[36];
   T volt, freq, h, lag, philag, energy;
+  T x[ORC_NX];
 };
 
 template <class T>
@@ -159,6 +193,7 @@ static void load_elem(const orc_elem& e, Elem<T>& o) {
   for (int i = 0; i < 6; ++i) { o.t1[i] = T(e.t1[i]); o.t2[i] = T(e.t2[i]); }
   for (int i = 0; i < 36; ++i) { o.r1[i] = T(e.r1[i]); o.r2[i] = T(e.r2[i]); }
   o.volt = T(e.volt); o.freq = T(e.freq); o.h = T(e.h); o.lag = T(e.lag); o.philag = T(e.philag); o.energy = T(e.energy);
+  for (int i = 0; i < ORC_NX; ++i) o.x[i] = T(e.x[i]);
 }
 
 template <class T> struct R6 { T v[6]; };
@@ -477,6 +512,307 @@ static inline void magnet_polys(const Elem<T>& e, T* pa, T* pb, bool kick_corr) 
   }
 }
 
+static const double EPS64 = 2.220446049250313e-16;      // eps(Float64)
+static const double SQRT_EPS64 = 1.4901161193847656e-08;  // sqrt(eps(Float64))
+
+template <class T> static inline T tmax(T a, T b) { return a > b ? a : b; }
+
+// src/elements/wake.jl:67-84
+template <class T>
+static T linear_interpolate(T x, const T* xs, const T* ys, int n) {
+  if (x <= xs[0]) {
+    T slope = (ys[1] - ys[0]) / (xs[1] - xs[0]);
+    return ys[0] + slope * (x - xs[0]);
+  } else if (x >= xs[n - 1]) {
+    T slope = (ys[n - 1] - ys[n - 2]) / (xs[n - 1] - xs[n - 2]);
+    return ys[n - 1] + slope * (x - xs[n - 1]);
+  }
+  for (int i = 1; i < n; ++i) {
+    if (x < xs[i]) {
+      T slope = (ys[i] - ys[i - 1]) / (xs[i] - xs[i - 1]);
+      return ys[i - 1] + slope * (x - xs[i - 1]);
+    }
+  }
+  return ys[n - 1];
+}
+
+// _lbend_body: src/tracking.jl:1076-1150
+template <class T>
+static R6<T> lbend_body(R6<T> r, T L, T grd, T b_angle, T by_error) {
+  if (L == T(0.0)) return r;
+  T p_norm = T(1.0) / (T(1.0) + r.v[5]);
+  T Kx = b_angle / L;
+  T G1 = (Kx * Kx + grd) * p_norm;
+  T G2 = -grd * p_norm;
+  T tol = T(SQRT_EPS64);
+  T MHD = T(1.0), M12 = L, M21 = T(0.0);
+  T MVD = T(1.0), M34 = L, M43 = T(0.0);
+  T arg1 = T(0.0), arg2 = T(0.0), sqrtG1 = T(0.0), sqrtG2 = T(0.0);
+  if (fabs(G1) >= tol) {
+    if (G1 > T(0.0)) {
+      sqrtG1 = sqrt(G1); arg1 = L * sqrtG1;
+      MHD = cos(arg1); M12 = sin(arg1) / sqrtG1; M21 = -sin(arg1) * sqrtG1;
+    } else {
+      sqrtG1 = sqrt(-G1); arg1 = L * sqrtG1;
+      MHD = cosh(arg1); M12 = sinh(arg1) / sqrtG1; M21 = sinh(arg1) * sqrtG1;
+    }
+  }
+  if (fabs(G2) >= tol) {
+    if (G2 > T(0.0)) {
+      sqrtG2 = sqrt(G2); arg2 = L * sqrtG2;
+      MVD = cos(arg2); M34 = sin(arg2) / sqrtG2; M43 = -sin(arg2) * sqrtG2;
+    } else {
+      sqrtG2 = sqrt(-G2); arg2 = L * sqrtG2;
+      MVD = cosh(arg2); M34 = sinh(arg2) / sqrtG2; M43 = sinh(arg2) * sqrtG2;
+    }
+  }
+  T x = r.v[0], xpr = r.v[1] * p_norm, y = r.v[2], ypr = r.v[3] * p_norm, delta = r.v[5];
+  T dterm = delta * p_norm - by_error;
+  T x_new = MHD * x + M12 * xpr;
+  T px_new = (M21 * x + MHD * xpr) / p_norm;
+  if (fabs(G1) < tol) {
+    x_new = x_new + dterm * L * L * Kx / T(2.0);
+    px_new = px_new + dterm * L * Kx / p_norm;
+  } else if (G1 > T(0.0)) {
+    x_new = x_new + dterm * (T(1.0) - cos(arg1)) * Kx / G1;
+    px_new = px_new + dterm * sin(arg1) * Kx / (sqrtG1 * p_norm);
+  } else {
+    x_new = x_new + dterm * (T(1.0) - cosh(arg1)) * Kx / G1;
+    px_new = px_new + dterm * sinh(arg1) * Kx / (sqrtG1 * p_norm);
+  }
+  T y_new = MVD * y + M34 * ypr;
+  T py_new = (M43 * y + MVD * ypr) / p_norm;
+  T z_new = r.v[4] + xpr * xpr * (L + MHD * M12) / T(4.0);
+  if (fabs(G1) >= tol) {
+    z_new = z_new + (L - MHD * M12) * (x * x * G1 + dterm * dterm * Kx * Kx / G1 - T(2.0) * x * Kx * dterm) / T(4.0);
+    z_new = z_new + M12 * M21 * (x * xpr - xpr * dterm * Kx / G1) / T(2.0);
+    z_new = z_new + (Kx * x * M12 + xpr * (T(1.0) - MHD) * Kx / G1 + dterm * (L - M12) * Kx * Kx / G1);
+  }
+  z_new = z_new + ((L - MVD * M34) * y * y * G2 + ypr * ypr * (L + MVD * M34)) / T(4.0);
+  z_new = z_new + M34 * M43 * y * ypr / T(2.0);
+  R6<T> o = {{x_new, px_new, y_new, py_new, z_new, r.v[5]}};
+  return o;
+}
+
+// pass! of the other element families: src/tracking.jl:529-605, 981-1218, 1531-1663
+template <class T>
+static R6<T> pass_extra(const Elem<T>& e, R6<T> r, T beti, bool exact) {
+  switch (e.kind) {
+    case ORC_SOLENOID: {  // :529-566
+      r = entrance(e, r);
+      T ks = e.k, L = e.L;
+      if (ks == T(0.0)) {
+        r = drift6(r, L, beti, exact);
+      } else {
+        T p_norm = T(1.0) / (T(1.0) + r.v[5]);
+        T x = r.v[0], xpr = r.v[1] * p_norm, y = r.v[2], ypr = r.v[3] * p_norm;
+        T H = ks * p_norm / T(2.0);
+        T S = sin(L * H), C = cos(L * H);
+        T x_new = x * C * C + xpr * C * S / H + y * C * S + ypr * S * S / H;
+        T px_new = (-x * H * C * S + xpr * C * C - y * H * S * S + ypr * C * S) / p_norm;
+        T y_new = -x * C * S - xpr * S * S / H + y * C * C + ypr * C * S / H;
+        T py_new = (x * H * S * S - xpr * C * S - y * C * S * H + ypr * C * C) / p_norm;
+        T z_new = r.v[4] + L * (H * H * (x * x + y * y) + T(2.0) * H * (xpr * y - ypr * x) + xpr * xpr + ypr * ypr) / T(2.0);
+        R6<T> o = {{x_new, px_new, y_new, py_new, z_new, r.v[5]}};
+        r = o;
+      }
+      return exit_(e, r);
+    }
+    case ORC_CORRECTOR: {  // :579-605
+      r = entrance(e, r);
+      T xk = e.x[0], yk = e.x[1];
+      T p_norm = T(1.0) / (T(1.0) + r.v[5]);
+      T NormL = e.L * p_norm;
+      T z_new = r.v[4] + NormL * p_norm *
+                (xk * xk / T(3.0) + yk * yk / T(3.0) + r.v[1] * r.v[1] + r.v[3] * r.v[3] + r.v[1] * xk + r.v[3] * yk) / T(2.0);
+      T x_new = r.v[0] + NormL * (r.v[1] + xk / T(2.0));
+      T px_new = r.v[1] + xk;
+      T y_new = r.v[2] + NormL * (r.v[3] + yk / T(2.0));
+      T py_new = r.v[3] + yk;
+      R6<T> o = {{x_new, px_new, y_new, py_new, z_new, r.v[5]}};
+      return exit_(e, o);
+    }
+    case ORC_TRANSLATION: {  // :1182-1192
+      T pz2 = T(1.0) + T(2.0) * r.v[5] * beti + r.v[5] * r.v[5] - r.v[1] * r.v[1] - r.v[3] * r.v[3];
+      if (pz2 <= T(0.0)) return nan6<T>();
+      T pz = sqrt(pz2);
+      T dx = e.x[0], dy = e.x[1], ds = e.x[2];
+      R6<T> o = r;
+      o.v[0] = r.v[0] - (dx + ds * r.v[1] / pz);
+      o.v[2] = r.v[2] - (dy + ds * r.v[3] / pz);
+      o.v[4] = r.v[4] + ds * (beti + r.v[5]) / pz;
+      return o;
+    }
+    case ORC_YROTATION: {  // :1194-1218
+      T angle = -e.angle;
+      if (angle == T(0.0)) return r;
+      T ca = cos(angle), sa = sin(angle), ta = tan(angle);
+      T pz2 = T(1.0) + T(2.0) * r.v[5] * beti + r.v[5] * r.v[5] - r.v[1] * r.v[1] - r.v[3] * r.v[3];
+      if (pz2 <= T(0.0)) return nan6<T>();
+      T pz = sqrt(pz2);
+      T ptt = T(1.0) - ta * r.v[1] / pz;
+      if (fabs(ptt) < T(SQRT_EPS64)) return nan6<T>();
+      R6<T> o = r;
+      o.v[0] = r.v[0] / (ca * ptt);
+      o.v[1] = ca * r.v[1] + sa * pz;
+      o.v[2] = r.v[2] + ta * r.v[0] * r.v[3] / (pz * ptt);
+      o.v[4] = r.v[4] + ta * r.v[0] * (beti + r.v[5]) / (pz * ptt);
+      return o;
+    }
+    case ORC_CRAB: {  // :1531-1547
+      T k = e.x[0];
+      T volt = e.volt * (T(1.0) + e.x[2]);
+      T phi = e.x[1] + e.x[3];
+      T E = tmax(fabs(e.energy), T(EPS64));
+      T ang = k * r.v[4] + phi;
+      if (e.L > T(0.0)) r = drift6(r, e.L / T(2.0), beti, exact);
+      T px_new = r.v[1] + (volt / E) * sin(ang * beti);
+      T delta_new = r.v[5] - (k * volt / E * beti) * r.v[0] * cos(ang * beti);
+      r.v[1] = px_new;
+      r.v[5] = delta_new;
+      if (e.L > T(0.0)) r = drift6(r, e.L / T(2.0), beti, exact);
+      return r;
+    }
+    case ORC_ACCEL: {  // :1549-1559
+      T beta = T(1.0) / beti;
+      T beta2 = beta * beta;
+      if (beta2 <= T(EPS64)) return r;
+      T E = tmax(fabs(e.energy), T(EPS64));
+      T sv = sin(e.x[0] * r.v[4] + e.x[1]) - sin(e.x[1]);
+      r.v[5] = r.v[5] + (e.volt / (beta2 * E)) * sv;
+      return r;
+    }
+    case ORC_LONGRF: {  // :1561-1585
+      int rk = (int)val(e.x[1]);
+      T k = rk == 1 ? T(TWO_PI) * e.x[2] / T(C_LIGHT) : (rk == 2 || rk == 3 ? e.x[3] : T(0.0));
+      if (fabs(k) <= T(EPS64)) return r;
+      T h = (rk == 1 || rk == 2) ? e.x[4] : T(1.0);
+      T beta = T(1.0) / beti;
+      T eta = e.x[0] - (T(1.0) - beta * beta);
+      r.v[4] = r.v[4] - (T(TWO_PI) * h * eta / k) * r.v[5];
+      return r;
+    }
+    case ORC_BOOST: {  // :1587-1595
+      if (val(e.x[2]) != 0.0) return r;
+      T invcos = T(1.0) / e.x[0];
+      T tanang = e.x[1];
+      T x_new = r.v[0] + tanang * r.v[4];
+      T delta_new = r.v[5] - tanang * r.v[1];
+      R6<T> o = {{x_new, r.v[1] * invcos, r.v[2], r.v[3] * invcos, r.v[4] * invcos, delta_new}};
+      return o;
+    }
+    case ORC_INVBOOST: {  // :1597-1604
+      if (val(e.x[2]) != 0.0) return r;
+      T sinang = e.x[0], cosang = e.x[1];
+      T x_new = r.v[0] - sinang * r.v[4];
+      T delta_new = r.v[5] + sinang * r.v[1];
+      R6<T> o = {{x_new, r.v[1] * cosang, r.v[2], r.v[3] * cosang, r.v[4] * cosang, delta_new}};
+      return o;
+    }
+    case ORC_THINBB: {  // :1610-1619
+      T sx2 = tmax(e.x[1] * e.x[1], T(EPS64));
+      T sy2 = tmax(e.x[2] * e.x[2], T(EPS64));
+      T dx = r.v[0] - e.x[3];
+      T dy = r.v[2] - e.x[4];
+      T g = e.x[0] * exp(T(-0.5) * (dx * dx / sx2 + dy * dy / sy2));
+      r.v[1] = r.v[1] - g * dx / sx2;
+      r.v[3] = r.v[3] - g * dy / sy2;
+      return r;
+    }
+    case ORC_SGB: {  // :1621-1641
+      int n = (int)val(e.x[4]);
+      if (n <= 0) return r;
+      T sx2 = tmax(e.x[2] * e.x[2], T(EPS64));
+      T sy2 = tmax(e.x[3] * e.x[3], T(EPS64));
+      T factor = e.x[0] / tmax(fabs(e.x[1]), T(EPS64));
+      T px = r.v[1], py = r.v[3];
+      for (int i = 0; i < n; ++i) {
+        T w = e.x[8 + i], xoff = e.x[8 + n + i], yoff = e.x[8 + 2 * n + i];
+        T dx = r.v[0] - xoff;
+        T dy = r.v[2] - yoff;
+        T g = w * exp(T(-0.5) * (dx * dx / sx2 + dy * dy / sy2));
+        px = px - factor * g * dx / sx2;
+        py = py - factor * g * dy / sy2;
+      }
+      r.v[1] = px;
+      r.v[3] = py;
+      return r;
+    }
+    case ORC_RLCWAKE: {  // :1647-1654, src/elements/wake.jl:25-36
+      T freq = e.x[0], Rshunt = e.x[1], Q0 = e.x[2], scale = e.x[3];
+      if (scale == T(0.0)) return r;
+      T t = r.v[4] / T(C_LIGHT);
+      if (!(t < T(0.0))) t = T(0.0);  // min(z / c, 0)
+      T Q0p = sqrt(Q0 * Q0 - T(0.25));
+      T w0 = T(TWO_PI) * freq;
+      T w0p = w0 / Q0 * Q0p;
+      T wake = (t > T(0.0)) ? T(0.0)
+                            : Rshunt * w0 / Q0 * (cos(w0p * t) + sin(w0p * t) / (T(2.0) * Q0p)) * exp(w0 * t / (T(2.0) * Q0));
+      r.v[5] = r.v[5] - scale * wake;
+      return r;
+    }
+    case ORC_TABWAKE: {  // :1656-1663, src/elements/wake.jl:86-91
+      T scale = e.x[0], flip = e.x[1];
+      int n = (int)val(e.x[2]);
+      if (scale == T(0.0)) return r;
+      T t = r.v[4] / T(C_LIGHT);
+      const T* ts = e.x + 3;
+      const T* ws = e.x + 3 + n;
+      T wake = (t > ts[0] * flip) ? T(0.0) : linear_interpolate(t * flip, ts, ws, n);
+      r.v[5] = r.v[5] - scale * wake;
+      return r;
+    }
+    case ORC_DRIFTSC:  // :981-991
+      r = entrance(e, r);
+      r = drift6(r, e.L, beti, exact);
+      return exit_(e, r);
+    case ORC_QUADSC:   // :997-1015
+    case ORC_SEXTSC:   // :1017-1027
+    case ORC_OCTSC: {  // :1029-1039
+      r = entrance(e, r);
+      T pa[ORC_NP], pb[ORC_NP];
+      for (int i = 0; i < ORC_NP; ++i) { pa[i] = e.pa[i]; pb[i] = T(0.0); }
+      pb[0] = e.x[0]; pb[1] = e.x[1]; pb[2] = e.x[2] / T(2.0); pb[3] = e.x[3] / T(6.0);
+      if (e.kind == ORC_QUADSC && e.L > T(0.0)) {
+        pb[0] = pb[0] - sin(e.kick_angle[0]) / e.L;
+        pa[0] = pa[0] + sin(e.kick_angle[1]) / e.L;
+      }
+      int m = std::min(std::max(e.max_order, 0), 3);
+      r = symplectic4_pass(r, e.L, pa, pb, m, e.num_int_steps, beti, exact);
+      return exit_(e, r);
+    }
+    case ORC_SBENDSC: {  // :1041-1074
+      r = entrance(e, r);
+      if (e.L == T(0.0)) return nan6<T>();
+      T irho = e.angle / e.L;
+      T pa[ORC_NP], pb[ORC_NP];
+      for (int i = 0; i < ORC_NP; ++i) { pa[i] = e.pa[i]; pb[i] = e.pb[i]; }
+      if (e.L > T(0.0)) {
+        pb[0] = pb[0] - sin(e.kick_angle[0]) / e.L;
+        pa[0] = pa[0] + sin(e.kick_angle[1]) / e.L;
+      }
+      if (e.fbe != 0) r = edge_fringe(r, irho, e.e1, e.fint1, e.gap, e.fbe, false);
+      int m = std::min(std::max(e.max_order, 0), 3);
+      r = symplectic4_bend_pass(r, e.L, pa, pb, irho, m, e.num_int_steps, beti, exact);
+      if (e.fbx != 0) r = edge_fringe(r, irho, e.e2, e.fint2, e.gap, e.fbx, true);
+      return exit_(e, r);
+    }
+    case ORC_LBEND: {  // :1152-1170
+      r = entrance(e, r);
+      if (e.L == T(0.0)) return nan6<T>();
+      T irho = e.angle / e.L;
+      r = edge_fringe(r, irho, e.e1, e.fint1, e.gap, 1, false);
+      r = lbend_body(r, e.L, e.x[0], e.angle, e.x[1]);
+      r = edge_fringe(r, irho, e.e2, e.fint2, e.gap, 1, true);
+      return exit_(e, r);
+    }
+    case ORC_SPACECHARGE:  // :1176-1180
+      return r;
+    default:
+      return nan6<T>();
+  }
+}
+
 // pass!: one element, one particle.  src/tracking.jl:115-141, 386-518, 616-706, 901-973
 template <class T>
 static R6<T> pass_elem(const Elem<T>& e, R6<T> r, T beti, bool exact) {
@@ -564,7 +900,7 @@ static R6<T> pass_elem(const Elem<T>& e, R6<T> r, T beti, bool exact) {
       return r;
     }
     default:
-      return nan6<T>();
+      return pass_extra(e, r, beti, exact);
   }
 }
 

@@ -1,0 +1,143 @@
+"""Device maps of the element families beyond the hot lattices against the oracle.
+
+The reference's element test (test/verify_elements.jl:60-117) runs each element on the
+10 x 6 input at 3.5 GeV and compares with JuTrack to atol 1e-15; here each element runs
+through libtrackpad_hip and is compared with the oracle (pinned in
+tests/test_oracle_elements_extra.py): bit for bit where the map needs only + - * / sqrt on
+coordinates (the parity build), within the reference's 1e-15 where it evaluates
+sin / cos / exp / sinh / cosh per particle (device and host libraries may differ in the
+last place).  The reference's own cases use zero strengths for several elements; each also
+runs with nonzero ones.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from trackpad_hip import (M_ELECTRON, AccelCavity, Beam, Corrector, CrabCavity, Drift, DriftSC, HIPParticles,
+                          HKicker, InvLorentzBoost, Lattice, LBend, LongitudinalRFMap, LongitudinalRLCWake,
+                          LongitudinalWake, LorentzBoost, OctupoleSC, Quadrupole, QuadrupoleSC, RBendSC, RFCavity,
+                          SBendSC, SextupoleSC, Solenoid, SpaceCharge, StrongGaussianBeam, StrongThinGaussianBeam,
+                          Translation, TrackPadHIPError, VKicker, YRotation, compile_table, ringpass_grad, track_)
+
+from conftest import ENERGY_VAL, PARITY_ATOL, PARTICLES_INITIAL
+
+pytestmark = pytest.mark.gpu
+E = ENERGY_VAL
+
+# (name, element, bit-exact in the parity build)
+CASES = [
+    # test/verify_elements.jl:75-111
+    ("CrabCavity", CrabCavity(0.3, volt=2.0e6, freq=500.0e6, phi=0.1, energy=E), False),
+    ("AccelCavity", AccelCavity(0.0, volt=2.5e6, freq=500.0e6, h=1200.0, phis=0.2, energy=E), False),
+    ("LongitudinalRFMap", LongitudinalRFMap(2.0e-4, AccelCavity(0.0, volt=2.5e6, freq=500.0e6, h=1200.0, phis=0.2,
+                                                                  energy=E)), True),
+    ("Solenoid", Solenoid(0.5, 0.8), False),
+    ("Corrector", Corrector(0.4, 1.5e-4, -2.2e-4), True),
+    ("HKicker", HKicker(L=0.0, xkick=2.5e-4), True),
+    ("VKicker", VKicker(L=0.0, ykick=-1.5e-4), True),
+    ("Translation", Translation(0.0, dx=1e-3, dy=-2e-3, ds=3e-3), True),
+    ("YRotation", YRotation(0.0, angle=0.02), True),
+    ("LorentzBoost", LorentzBoost(0.03), True),
+    ("InvLorentzBoost", InvLorentzBoost(0.03), True),
+    ("DriftSC", DriftSC(0.5, a=0.01, b=0.02, Nl=12, Nm=14, Nsteps=2), True),
+    ("QuadrupoleSC", QuadrupoleSC(0.4, k1=0.0, a=0.01, b=0.02), True),
+    ("SextupoleSC", SextupoleSC(0.3, k2=2.4, a=0.01, b=0.02), True),
+    ("OctupoleSC", OctupoleSC(0.2, k3=3.6, a=0.01, b=0.02), True),
+    ("SBendSC", SBendSC(1.0, 0.0, 0.0, 0.0, a=0.01, b=0.02), True),
+    ("RBendSC", RBendSC(1.2, 0.0), True),
+    ("LBend", LBend(0.7, 0.0, K=0.0), True),
+    ("SpaceCharge", SpaceCharge(0.9, effective_len=0.4, Nl=11, Nm=9, a=0.015, b=0.017), True),
+    ("LongitudinalRLCWake", LongitudinalRLCWake(freq=1.0e9, Rshunt=0.0, Q0=1.2, scale=1.0), True),
+    ("StrongGaussianBeam", StrongGaussianBeam(0.0, M_ELECTRON, 1.0, 1_000_000, E, [1e-3, 1.2e-3], nzslice=3), True),
+    # nonzero strengths
+    ("QuadrupoleSC k1", QuadrupoleSC(0.4, k1=1.3, kick_angle=[1e-4, -2e-4]), True),
+    ("SBendSC bend", SBendSC(0.9, 0.15, 0.03, 0.02, fint1=0.5, fint2=0.4, gap=0.03), False),
+    ("LBend bend", LBend(0.7, 0.08, 0.02, 0.01, K=0.6, by_error=1e-4, fint1=0.5, fint2=0.5, full_gap=0.03), False),
+    ("LBend defocusing", LBend(0.7, 0.02, K=-1.5), False),
+    ("Solenoid misaligned", Solenoid(0.5, -1.1, t1=[1e-4, 0, -2e-4, 0, 0, 0], t2=[-1e-4, 0, 2e-4, 0, 0, 0]), False),
+    ("Corrector misaligned", Corrector(0.4, 1.5e-4, -2.2e-4, t1=[0, 1e-5, 0, 0, 0, 0]), True),
+    ("LongitudinalRLCWake on", LongitudinalRLCWake(freq=1.0e9, Rshunt=1.0e6, Q0=1.2, scale=1e-22), False),
+    ("LongitudinalWake", LongitudinalWake([0.0, 1e-12, 2e-12, 4e-12], [5.0, 3.0, -1.0, 0.5], scale=1e-6), True),
+    ("StrongThinGaussianBeam", StrongThinGaussianBeam(2e-10, 1e-3, 1.2e-3, xoffset=1e-4, yoffset=-2e-4), False),
+    ("StrongGaussianBeam on", StrongGaussianBeam(3e-3, M_ELECTRON, 1.0, 10 ** 6, E, [1e-3, 1.2e-3], nzslice=3,
+                                                 xoffsets=[0, 1e-4, -1e-4], zslice_npar=[0.2, 0.5, 0.3]), False),
+    ("LongitudinalRFMap RF", LongitudinalRFMap(3e-4, RFCavity(0.0, 1e6, 5e8, h=1200.0)), True),
+    ("YRotation negative", YRotation(0.0, angle=-0.05), True),
+    ("SBendSC zero length", SBendSC(0.0, 0.1), True),
+]
+
+
+def p10():
+    return np.array(PARTICLES_INITIAL, dtype=np.float64)
+
+
+def gpu_line(ctx, els, c, beam, nturns=1, fast=False):
+    tab = ctx.table(compile_table(Lattice(list(els)), beam))
+    p = HIPParticles.from_coords(c, None, ctx)
+    track_(p, tab, nturns=nturns, fast=fast)
+    g = p.download()
+    return np.ascontiguousarray(g[0]), g[1], g[2], g[3]
+
+
+@pytest.mark.parametrize("name,elem,bitexact", CASES, ids=[c[0] for c in CASES])
+def test_element_single_pass(ctx, name, elem, bitexact):
+    beam = Beam(E)
+    oc, olf, olt, ole = O.linepass(p10(), Lattice([elem]), beam, np.zeros(10, np.int64))
+    gc, glf, glt, gle = gpu_line(ctx, [elem], p10(), beam)
+    assert np.array_equal(glf, olf) and np.array_equal(glt, olt) and np.array_equal(gle, ole)
+    alive = olf == 0
+    if name == "SBendSC zero length":
+        assert not alive.any()  # the reference's NaN map: every particle is lost there
+        return
+    if bitexact:
+        assert np.array_equal(gc, oc), np.max(np.abs(gc - oc))
+    assert np.max(np.abs(gc - oc)) < PARITY_ATOL, np.max(np.abs(gc - oc))
+    fc, flf, _, _ = gpu_line(ctx, [elem], p10(), beam, fast=True)
+    assert np.array_equal(flf, olf) and np.max(np.abs(fc - oc)) < PARITY_ATOL
+
+
+@pytest.mark.parametrize("nturns", [1, 100])
+def test_ring_of_all_families(ctx, nturns):
+    """A ring mixing every family with FODO cells, 512 particles, loss records identical
+    with the oracle.  1 turn: the parity build within 1e-12 of the oracle, relative to the
+    coordinate's plane.  100 turns: this ring (sextupole, beam-beam lens, crab cavity)
+    amplifies the last-place differences of the per-particle transcendentals between the
+    device and the host library; the reference algorithm itself is only ~2e-8 accurate there
+    against the same algorithm in long double (measured), so the device is held to accuracy
+    parity: its error against the long-double run at most 3x the oracle's, per plane.  The
+    adjoint refuses the table."""
+    beam = Beam(3.0e9)
+    q = lambda k: Quadrupole(0.4, k, num_int_steps=8)
+    els = [q(1.2), Drift(1.0), Solenoid(0.4, 0.3), Drift(0.5), q(-1.2), Drift(1.0),
+           Corrector(0.2, 1e-5, -2e-5), LBend(0.8, 0.05, K=0.1, fint1=0.3, fint2=0.3, full_gap=0.02), Drift(0.5),
+           q(1.2), Drift(1.0), QuadrupoleSC(0.4, k1=-1.2), DriftSC(1.0), SextupoleSC(0.2, k2=3.0),
+           SBendSC(0.8, 0.05, 0.025, 0.025), Drift(0.5), LorentzBoost(0.01), StrongThinGaussianBeam(1e-7, 1e-3, 1e-3),
+           InvLorentzBoost(0.01), Translation(0.0, ds=1e-3), Translation(0.0, ds=-1e-3), YRotation(0.0, angle=1e-3),
+           YRotation(0.0, angle=-1e-3), CrabCavity(0.0, volt=1e5, freq=5e8, phi=0.0, energy=3e9),
+           AccelCavity(0.0, volt=1e6, freq=5e8, h=400.0, phis=0.0, energy=3e9),
+           LongitudinalRFMap(1e-3, AccelCavity(0.0, volt=1e6, freq=5e8, h=400.0)),
+           LongitudinalRLCWake(freq=1e9, Rshunt=1e4, Q0=2.0, scale=1e-20), SpaceCharge(0.0)]
+    lat = Lattice(els)
+    rng = np.random.default_rng(61)
+    c0 = rng.normal(0, 1, (512, 6)) * np.array([2e-4, 2e-5, 2e-4, 2e-5, 1e-3, 5e-4])
+    c0[:4, 1] = 0.9  # lost in the first drifts
+    oc, olf, olt, ole = O.ringpass(c0, lat, beam, np.zeros(512, np.int64), nturns, nthreads=8)
+    tab = ctx.table(compile_table(lat, beam))
+    p = HIPParticles.from_coords(c0, None, ctx)
+    track_(p, tab, nturns=nturns)
+    gc, glf, glt, gle = p.download()
+    assert np.array_equal(glf, olf) and np.array_equal(glt, olt) and np.array_equal(gle, ole)
+    assert olf.sum() >= 4
+    alive = olf == 0
+    scale = np.maximum(np.abs(oc), np.sqrt(np.mean(oc[alive] ** 2, axis=0)))
+    if nturns == 1:
+        assert np.max((np.abs(gc - oc) / scale)[alive]) <= 1e-12
+    else:
+        ld = O.ringpass(c0, lat, beam, np.zeros(512, np.int64), nturns, nthreads=8, extended=True)[0]
+        rms = np.sqrt(np.mean(ld[alive] ** 2, axis=0))
+        eg = np.max(np.abs(gc - ld)[alive], axis=0) / rms
+        eo = np.max(np.abs(oc - ld)[alive], axis=0) / rms
+        assert np.all(eg <= 3.0 * eo + 1e-14), (eg, eo)
+    with pytest.raises(TrackPadHIPError, match="TP_EUNSUPPORTED"):
+        ringpass_grad(c0[8:16], lat, beam, 2, ctx=ctx)

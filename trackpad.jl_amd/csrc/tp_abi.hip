@@ -33,7 +33,7 @@ int fail(int code, const char* fmt, ...) {
 
 
 // Length in doubles of a record's parameter block (trackpad_table.h).
-int64_t block_len(int tag, int m) {
+int64_t block_len(int tag, int m, int nsteps) {
   const int64_t poly = 2 * (int64_t)(m + 1);
   switch (tag) {
     case TP_MARKER: return 0;
@@ -44,6 +44,20 @@ int64_t block_len(int tag, int m) {
     case TP_ESBEND:
     case TP_ESBEND_LE: return 24 + poly;
     case TP_RFCAV: return 7;
+    case TP_SOLENOID: return 2;
+    case TP_CORRECTOR: return 6;
+    case TP_TRANSLATION: return 3;
+    case TP_YROTATION: return 3;
+    case TP_CRAB: return 6;
+    case TP_ACCEL: return 4;
+    case TP_LONGRF: return 1;
+    case TP_BOOST:
+    case TP_INVBOOST: return 2;
+    case TP_GAUSSBB: return 3 + 3 * (int64_t)nsteps;
+    case TP_RLCWAKE: return 6;
+    case TP_TABWAKE: return 3 + 2 * (int64_t)nsteps;
+    case TP_LBEND: return 13;
+    case TP_NAN: return 0;
     default: return -1;
   }
 }
@@ -234,11 +248,13 @@ int parse_table(const void* bytes, size_t nbytes, tp_table* t) {
     if (r.max_order < 0 || r.max_order > TP_MAXORD)
       return fail(TP_EUNSUPPORTED, "table: record %u: max_order %d outside [0, 10]", i, r.max_order);
     if (r.nsteps < 0 || r.nsteps > 1000000) return fail(TP_ETABLE, "table: record %u: nsteps %d", i, r.nsteps);
-    const int64_t len = block_len(r.tag, r.max_order);
+    if (r.tag == TP_GAUSSBB && r.nsteps < 1) return fail(TP_ETABLE, "table: record %u: beam-beam without slices", i);
+    if (r.tag == TP_TABWAKE && r.nsteps < 2) return fail(TP_ETABLE, "table: record %u: wake table shorter than 2", i);
+    const int64_t len = block_len(r.tag, r.max_order, r.nsteps);
     if (r.off < 0 || (int64_t)r.off + len > (int64_t)h.pool_len)
       return fail(TP_ETABLE, "table: record %u: parameter block outside the pool", i);
     if (r.flags & TP_F_MISALIGN) {
-      if (r.tag == TP_MARKER || r.tag == TP_RFCAV)
+      if (r.tag == TP_MARKER || r.tag == TP_RFCAV || (r.tag >= TP_TRANSLATION && r.tag != TP_LBEND))
         return fail(TP_ETABLE, "table: record %u: misalignment on an element without one", i);
       if (r.moff < 0 || (int64_t)r.moff + 84 > (int64_t)h.pool_len)
         return fail(TP_ETABLE, "table: record %u: misalignment block outside the pool", i);
@@ -255,6 +271,7 @@ int parse_table(const void* bytes, size_t nbytes, tp_table* t) {
     if (r.tag == TP_ESBEND) feat |= FT_ESBEND;
     if (r.tag == TP_ESBEND_LE) feat |= FT_ESBEND_LE;
     if (r.tag == TP_RFCAV) feat |= FT_RF;
+    if (r.tag >= TP_SOLENOID) feat |= FT_EXTRA;
   }
   t->features = feat;
   if ((feat & ~TP_FEAT_A) == 0) t->variant = TP_VARIANT_A;
@@ -563,8 +580,9 @@ int adj_check(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t nturns, i
   if (nturns < 1) return fail(TP_EINVAL, "%s: nturns = %d", who, nturns);
   if (ckpt_stride < 0) return fail(TP_EINVAL, "%s: ckpt_stride = %d", who, ckpt_stride);
   if (flags & ~(TP_TRACK_FAST | TP_TRACK_TIMED)) return fail(TP_EINVAL, "%s: unsupported flags 0x%x", who, flags);
-  if (t->features & (FT_ESBEND | FT_LINH))
-    return fail(TP_EUNSUPPORTED, "%s: Forest exact bends and the linearised drift have no adjoint here", who);
+  if (t->features & (FT_ESBEND | FT_LINH | FT_EXTRA))
+    return fail(TP_EUNSUPPORTED, "%s: Forest exact bends, the linearised drift and the element families beyond "
+                "drifts, multipoles, bends and RF cavities have no adjoint here", who);
   for (const tp_rec& r : t->recs)
     if (r.flags & (TP_F_R1 | TP_F_R2)) return fail(TP_EUNSUPPORTED, "%s: r1/r2 rotations have no adjoint here", who);
   if (t->hdr.n_rows > 1) return fail(TP_EUNSUPPORTED, "%s: time-varying tables have no adjoint here", who);
@@ -1072,8 +1090,9 @@ int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_part
     b->max_slots = std::max(b->max_slots, ns);
     if (b->no_adjoint.empty()) {
       if (t->hdr.n_rows > 1) b->no_adjoint = "time-varying tables have no adjoint here";
-      if (t->features & (FT_ESBEND | FT_LINH))
-        b->no_adjoint = "Forest exact bends and the linearised drift have no adjoint here";
+      if (t->features & (FT_ESBEND | FT_LINH | FT_EXTRA))
+        b->no_adjoint = "Forest exact bends, the linearised drift and the element families beyond drifts, "
+                        "multipoles, bends and RF cavities have no adjoint here";
       for (const tp_rec& r : t->recs)
         if (r.flags & (TP_F_R1 | TP_F_R2)) b->no_adjoint = "r1/r2 rotations have no adjoint here";
     }

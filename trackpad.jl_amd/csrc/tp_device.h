@@ -27,13 +27,14 @@ enum : unsigned {
   FT_RF = 8u,         // RF cavities
   FT_MISALIGN = 16u,  // t1/t2/r1/r2 misalignments
   FT_GENKICK = 32u,   // general Horner kicks up to order 10
-  FT_LINH = 64u       // linearised drift (USE_EXACT_HAMILTONIAN = false)
+  FT_LINH = 64u,      // linearised drift (USE_EXACT_HAMILTONIAN = false)
+  FT_EXTRA = 128u     // the other element families (tags TP_SOLENOID .. TP_NAN, tp_extra.h)
 };
 // Compiled variants, smallest first; the host picks the first whose mask covers the table.
 #define TP_FEAT_A 0u                                              // drift/multipole lattices
 #define TP_FEAT_B (FT_ESBEND_LE | FT_RF)                          // rings of exact bends + RF
 #define TP_FEAT_C (FT_ESBEND_LE | FT_RF | FT_SBEND | FT_GENKICK)  // + high-order multipoles
-#define TP_FEAT_ALL 127u
+#define TP_FEAT_ALL 255u
 enum { TP_VARIANT_A = 0, TP_VARIANT_B = 1, TP_VARIANT_C = 2, TP_VARIANT_ALL = 3 };
 
 struct TpKArgs {

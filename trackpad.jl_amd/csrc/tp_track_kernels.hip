@@ -14,7 +14,7 @@
 // the hot loop has no divergent control flow.
 //
 // Compiled twice: TP_FAST=0 (-ffp-contract=off, parity build) and TP_FAST=1.
-#include "tp_maps.h"
+#include "tp_extra.h"
 
 #if TP_FAST
 #define TP_LAUNCH tp_launch_track_fast
@@ -310,7 +310,16 @@ TP_DEV void apply_elem(St* s, Cx* c, const tp_rec& rec, const double* __restrict
         }
       }
       break;
-    default:  // TP_MARKER
+    case TP_MARKER:
+      break;
+    default:  // the other element families, out of line (tp_extra.h)
+      if constexpr ((FEAT & FT_EXTRA) != 0) {
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) {
+          s[k] = extra_map<EXH>(s[k], rec.tag, fl, ns, p, beti);
+          mkctx(s[k], c[k], beti);
+        }
+      }
       break;
   }
   if constexpr ((FEAT & FT_MISALIGN) != 0) {

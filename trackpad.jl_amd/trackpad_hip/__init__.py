@@ -10,6 +10,9 @@ from .elements import (  # noqa: F401
     MAXORD, C_LIGHT, AbstractCavity, AbstractDrift, AbstractElement, AbstractMagnet, Drift, ERBend,
     ExactSBend, ExactSBendLinearEdge, Marker, Octupole, Quadrupole, RBend, RFCavity, SBend, Sextupole,
     ThinMultipole, TimeVaryingElement, get_length, timed,
+    AccelCavity, Corrector, CrabCavity, DriftSC, HKicker, InvLorentzBoost, LBend, LongitudinalRFMap,
+    LongitudinalRLCWake, LongitudinalWake, LorentzBoost, OctupoleSC, QuadrupoleSC, RBendSC, SBendSC, SextupoleSC,
+    Solenoid, SpaceCharge, StrongGaussianBeam, StrongThinGaussianBeam, Translation, VKicker, YRotation,
 )
 from .lattice import (  # noqa: F401
     M_ELECTRON, M_PROTON, Beam, Lattice, beti, findelem, materialize_lattice, spos, total_length,

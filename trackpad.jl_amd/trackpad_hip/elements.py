@@ -13,6 +13,14 @@ constructors so that a lattice written for TrackPad reads the same here:
     ExactSBend     src/elements/bend.jl:88-149
     ERBend         src/elements/SCelements.jl:339
     RFCavity       src/elements/rf.jl:6-21
+    CrabCavity, AccelCavity, LongitudinalRFMap      src/elements/rf.jl:35-109
+    Solenoid       src/elements/solenoid.jl:6-30
+    Corrector, HKicker, VKicker                     src/elements/corrector.jl:6-51
+    DriftSC, QuadrupoleSC, SextupoleSC, OctupoleSC, SBendSC, RBendSC, LBend,
+    SpaceCharge, Translation, YRotation             src/elements/SCelements.jl:6-460
+    LorentzBoost, InvLorentzBoost, StrongThinGaussianBeam, StrongGaussianBeam
+                                                    src/elements/beambeam.jl:6-125
+    LongitudinalRLCWake, LongitudinalWake           src/elements/wake.jl:6-90
 
 Two deliberate extensions, both absent from the reference:
 
@@ -37,6 +45,10 @@ __all__ = [
     "Drift", "Marker", "Quadrupole", "Sextupole", "Octupole", "ThinMultipole",
     "SBend", "RBend", "ExactSBend", "ERBend", "ExactSBendLinearEdge", "RFCavity",
     "TimeVaryingElement", "timed", "C_LIGHT", "get_length",
+    "CrabCavity", "AccelCavity", "LongitudinalRFMap", "Solenoid", "Corrector", "HKicker", "VKicker",
+    "DriftSC", "QuadrupoleSC", "SextupoleSC", "OctupoleSC", "SBendSC", "RBendSC", "LBend", "SpaceCharge",
+    "Translation", "YRotation", "LorentzBoost", "InvLorentzBoost", "StrongThinGaussianBeam",
+    "StrongGaussianBeam", "LongitudinalRLCWake", "LongitudinalWake",
 ]
 
 # src/elements/rf.jl:349
@@ -305,6 +317,292 @@ class RFCavity(AbstractCavity):
         self.lag = float(lag)
         self.philag = float(philag)
         self.energy = float(energy)
+
+
+class CrabCavity(AbstractCavity):
+    """CrabCavity(L; volt=0, freq=0, phi=0, errors=(0, 0), energy=1e9) -- src/elements/rf.jl:35-54.
+    ``k = 2 pi freq / c`` is stored as in the reference."""
+
+    def __init__(self, L, *, name="CRABCAVITY", volt=0.0, freq=0.0, phi=0.0, errors=None, energy=1.0e9):
+        self.name = str(name)
+        self.L = float(L)
+        self.volt = float(volt)
+        self.freq = float(freq)
+        self.k = 6.283185307179586 * self.freq / C_LIGHT
+        self.phi = float(phi)
+        self.errors = _vec(errors, 2, "errors")
+        self.energy = float(energy)
+
+
+class AccelCavity(AbstractCavity):
+    """AccelCavity(L; volt=0, freq=0, h=1, phis=0, energy=1e9) -- src/elements/rf.jl:69-87."""
+
+    def __init__(self, L, *, name="ACCELCAVITY", volt=0.0, freq=0.0, h=1.0, phis=0.0, energy=1.0e9):
+        self.name = str(name)
+        self.L = float(L)
+        self.volt = float(volt)
+        self.freq = float(freq)
+        self.k = 6.283185307179586 * self.freq / C_LIGHT
+        self.h = float(h)
+        self.phis = float(phis)
+        self.energy = float(energy)
+
+
+class LongitudinalRFMap(AbstractElement):
+    """LongitudinalRFMap(alphac, rf) -- src/elements/rf.jl:102-105; a zero-length slip map
+    z -= 2 pi h eta / k * delta tied to an RF element (src/tracking.jl:1562-1585)."""
+
+    def __init__(self, alphac, rf: AbstractElement, *, name="LONGITUDINALRFMAP"):
+        self.name = str(name)
+        self.L = 0.0
+        self.alphac = float(alphac)
+        self.rf = rf
+
+
+class Solenoid(AbstractMagnet, _Misaligned):
+    """Solenoid(L, ks; t1, t2, r1, r2) -- src/elements/solenoid.jl:6-26."""
+
+    def __init__(self, L, ks, *, name="SOLENOID", t1=None, t2=None, r1=None, r2=None):
+        self.name = str(name)
+        self.L = float(L)
+        self.ks = float(ks)
+        self._init_misalign(t1, t2, r1, r2, None, None)
+
+
+class Corrector(AbstractMagnet, _Misaligned):
+    """Corrector(L, hk, vk; t1, t2, r1, r2) -- src/elements/corrector.jl:6-27."""
+
+    def __init__(self, L, hk, vk, *, name="CORRECTOR", t1=None, t2=None, r1=None, r2=None):
+        self.name = str(name)
+        self.L = float(L)
+        self.xkick = float(hk)
+        self.ykick = float(vk)
+        self._init_misalign(t1, t2, r1, r2, None, None)
+
+
+def HKicker(*, name="HKICKER", L=0.0, xkick=0.0) -> Corrector:
+    """src/elements/corrector.jl:39-41"""
+    return Corrector(L, xkick, 0.0, name=name)
+
+
+def VKicker(*, name="VKICKER", L=0.0, ykick=0.0) -> Corrector:
+    """src/elements/corrector.jl:48-50"""
+    return Corrector(L, 0.0, ykick, name=name)
+
+
+class _SCMeta:
+    def _init_sc(self, a, b, Nl, Nm, Nsteps):
+        # space-charge metadata: carried, not used by the single-particle maps
+        self.a, self.b = float(a), float(b)
+        self.Nl, self.Nm, self.Nsteps = int(Nl), int(Nm), int(Nsteps)
+
+
+class DriftSC(AbstractDrift, _Misaligned, _SCMeta):
+    """DriftSC(L; a, b, Nl, Nm, Nsteps, t1, t2, r1, r2) -- src/elements/SCelements.jl:6-38;
+    tracked as a drift (src/tracking.jl:981-991)."""
+
+    def __init__(self, L, *, name="DRIFT_SC", t1=None, t2=None, r1=None, r2=None, r_apertures=None,
+                 e_apertures=None, a=1.0, b=1.0, Nl=10, Nm=10, Nsteps=1):
+        self.name = str(name)
+        self.L = float(L)
+        self._init_misalign(t1, t2, r1, r2, r_apertures, e_apertures)
+        self._init_sc(a, b, Nl, Nm, Nsteps)
+
+
+class _MultipoleSC(AbstractMagnet, _Misaligned, _SCMeta):
+    _default_order = 1
+
+    def __init__(self, L, *, name=None, k0=0.0, k1=0.0, k2=0.0, k3=0.0, polynom_a=None, max_order=None,
+                 num_int_steps=10, rad=0, fringe_entrance=0, fringe_exit=0, t1=None, t2=None, r1=None, r2=None,
+                 r_apertures=None, e_apertures=None, kick_angle=None, a=1.0, b=1.0, Nl=10, Nm=10, Nsteps=1):
+        self.name = str(name or self._default_name)
+        self.L = float(L)
+        self.k0, self.k1, self.k2, self.k3 = float(k0), float(k1), float(k2), float(k3)
+        self.polynom_a = _vec(polynom_a, 4, "polynom_a")
+        self.max_order = int(self._default_order if max_order is None else max_order)
+        self.num_int_steps = int(num_int_steps)
+        self.rad = int(rad)
+        self.fringe_entrance, self.fringe_exit = int(fringe_entrance), int(fringe_exit)
+        self.kick_angle = _vec(kick_angle, 2, "kick_angle")
+        self._init_misalign(t1, t2, r1, r2, r_apertures, e_apertures)
+        self._init_sc(a, b, Nl, Nm, Nsteps)
+
+
+class QuadrupoleSC(_MultipoleSC):
+    """QuadrupoleSC(L; k0, k1, k2, k3, polynom_a, max_order=1, ...) -- src/elements/SCelements.jl:51-101."""
+    _default_order = 1
+    _default_name = "KQUAD_SC"
+
+
+class SextupoleSC(_MultipoleSC):
+    """SextupoleSC(L; ..., max_order=2) -- src/elements/SCelements.jl:116-166."""
+    _default_order = 2
+    _default_name = "KSEXT_SC"
+
+
+class OctupoleSC(_MultipoleSC):
+    """OctupoleSC(L; ..., max_order=3) -- src/elements/SCelements.jl:181-231."""
+    _default_order = 3
+    _default_name = "KOCT_SC"
+
+
+class SBendSC(AbstractMagnet, _Misaligned, _SCMeta):
+    """SBendSC(L, angle, e1=0, e2=0; ...) -- src/elements/SCelements.jl:246-311; tracked as
+    SBend's map with fringe methods gated by the fringe flags (src/tracking.jl:1041-1074)."""
+
+    def __init__(self, L, angle, e1=0.0, e2=0.0, *, name="SBEND_SC", polynom_a=None, polynom_b=None,
+                 max_order=0, num_int_steps=10, rad=0, fint1=0.0, fint2=0.0, gap=0.0,
+                 fringe_bend_entrance=1, fringe_bend_exit=1, fringe_quad_entrance=0, fringe_quad_exit=0,
+                 fringe_int_m0=None, fringe_int_p0=None, t1=None, t2=None, r1=None, r2=None,
+                 r_apertures=None, e_apertures=None, kick_angle=None, a=1.0, b=1.0, Nl=10, Nm=10, Nsteps=1):
+        self.name = str(name)
+        self.L, self.angle, self.e1, self.e2 = float(L), float(angle), float(e1), float(e2)
+        self.polynom_a = _vec(polynom_a, 4, "polynom_a")
+        self.polynom_b = _vec(polynom_b, 4, "polynom_b")
+        self.max_order, self.num_int_steps, self.rad = int(max_order), int(num_int_steps), int(rad)
+        self.fint1, self.fint2, self.gap = float(fint1), float(fint2), float(gap)
+        self.fringe_bend_entrance, self.fringe_bend_exit = int(fringe_bend_entrance), int(fringe_bend_exit)
+        self.fringe_quad_entrance, self.fringe_quad_exit = int(fringe_quad_entrance), int(fringe_quad_exit)
+        self.fringe_int_m0 = _vec(fringe_int_m0, 5, "fringe_int_m0")
+        self.fringe_int_p0 = _vec(fringe_int_p0, 5, "fringe_int_p0")
+        self.kick_angle = _vec(kick_angle, 2, "kick_angle")
+        self._init_misalign(t1, t2, r1, r2, r_apertures, e_apertures)
+        self._init_sc(a, b, Nl, Nm, Nsteps)
+
+
+def RBendSC(L, angle, **kw) -> SBendSC:
+    """src/elements/SCelements.jl:332"""
+    return SBendSC(L, angle, angle / 2, angle / 2, **kw)
+
+
+class LBend(AbstractMagnet, _Misaligned):
+    """LBend(L, angle, e1=0, e2=0; K, by_error, fint1, fint2, full_gap, ...) --
+    src/elements/SCelements.jl:346-379; linear-matrix bend (src/tracking.jl:1076-1170)."""
+
+    def __init__(self, L, angle, e1=0.0, e2=0.0, *, name="LBEND", K=0.0, by_error=0.0, fint1=0.0, fint2=0.0,
+                 full_gap=0.0, t1=None, t2=None, r1=None, r2=None, r_apertures=None, e_apertures=None):
+        self.name = str(name)
+        self.L, self.angle, self.e1, self.e2 = float(L), float(angle), float(e1), float(e2)
+        self.K, self.by_error = float(K), float(by_error)
+        self.fint1, self.fint2, self.full_gap = float(fint1), float(fint2), float(full_gap)
+        self._init_misalign(t1, t2, r1, r2, r_apertures, e_apertures)
+
+
+class SpaceCharge(AbstractElement):
+    """SpaceCharge(L; effective_len, Nl, Nm, a, b) -- src/elements/SCelements.jl:394-408; the
+    single-particle map is the identity (src/tracking.jl:1176-1180)."""
+
+    def __init__(self, L, *, name="SPACECHARGE", effective_len=0.0, Nl=10, Nm=10, a=1.0, b=1.0):
+        self.name = str(name)
+        self.L, self.effective_len = float(L), float(effective_len)
+        self.Nl, self.Nm, self.a, self.b = int(Nl), int(Nm), float(a), float(b)
+
+
+class Translation(AbstractElement):
+    """Translation(L; dx=0, dy=0, ds=0) -- src/elements/SCelements.jl:420-433."""
+
+    def __init__(self, L, *, name="TRANSLATION", dx=0.0, dy=0.0, ds=0.0):
+        self.name = str(name)
+        self.L, self.dx, self.dy, self.ds = float(L), float(dx), float(dy), float(ds)
+
+
+class YRotation(AbstractElement):
+    """YRotation(L; angle=0) -- src/elements/SCelements.jl:444-455."""
+
+    def __init__(self, L, *, name="YROTATION", angle=0.0):
+        self.name = str(name)
+        self.L, self.angle = float(L), float(angle)
+
+
+class LorentzBoost(AbstractElement):
+    """LorentzBoost(angle; mode=0) -- src/elements/beambeam.jl:6-19 (cos and tan stored)."""
+
+    def __init__(self, angle, *, name="LORENTZBOOST", mode=0):
+        import math
+
+        self.name = str(name)
+        self.L = 0.0
+        self.angle = float(angle)
+        self.cosang, self.tanang = math.cos(self.angle), math.tan(self.angle)
+        self.mode = int(mode)
+
+
+class InvLorentzBoost(AbstractElement):
+    """InvLorentzBoost(angle; mode=0) -- src/elements/beambeam.jl:30-43 (sin and cos stored)."""
+
+    def __init__(self, angle, *, name="INVLORENTZBOOST", mode=0):
+        import math
+
+        self.name = str(name)
+        self.L = 0.0
+        self.angle = float(angle)
+        self.sinang, self.cosang = math.sin(self.angle), math.cos(self.angle)
+        self.mode = int(mode)
+
+
+class StrongThinGaussianBeam(AbstractElement):
+    """StrongThinGaussianBeam(amplitude, rmssizex, rmssizey; zloc, xoffset, yoffset) --
+    src/elements/beambeam.jl:54-69."""
+
+    def __init__(self, amplitude, rmssizex, rmssizey, *, name="STRONGTHINGAUSSIANBEAM", zloc=0.0, xoffset=0.0,
+                 yoffset=0.0):
+        self.name = str(name)
+        self.L = 0.0
+        self.amplitude, self.rmssizex, self.rmssizey = float(amplitude), float(rmssizex), float(rmssizey)
+        self.zloc, self.xoffset, self.yoffset = float(zloc), float(xoffset), float(yoffset)
+
+
+class StrongGaussianBeam(AbstractElement):
+    """StrongGaussianBeam(charge, mass, atomnum, num_particle, total_energy, beamsize; nzslice=1,
+    zslice_center, zslice_npar, xoffsets, yoffsets) -- src/elements/beambeam.jl:83-119."""
+
+    def __init__(self, charge, mass, atomnum, num_particle, total_energy, beamsize, *, name="STRONGGAUSSIANBEAM",
+                 nzslice=1, zslice_center=None, zslice_npar=None, xoffsets=None, yoffsets=None):
+        import math
+
+        self.name = str(name)
+        self.L = 0.0
+        self.charge, self.mass, self.atomnum = float(charge), float(mass), float(atomnum)
+        self.num_particle = int(num_particle)
+        self.total_energy = float(total_energy)
+        mom2 = max(self.total_energy ** 2 - self.mass ** 2, 0.0)
+        self.momentum = math.sqrt(mom2)
+        self.gamma = self.total_energy / self.mass
+        self.beta = 0.0 if self.total_energy == 0 else self.momentum / self.total_energy
+        self.beamsize = np.array([float(beamsize[0]), float(beamsize[1])])
+        self.nzslice = int(nzslice)
+        n = self.nzslice
+        self.zslice_center = (np.linspace(-1.0, 1.0, n) if zslice_center is None
+                              else np.array(zslice_center, dtype=np.float64))
+        self.zslice_npar = (np.full(n, 1.0 / max(n, 1)) if zslice_npar is None
+                            else np.array(zslice_npar, dtype=np.float64))
+        self.xoffsets = np.zeros(n) if xoffsets is None else np.array(xoffsets, dtype=np.float64)
+        self.yoffsets = np.zeros(n) if yoffsets is None else np.array(yoffsets, dtype=np.float64)
+
+
+class LongitudinalRLCWake(AbstractElement):
+    """LongitudinalRLCWake(; freq=1e9, Rshunt=1e6, Q0=1, scale=0) -- src/elements/wake.jl:6-18."""
+
+    def __init__(self, *, name="LONGITUDINALRLCWAKE", freq=1.0e9, Rshunt=1.0e6, Q0=1.0, scale=0.0):
+        self.name = str(name)
+        self.L = 0.0
+        self.freq, self.Rshunt, self.Q0, self.scale = float(freq), float(Rshunt), float(Q0), float(scale)
+
+
+class LongitudinalWake(AbstractElement):
+    """LongitudinalWake(times, wakefields; fliphalf=-1, scale=0) -- src/elements/wake.jl:44-59."""
+
+    def __init__(self, times, wakefields, *, name="LONGITUDINALWAKE", fliphalf=-1.0, scale=0.0):
+        t = np.array(times, dtype=np.float64).reshape(-1)
+        w = np.array(wakefields, dtype=np.float64).reshape(-1)
+        if t.shape != w.shape:
+            raise ValueError("times and wakefields must have the same length")
+        if t.shape[0] < 2:
+            raise ValueError("LongitudinalWake requires at least two samples")
+        self.name = str(name)
+        self.L = 0.0
+        self.times, self.wakefields = t, w
+        self.fliphalf, self.scale = float(fliphalf), float(scale)
 
 
 class TimeVaryingElement(AbstractElement):

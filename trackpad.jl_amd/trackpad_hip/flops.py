@@ -15,6 +15,14 @@ they follow:
     yrot                        23            :727-745
     bend_edge                   40            :822-851
     bend_fringe                 128           :752-815
+    Solenoid                    83            :542-560
+    Corrector                   25            :586-596
+    Translation / YRotation     17 / 26       :1182-1218
+    Crab / Accel cavity         6 / 4         :1531-1559
+    LongitudinalRFMap, boosts   2, 7          :1575-1604
+    Gaussian beam-beam          15 per slice  :1610-1641
+    RLC / tabulated wake        12 / 6        :1647-1663
+    LBend                       81 + edges    :1076-1170
 
 so one Yoshida-4 step of a thick multipole is 4 x 19 + 3 x (8m + 4).  With these counts
 the 20-cell DBA ring of the benchmark is 313,587 flops per particle-turn and the FODO
@@ -23,8 +31,11 @@ workload 7,472.
 
 from __future__ import annotations
 
-from .elements import (Drift, ExactSBend, ExactSBendLinearEdge, Marker, Octupole, Quadrupole, RFCavity, SBend,
-                       Sextupole, ThinMultipole)
+from .elements import (AccelCavity, Corrector, CrabCavity, Drift, DriftSC, ExactSBend, ExactSBendLinearEdge,
+                       InvLorentzBoost, LBend, LongitudinalRFMap, LongitudinalRLCWake, LongitudinalWake, LorentzBoost,
+                       Marker, Octupole, OctupoleSC, Quadrupole, QuadrupoleSC, RFCavity, SBend, SBendSC, Sextupole,
+                       SextupoleSC, Solenoid, SpaceCharge, StrongGaussianBeam, StrongThinGaussianBeam, ThinMultipole,
+                       Translation, YRotation)
 from .lattice import Lattice
 
 __all__ = ["elem_flops", "flops_per_turn"]
@@ -41,12 +52,43 @@ def _bkick(m: int) -> int:
 
 
 def elem_flops(e) -> int:
-    if isinstance(e, Marker):
+    if isinstance(e, (Marker, SpaceCharge)):
         return 0
-    if isinstance(e, Drift):
+    if isinstance(e, (Drift, DriftSC)):
         return DRIFT
     if isinstance(e, (Quadrupole, Sextupole, Octupole)):
         return e.num_int_steps * (4 * DRIFT + 3 * _kick(e.max_order))
+    if isinstance(e, (QuadrupoleSC, SextupoleSC, OctupoleSC)):
+        return e.num_int_steps * (4 * DRIFT + 3 * _kick(min(max(e.max_order, 0), 3)))
+    if isinstance(e, SBendSC):
+        edges = EDGE * ((e.fringe_bend_entrance != 0) + (e.fringe_bend_exit != 0))
+        return edges + e.num_int_steps * (4 * DRIFT + 3 * _bkick(min(max(e.max_order, 0), 3)))
+    if isinstance(e, Solenoid):
+        return DRIFT if e.ks == 0 else 83
+    if isinstance(e, Corrector):
+        return 25
+    if isinstance(e, Translation):
+        return 17
+    if isinstance(e, YRotation):
+        return 26 if e.angle != 0 else 0
+    if isinstance(e, CrabCavity):
+        return 6 + (2 * DRIFT if e.L > 0 else 0)
+    if isinstance(e, AccelCavity):
+        return 4
+    if isinstance(e, LongitudinalRFMap):
+        return 2
+    if isinstance(e, (LorentzBoost, InvLorentzBoost)):
+        return 7 if e.mode == 0 else 0
+    if isinstance(e, StrongThinGaussianBeam):
+        return 15
+    if isinstance(e, StrongGaussianBeam):
+        return 15 * max(e.nzslice, 0)
+    if isinstance(e, LongitudinalRLCWake):
+        return 12 if e.scale != 0 else 0
+    if isinstance(e, LongitudinalWake):
+        return 6 if e.scale != 0 else 0
+    if isinstance(e, LBend):
+        return 81 + 2 * EDGE
     if isinstance(e, ThinMultipole):
         return _kick(e.max_order)
     if isinstance(e, SBend):

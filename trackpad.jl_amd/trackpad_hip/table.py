@@ -17,8 +17,11 @@ from typing import Dict, List, Optional, Tuple
 import numpy as np
 
 from .elements import (
-    MAXORD, AbstractElement, Drift, ExactSBend, ExactSBendLinearEdge, Marker, Octupole,
-    Quadrupole, RFCavity, SBend, Sextupole, ThinMultipole, TimeVaryingElement,
+    MAXORD, AbstractElement, AccelCavity, C_LIGHT, Corrector, CrabCavity, Drift, DriftSC, ExactSBend,
+    ExactSBendLinearEdge, InvLorentzBoost, LBend, LongitudinalRFMap, LongitudinalRLCWake, LongitudinalWake,
+    LorentzBoost, Marker, Octupole, OctupoleSC, Quadrupole, QuadrupoleSC, RFCavity, SBend, SBendSC, Sextupole,
+    SextupoleSC, Solenoid, SpaceCharge, StrongGaussianBeam, StrongThinGaussianBeam, ThinMultipole,
+    TimeVaryingElement, Translation, YRotation,
 )
 from .lattice import Beam, Lattice, beti
 
@@ -31,7 +34,10 @@ TP_TABLE_MAGIC = 0x54504854
 TP_TABLE_VERSION = 1
 
 TAGS = {"MARKER": 0, "DRIFT": 1, "MPOLE": 2, "THINMP": 3, "SBEND": 4, "ESBEND": 5,
-        "ESBEND_LE": 6, "RFCAV": 7}
+        "ESBEND_LE": 6, "RFCAV": 7, "SOLENOID": 8, "CORRECTOR": 9, "TRANSLATION": 10, "YROTATION": 11,
+        "CRAB": 12, "ACCEL": 13, "LONGRF": 14, "BOOST": 15, "INVBOOST": 16, "GAUSSBB": 17, "RLCWAKE": 18,
+        "TABWAKE": 19, "LBEND": 20, "NAN": 21}
+EPS = 2.220446049250313e-16  # eps(Float64)
 
 F_T1, F_T2, F_R1, F_R2 = 0x1, 0x2, 0x4, 0x8
 F_FIN_SHIFT, F_FOUT_SHIFT = 4, 8
@@ -142,9 +148,9 @@ def _record(e: AbstractElement, bt: float):
             f"element {e.name!r} is time-varying: compile_table resolves it per turn when given "
             "the turn range (turn=, nturns=, time=, dt_turn=), or call materialize_lattice first "
             "(src/time_dependence.jl:137-139)")
-    if isinstance(e, Marker):
+    if isinstance(e, (Marker, SpaceCharge)):  # SpaceCharge: identity (src/tracking.jl:1176-1180)
         return TAGS["MARKER"], 0, 0, 0, -1, 0.0, [], None
-    if isinstance(e, Drift):
+    if isinstance(e, (Drift, DriftSC)):
         mflags, mblk = _misalign(e)
         return TAGS["DRIFT"], 0, 0, mflags, -1, 0.0, [e.L, e.L * bt], mblk
     if isinstance(e, (Quadrupole, Sextupole, Octupole)):
@@ -225,6 +231,9 @@ def _record(e: AbstractElement, bt: float):
         assert len(body) + len(edges) == 24
         mflags, mblk = _misalign(e)
         return tag, n, m, flags | mflags, -1, 0.0, blk, mblk
+    extra = _record_extra(e, bt)
+    if extra is not None:
+        return extra
     if isinstance(e, RFCavity):
         beta = 1.0 / bt                  # src/tracking.jl:504
         flags = (F_RF_DRIFT if e.L > 0.0 else 0) | (F_RF_KICK if e.energy > 0.0 else 0)
@@ -233,6 +242,151 @@ def _record(e: AbstractElement, bt: float):
         return TAGS["RFCAV"], 0, 0, flags, -1, 0.0, blk, None
     raise NotImplementedError(
         f"element type {type(e).__name__} ({getattr(e, 'name', '?')!r}) has no device map")
+
+
+def _sc_poly(e) -> Tuple[np.ndarray, np.ndarray, int]:
+    """polynom (k0, k1, k2/2, k3/6) of the SC multipoles (src/tracking.jl:993-995) and
+    max_order clamped to [0, 3] (:1008)."""
+    a = np.zeros(MAXORD + 1)
+    b = np.zeros(MAXORD + 1)
+    a[:4] = e.polynom_a
+    b[:4] = [e.k0, e.k1, e.k2 / 2, e.k3 / 6]
+    return a, b, min(max(int(e.max_order), 0), 3)
+
+
+def _rf_k(rf) -> float:
+    """src/tracking.jl:1567-1570"""
+    if isinstance(rf, RFCavity):
+        return TWO_PI * rf.freq / C_LIGHT
+    if isinstance(rf, (AccelCavity, CrabCavity)):
+        return rf.k
+    return 0.0
+
+
+def _rf_h(rf) -> float:
+    """src/tracking.jl:1562-1565"""
+    if isinstance(rf, (RFCavity, AccelCavity)):
+        return rf.h
+    return 1.0
+
+
+def _record_extra(e, bt: float):
+    """Records of the remaining element families (src/tracking.jl:529-605, 981-1218,
+    1531-1663), or None."""
+    if isinstance(e, (QuadrupoleSC, SextupoleSC, OctupoleSC)):
+        a, b, m = _sc_poly(e)
+        if isinstance(e, QuadrupoleSC):
+            _kick_corrected(e, a, b)     # src/tracking.jl:1004-1007
+        n = int(e.num_int_steps)
+        if n < 0:
+            raise ValueError(f"{e.name!r}: num_int_steps {n} < 0")
+        L1, L2, K1, K2 = _yoshida(e.L, n)
+        mflags, mblk = _misalign(e)
+        return TAGS["MPOLE"], n, m, mflags, -1, 0.0, [L1, L2, K1, K2, L1 * bt, L2 * bt] + _interleave(a, b, m), mblk
+    if isinstance(e, SBendSC):
+        mflags, mblk = _misalign(e)
+        if e.L == 0.0:                   # src/tracking.jl:1047-1049
+            return TAGS["NAN"], 0, 0, 0, -1, 0.0, [], None
+        a = np.zeros(MAXORD + 1)
+        b = np.zeros(MAXORD + 1)
+        a[:4], b[:4] = e.polynom_a, e.polynom_b
+        m = min(max(int(e.max_order), 0), 3)
+        _kick_corrected(e, a, b)
+        irho = e.angle / e.L
+        n = int(e.num_int_steps)
+        L1, L2, K1, K2 = _yoshida(e.L, n)
+        fin = _fringe_method(e.fringe_bend_entrance, "fringe_bend_entrance")
+        fout = _fringe_method(e.fringe_bend_exit, "fringe_bend_exit")
+        blk = [L1, L2, K1, K2, L1 * bt, L2 * bt, irho, K1 * irho, K2 * irho]
+        blk += _edge_block(irho, e.e1, e.fint1, e.gap, fin) + _edge_block(irho, e.e2, e.fint2, e.gap, fout)
+        blk += _interleave(a, b, m)
+        return TAGS["SBEND"], n, m, mflags | (fin << F_FIN_SHIFT) | (fout << F_FOUT_SHIFT), -1, 0.0, blk, mblk
+    if isinstance(e, Solenoid):
+        mflags, mblk = _misalign(e)
+        if e.ks == 0.0:                  # src/tracking.jl:539-540: a drift
+            return TAGS["DRIFT"], 0, 0, mflags, -1, 0.0, [e.L, e.L * bt], mblk
+        return TAGS["SOLENOID"], 0, 0, mflags, -1, 0.0, [e.L, e.ks], mblk
+    if isinstance(e, Corrector):
+        mflags, mblk = _misalign(e)
+        xk, yk = e.xkick, e.ykick        # src/tracking.jl:588-596
+        blk = [e.L, xk, yk, (xk * xk) / 3 + (yk * yk) / 3, xk / 2, yk / 2]
+        return TAGS["CORRECTOR"], 0, 0, mflags, -1, 0.0, blk, mblk
+    if isinstance(e, Translation):
+        return TAGS["TRANSLATION"], 0, 0, 0, -1, 0.0, [e.dx, e.dy, e.ds], None
+    if isinstance(e, YRotation):
+        ang = -e.angle                   # src/tracking.jl:1195
+        if ang == 0.0:
+            return TAGS["MARKER"], 0, 0, 0, -1, 0.0, [], None
+        return TAGS["YROTATION"], 0, 0, 0, -1, 0.0, [math.cos(ang), math.sin(ang), math.tan(ang)], None
+    if isinstance(e, CrabCavity):
+        volt = e.volt * (1.0 + float(e.errors[0]))   # src/tracking.jl:1532-1534
+        phi = e.phi + float(e.errors[1])
+        E = max(abs(e.energy), EPS)
+        blk = [e.L / 2, (e.L / 2) * bt, volt / E, e.k, phi, e.k * volt / E * bt]
+        return TAGS["CRAB"], 0, 0, (F_RF_DRIFT if e.L > 0.0 else 0), -1, 0.0, blk, None
+    if isinstance(e, AccelCavity):
+        beta = 1.0 / bt                  # src/tracking.jl:1550-1558
+        beta2 = beta * beta
+        if beta2 <= EPS:
+            return TAGS["MARKER"], 0, 0, 0, -1, 0.0, [], None
+        E = max(abs(e.energy), EPS)
+        return TAGS["ACCEL"], 0, 0, 0, -1, 0.0, [e.volt / (beta2 * E), e.k, e.phis, math.sin(e.phis)], None
+    if isinstance(e, LongitudinalRFMap):
+        k = _rf_k(e.rf)                  # src/tracking.jl:1575-1585
+        if abs(k) <= EPS:
+            return TAGS["MARKER"], 0, 0, 0, -1, 0.0, [], None
+        beta = 1.0 / bt
+        eta = e.alphac - (1.0 - beta * beta)
+        return TAGS["LONGRF"], 0, 0, 0, -1, 0.0, [TWO_PI * _rf_h(e.rf) * eta / k], None
+    if isinstance(e, LorentzBoost):
+        if e.mode != 0:
+            return TAGS["MARKER"], 0, 0, 0, -1, 0.0, [], None
+        return TAGS["BOOST"], 0, 0, 0, -1, 0.0, [e.tanang, 1.0 / e.cosang], None
+    if isinstance(e, InvLorentzBoost):
+        if e.mode != 0:
+            return TAGS["MARKER"], 0, 0, 0, -1, 0.0, [], None
+        return TAGS["INVBOOST"], 0, 0, 0, -1, 0.0, [e.sinang, e.cosang], None
+    if isinstance(e, StrongThinGaussianBeam):
+        sx2 = max(e.rmssizex * e.rmssizex, EPS)      # src/tracking.jl:1611-1612
+        sy2 = max(e.rmssizey * e.rmssizey, EPS)
+        return TAGS["GAUSSBB"], 1, 0, 0, -1, 0.0, [sx2, sy2, 1.0, e.amplitude, e.xoffset, e.yoffset], None
+    if isinstance(e, StrongGaussianBeam):
+        n = int(e.nzslice)
+        if n <= 0:
+            return TAGS["MARKER"], 0, 0, 0, -1, 0.0, [], None
+        sx2 = max(float(e.beamsize[0]) ** 2, EPS)    # src/tracking.jl:1625-1627
+        sy2 = max(float(e.beamsize[1]) ** 2, EPS)
+        blk = [sx2, sy2, e.charge / max(abs(e.total_energy), EPS)]
+        for i in range(n):
+            w = float(e.zslice_npar[i]) if i < len(e.zslice_npar) else 1.0 / n
+            xo = float(e.xoffsets[i]) if i < len(e.xoffsets) else 0.0
+            yo = float(e.yoffsets[i]) if i < len(e.yoffsets) else 0.0
+            blk += [w, xo, yo]
+        return TAGS["GAUSSBB"], n, 0, 0, -1, 0.0, blk, None
+    if isinstance(e, LongitudinalRLCWake):
+        if e.scale == 0.0:
+            return TAGS["MARKER"], 0, 0, 0, -1, 0.0, [], None
+        Q0p = math.sqrt(e.Q0 * e.Q0 - 0.25) if e.Q0 * e.Q0 >= 0.25 else math.nan  # src/elements/wake.jl:25-36
+        w0 = TWO_PI * e.freq
+        w0p = w0 / e.Q0 * Q0p
+        blk = [e.scale, e.Rshunt * w0 / e.Q0, w0p, 2 * Q0p, w0, 2 * e.Q0]
+        return TAGS["RLCWAKE"], 0, 0, 0, -1, 0.0, blk, None
+    if isinstance(e, LongitudinalWake):
+        if e.scale == 0.0:
+            return TAGS["MARKER"], 0, 0, 0, -1, 0.0, [], None
+        n = len(e.times)
+        blk = [e.scale, e.fliphalf, float(e.times[0]) * e.fliphalf]
+        blk += list(map(float, e.times)) + list(map(float, e.wakefields))
+        return TAGS["TABWAKE"], n, 0, 0, -1, 0.0, blk, None
+    if isinstance(e, LBend):
+        mflags, mblk = _misalign(e)
+        if e.L == 0.0:                   # src/tracking.jl:1158-1160
+            return TAGS["NAN"], 0, 0, 0, -1, 0.0, [], None
+        irho = e.angle / e.L
+        blk = [e.L, e.K, e.angle, e.by_error, e.angle / e.L]
+        blk += _edge_block(irho, e.e1, e.fint1, e.full_gap, 1) + _edge_block(irho, e.e2, e.fint2, e.full_gap, 1)
+        return TAGS["LBEND"], 0, 0, mflags | (1 << F_FIN_SHIFT) | (1 << F_FOUT_SHIFT), -1, 0.0, blk, mblk
+    return None
 
 
 def compile_table(lat: Lattice, beam: Beam, *, aperture: Optional[float] = None,
