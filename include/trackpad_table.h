@@ -70,6 +70,11 @@
  *                n = rec.nsteps                                           src/tracking.jl:1656-1663
  *   TP_LBEND     [0] L [1] K [2] angle [3] by_error [4] angle/L, then the entrance and exit
  *                edge blocks (fx, fy, e, fc) of TP_SBEND at [5] and [9]   src/tracking.jl:1076-1170
+ *   TP_WIGGLER   [0] Nstep [1] dl*1.3512.. [2] dl*(-1.7024..) [3] NH [4] NV [5] rad
+ *                [6] Po*m/c [7] srCoef [8] dl (dl = lw/Nsteps, Nstep = rec.nsteps), then 11
+ *                doubles per harmonic, NH of By then NV of Bx: kx ky kz tz, the leading
+ *                factors of the ax, axpy, ay, aypx terms and of Bx, By, and 1 / 0 for the
+ *                sin(k u)/k branch                                        src/tracking.jl:1224-1525
  *   TP_NAN       (none): every coordinate becomes NaN -- the reference's _nan6 for SBendSC and
  *                LBend of zero length (src/tracking.jl:1047-1049, 1158-1160)
  *
@@ -108,7 +113,8 @@ enum tp_tag {
   TP_TABWAKE = 19,   /* LongitudinalWake (tabulated) */
   TP_LBEND = 20,     /* LBend */
   TP_NAN = 21,       /* the reference's NaN map */
-  TP_NTAGS = 22
+  TP_WIGGLER = 22,   /* Wiggler */
+  TP_NTAGS = 23
 };
 /* Also written with the tags above: DriftSC (TP_DRIFT), QuadrupoleSC / SextupoleSC /
  * OctupoleSC (TP_MPOLE), SBendSC / RBendSC (TP_SBEND), SpaceCharge (TP_MARKER: its

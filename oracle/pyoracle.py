@@ -23,7 +23,7 @@ ORC_MARKER, ORC_DRIFT, ORC_QUAD, ORC_SEXT, ORC_OCT, ORC_THINMP, ORC_SBEND, ORC_E
     ORC_RFCAVITY, ORC_EXACTSBEND_LE = range(10)
 (ORC_SOLENOID, ORC_CORRECTOR, ORC_TRANSLATION, ORC_YROTATION, ORC_CRAB, ORC_ACCEL, ORC_LONGRF, ORC_BOOST,
  ORC_INVBOOST, ORC_THINBB, ORC_SGB, ORC_RLCWAKE, ORC_TABWAKE, ORC_DRIFTSC, ORC_QUADSC, ORC_SEXTSC, ORC_OCTSC,
- ORC_SBENDSC, ORC_LBEND, ORC_SPACECHARGE) = range(10, 30)
+ ORC_SBENDSC, ORC_LBEND, ORC_SPACECHARGE, ORC_WIGGLER) = range(10, 31)
 NX = 128
 
 
@@ -84,7 +84,7 @@ def _kind(e) -> int:
             "InvLorentzBoost": ORC_INVBOOST, "StrongThinGaussianBeam": ORC_THINBB, "StrongGaussianBeam": ORC_SGB,
             "LongitudinalRLCWake": ORC_RLCWAKE, "LongitudinalWake": ORC_TABWAKE, "DriftSC": ORC_DRIFTSC,
             "QuadrupoleSC": ORC_QUADSC, "SextupoleSC": ORC_SEXTSC, "OctupoleSC": ORC_OCTSC, "SBendSC": ORC_SBENDSC,
-            "LBend": ORC_LBEND, "SpaceCharge": ORC_SPACECHARGE}[n]
+            "LBend": ORC_LBEND, "SpaceCharge": ORC_SPACECHARGE, "Wiggler": ORC_WIGGLER}[n]
 
 
 def _extra_fields(e, o) -> None:
@@ -131,6 +131,11 @@ def _extra_fields(e, o) -> None:
     elif n == "LBend":
         x = [e.K, e.by_error]
         o.gap = e.full_gap
+    elif n == "Wiggler":
+        nh, nv = min(e.NHharm, len(e.By) // 6), min(e.NVharm, len(e.Bx) // 6)
+        if 6 + 6 * (nh + nv) > NX:
+            raise ValueError("the oracle holds at most 20 wiggler harmonics")
+        x = [e.lw, e.Bmax, e.Nsteps, e.rad, nh, nv] + list(map(float, e.By[:6 * nh])) + list(map(float, e.Bx[:6 * nv]))
     for i, v in enumerate(x):
         o.x[i] = float(v)
 

@@ -153,6 +153,8 @@ enum OrcKind {
   ORC_LBEND = 28,         // src/elements/SCelements.jl:346-379  L, angle, e1, e2, fint1, fint2, gap =
                           //                                     full_gap, x = K, by_error
   ORC_SPACECHARGE = 29,   // src/elements/SCelements.jl:394-408
+  ORC_WIGGLER = 30,       // src/elements/wiggler.jl:6-44         L, energy, x = lw, Bmax, Nsteps, rad,
+                          //   NHharm, NVharm, By[6 NHharm], Bx[6 NVharm]
 };
 #define ORC_NX 128
 
@@ -594,6 +596,190 @@ static R6<T> lbend_body(R6<T> r, T L, T grd, T b_angle, T by_error) {
   return o;
 }
 
+// Wiggler: src/tracking.jl:1224-1525.  The harmonic tables are read from the raw By / Bx
+// integers every call, as the reference's _wig_* functions read them.
+static const double M_ELECTRON_EV = 0.51099895069e6;  // src/lattice.jl:42
+
+template <class T>
+struct Wig {
+  const Elem<T>* e;
+  int nh, nv;
+  T lw, Bmax, Aw, Po, kw;
+  const T* by() const { return e->x + 6; }
+  const T* bx() const { return e->x + 6 + 6 * nh; }
+};
+
+template <class T>
+static T sinc_taylor(T x) {  // :1224-1227
+  T x2 = x * x;
+  return T(1.0) - x2 / T(6.0) * (T(1.0) - x2 / T(20.0) * (T(1.0) - x2 / T(42.0) * (T(1.0) - x2 / T(72.0))));
+}
+
+template <class T>
+static void wig_ax_axpy(const Wig<T>& w, T x, T y, T Zw, T& ax, T& axpy) {  // :1229-1279
+  T kw = w.kw;
+  ax = T(0.0);
+  axpy = T(0.0);
+  for (int i = 0; i < w.nh; ++i) {
+    const T* b = w.by() + 6 * i;
+    T HCw = b[1] * w.Aw / w.Po;
+    T kx = b[2] * kw, ky = b[3] * kw, kz = b[4] * kw, tz = b[5];
+    T cx = cos(kx * x), chy = cosh(ky * y), sz = sin(kz * Zw + tz);
+    ax = ax + HCw * (kw / kz) * cx * chy * sz;
+    T shy = sinh(ky * y);
+    T sxkx = fabs(kx / kw) > T(1e-6) ? sin(kx * x) / kx : x * sinc_taylor(kx * x);
+    axpy = axpy + HCw * (kw / kz) * ky * sxkx * shy * sz;
+  }
+  for (int i = 0; i < w.nv; ++i) {
+    const T* b = w.bx() + 6 * i;
+    T VCw = b[1] * w.Aw / w.Po;
+    T kx = b[2] * kw, ky = b[3] * kw, kz = b[4] * kw, tz = b[5];
+    T shx = sinh(kx * x), sy = sin(ky * y), sz = sin(kz * Zw + tz);
+    ax = ax + VCw * (kw / kz) * (ky / kx) * shx * sy * sz;
+    T chx = cosh(kx * x), cy = cos(ky * y);
+    axpy = axpy + VCw * (kw / kz) * ((ky / kx) * (ky / kx)) * chx * cy * sz;
+  }
+}
+
+template <class T>
+static void wig_ay_aypx(const Wig<T>& w, T x, T y, T Zw, T& ay, T& aypx) {  // :1281-1331
+  T kw = w.kw;
+  ay = T(0.0);
+  aypx = T(0.0);
+  for (int i = 0; i < w.nh; ++i) {
+    const T* b = w.by() + 6 * i;
+    T HCw = b[1] * w.Aw / w.Po;
+    T kx = b[2] * kw, ky = b[3] * kw, kz = b[4] * kw, tz = b[5];
+    T sx = sin(kx * x), shy = sinh(ky * y), sz = sin(kz * Zw + tz);
+    ay = ay + HCw * (kw / kz) * (kx / ky) * sx * shy * sz;
+    T cx = cos(kx * x), chy = cosh(ky * y);
+    aypx = aypx + HCw * (kw / kz) * ((kx / ky) * (kx / ky)) * cx * chy * sz;
+  }
+  for (int i = 0; i < w.nv; ++i) {
+    const T* b = w.bx() + 6 * i;
+    T VCw = b[1] * w.Aw / w.Po;
+    T kx = b[2] * kw, ky = b[3] * kw, kz = b[4] * kw, tz = b[5];
+    T chx = cosh(kx * x), cy = cos(ky * y), sz = sin(kz * Zw + tz);
+    ay = ay + VCw * (kw / kz) * chx * cy * sz;
+    T shx = sinh(kx * x);
+    T syky = fabs(ky / kw) > T(1e-6) ? sin(ky * y) / ky : y * sinc_taylor(ky * y);
+    aypx = aypx + VCw * (kw / kz) * kx * shx * syky * sz;
+  }
+}
+
+template <class T>
+static R6<T> wig_map_2nd(const Wig<T>& w, R6<T> r, T dl, T& Zw) {  // :1333-1379
+  T delta = r.v[5];
+  T inv1pd = T(1.0) / (T(1.0) + delta);
+  T dld = dl * inv1pd, dl2 = dl / T(2.0), dl2d = dl2 * inv1pd;
+  Zw = Zw + dl2;
+  T x = r.v[0], px = r.v[1], y = r.v[2], py = r.v[3], z = r.v[4];
+  T a, b;
+  wig_ay_aypx(w, x, y, Zw, a, b);
+  px = px - b; py = py - a;
+  y = y + dl2d * py;
+  z = z + (dl2d / T(2.0)) * (py * py) * inv1pd;
+  wig_ay_aypx(w, x, y, Zw, a, b);
+  px = px + b; py = py + a;
+  wig_ax_axpy(w, x, y, Zw, a, b);
+  px = px - a; py = py - b;
+  x = x + dld * px;
+  z = z + (dld / T(2.0)) * (px * px) * inv1pd;
+  wig_ax_axpy(w, x, y, Zw, a, b);
+  px = px + a; py = py + b;
+  wig_ay_aypx(w, x, y, Zw, a, b);
+  px = px - b; py = py - a;
+  y = y + dl2d * py;
+  z = z + (dl2d / T(2.0)) * (py * py) * inv1pd;
+  wig_ay_aypx(w, x, y, Zw, a, b);
+  px = px + b; py = py + a;
+  Zw = Zw + dl2;
+  R6<T> o = {{x, px, y, py, z, delta}};
+  return o;
+}
+
+template <class T>
+static void wig_B(const Wig<T>& w, T x, T y, T Zw, T& Bx, T& By) {  // :1401-1457 (Bz unused)
+  T kw = w.kw, PB0 = w.Bmax;
+  Bx = T(0.0);
+  By = T(0.0);
+  for (int i = 0; i < w.nh; ++i) {
+    const T* b = w.by() + 6 * i;
+    T HCw = b[1];
+    T kx = b[2] * kw, ky = b[3] * kw, kz = b[4] * kw, tz = b[5];
+    T sx = sin(kx * x), cx = cos(kx * x), chy = cosh(ky * y), shy = sinh(ky * y);
+    T cz = cos(kz * Zw + tz);
+    Bx = Bx + PB0 * HCw * (kx / ky) * sx * shy * cz;
+    By = By - PB0 * HCw * cx * chy * cz;
+  }
+  for (int i = 0; i < w.nv; ++i) {
+    const T* b = w.bx() + 6 * i;
+    T VCw = b[1];
+    T kx = b[2] * kw, ky = b[3] * kw, kz = b[4] * kw, tz = b[5];
+    T shx = sinh(kx * x), chx = cosh(kx * x), sy = sin(ky * y), cy = cos(ky * y);
+    T cz = cos(kz * Zw + tz);
+    Bx = Bx + PB0 * VCw * chx * cy * cz;
+    By = By - PB0 * VCw * (ky / kx) * shx * sy * cz;
+  }
+}
+
+// the ax / ay transform around _wig_B and _wig_radiation_kicks, :1459-1470, 1491-1496
+template <class T>
+static R6<T> wig_rad(const Wig<T>& w, R6<T> r, T Zw, T srCoef, T SL) {
+  T ax, ay, unused;
+  wig_ax_axpy(w, r.v[0], r.v[2], Zw, ax, unused);
+  wig_ay_aypx(w, r.v[0], r.v[2], Zw, ay, unused);
+  r.v[1] = r.v[1] - ax;
+  r.v[3] = r.v[3] - ay;
+  T Bx, By;
+  wig_B(w, r.v[0], r.v[2], Zw, Bx, By);
+  T B2 = Bx * Bx + By * By;
+  if (B2 != T(0.0)) {
+    T H = w.Po * T(M_ELECTRON_EV) / T(C_LIGHT);
+    T irho2 = B2 / (H * H);
+    T dFactor = (T(1.0) + r.v[5]) * (T(1.0) + r.v[5]);
+    T dDelta = -srCoef * dFactor * irho2 * SL;
+    T scale = T(1.0) + dDelta;
+    r.v[1] = r.v[1] * scale;
+    r.v[3] = r.v[3] * scale;
+    r.v[5] = r.v[5] + dDelta;
+  }
+  r.v[1] = r.v[1] + ax;
+  r.v[3] = r.v[3] + ay;
+  return r;
+}
+
+// _wig_pass_4th / _wig_pass_4th_rad, :1381-1399, 1472-1507
+template <class T>
+static R6<T> wiggler_pass(const Elem<T>& e, R6<T> r) {
+  Wig<T> w;
+  w.e = &e;
+  w.lw = e.x[0];
+  w.Bmax = e.x[1];
+  int PN = (int)val(e.x[2]);
+  int rad = (int)val(e.x[3]);
+  w.nh = (int)val(e.x[4]);
+  w.nv = (int)val(e.x[5]);
+  long Nw = (long)std::nearbyint(val(e.L) / val(w.lw));  // round(Int, L / lw): ties to even
+  long Nstep = (long)PN * std::max(1L, Nw);
+  T SL = w.lw / T((double)PN);
+  T dl1 = SL * T(1.3512071919596573), dl0 = SL * T(-1.7024143839193146);
+  T gamma = e.energy / T(M_ELECTRON_EV);
+  w.Po = sqrt(gamma * gamma - T(1.0));
+  w.Aw = T(1e-9) * T(C_LIGHT) / (T(M_ELECTRON_EV) * T(1e-9)) / T(TWO_PI) * w.lw * w.Bmax;
+  w.kw = T(TWO_PI) / w.lw;
+  T srCoef = T(2.0 / 3.0) * T(2.8179403205e-15) * (gamma * gamma * gamma);
+  T Zw = T(0.0);
+  if (rad == 1) r = wig_rad(w, r, Zw, srCoef, SL);
+  for (long k = 0; k < Nstep; ++k) {
+    r = wig_map_2nd(w, r, dl1, Zw);
+    r = wig_map_2nd(w, r, dl0, Zw);
+    r = wig_map_2nd(w, r, dl1, Zw);
+    if (rad == 1) r = wig_rad(w, r, Zw, srCoef, SL);
+  }
+  return r;
+}
+
 // pass! of the other element families: src/tracking.jl:529-605, 981-1218, 1531-1663
 template <class T>
 static R6<T> pass_extra(const Elem<T>& e, R6<T> r, T beti, bool exact) {
@@ -808,6 +994,10 @@ static R6<T> pass_extra(const Elem<T>& e, R6<T> r, T beti, bool exact) {
     }
     case ORC_SPACECHARGE:  // :1176-1180
       return r;
+    case ORC_WIGGLER:  // :1509-1525
+      r = entrance(e, r);
+      r = wiggler_pass(e, r);
+      return exit_(e, r);
     default:
       return nan6<T>();
   }

@@ -139,3 +139,24 @@ def test_time_varying_lattice_rows():
     assert tq.materialize(0.2, 0).num_int_steps == 8 and tq.materialize(1.0, 5).num_int_steps == 12
     with pytest.raises(ValueError, match="no field"):
         timed(Quadrupole(0.4, 0.6), k7=1.0)
+
+def test_wiggler_block_is_validated():
+    """The wiggler's block length follows its harmonic counts; counts that run past the pool
+    are refused."""
+    from trackpad_hip import Wiggler
+
+    beam = Beam(3e9)
+    blob = bytearray(compile_table(Lattice([Wiggler(1.2, lw=0.2, Bmax=0.8, Nsteps=8, Bx=[1, 1, 5, 3, 4, 0])]), beam))
+    rc, f, _ = _check(bytes(blob))
+    assert rc == 0 and f & 128
+    n_seq = struct.unpack_from("<I", blob, 8)[0]
+    rec0 = 64 + ((n_seq * 4 + 7) // 8) * 8
+    n_rec = struct.unpack_from("<I", blob, 12)[0]
+    pool0 = rec0 + 32 * n_rec
+    off = struct.unpack_from("<i", blob, rec0 + 12)[0]
+    struct.pack_into("<d", blob, pool0 + 8 * (off + 3), 3.0)  # three By harmonics: past the pool
+    rc, _, msg = _check(bytes(blob))
+    assert rc == -5 and "pool" in msg
+    struct.pack_into("<d", blob, pool0 + 8 * (off + 3), -1.0)
+    rc, _, msg = _check(bytes(blob))
+    assert rc == -5 and "harmonics" in msg

@@ -58,6 +58,7 @@ int64_t block_len(int tag, int m, int nsteps) {
     case TP_TABWAKE: return 3 + 2 * (int64_t)nsteps;
     case TP_LBEND: return 13;
     case TP_NAN: return 0;
+    case TP_WIGGLER: return -2;  // length depends on the block's harmonic counts
     default: return -1;
   }
 }
@@ -250,11 +251,20 @@ int parse_table(const void* bytes, size_t nbytes, tp_table* t) {
     if (r.nsteps < 0 || r.nsteps > 1000000) return fail(TP_ETABLE, "table: record %u: nsteps %d", i, r.nsteps);
     if (r.tag == TP_GAUSSBB && r.nsteps < 1) return fail(TP_ETABLE, "table: record %u: beam-beam without slices", i);
     if (r.tag == TP_TABWAKE && r.nsteps < 2) return fail(TP_ETABLE, "table: record %u: wake table shorter than 2", i);
-    const int64_t len = block_len(r.tag, r.max_order, r.nsteps);
+    int64_t len = block_len(r.tag, r.max_order, r.nsteps);
+    if (len == -2) {  // TP_WIGGLER: 9 + 11 per harmonic
+      if (r.off < 0 || (int64_t)r.off + 9 > (int64_t)h.pool_len)
+        return fail(TP_ETABLE, "table: record %u: parameter block outside the pool", i);
+      const double nh = t->pool[(size_t)r.off + 3], nv = t->pool[(size_t)r.off + 4];
+      if (!(nh >= 0.0 && nv >= 0.0 && nh + nv <= 64.0) || r.nsteps < 0)
+        return fail(TP_ETABLE, "table: record %u: wiggler with %g + %g harmonics", i, nh, nv);
+      len = 9 + 11 * ((int64_t)nh + (int64_t)nv);
+    }
     if (r.off < 0 || (int64_t)r.off + len > (int64_t)h.pool_len)
       return fail(TP_ETABLE, "table: record %u: parameter block outside the pool", i);
     if (r.flags & TP_F_MISALIGN) {
-      if (r.tag == TP_MARKER || r.tag == TP_RFCAV || (r.tag >= TP_TRANSLATION && r.tag != TP_LBEND))
+      if (r.tag == TP_MARKER || r.tag == TP_RFCAV ||
+          (r.tag >= TP_TRANSLATION && r.tag != TP_LBEND && r.tag != TP_WIGGLER))
         return fail(TP_ETABLE, "table: record %u: misalignment on an element without one", i);
       if (r.moff < 0 || (int64_t)r.moff + 84 > (int64_t)h.pool_len)
         return fail(TP_ETABLE, "table: record %u: misalignment block outside the pool", i);

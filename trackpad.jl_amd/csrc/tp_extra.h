@@ -207,6 +207,152 @@ TP_DEV St lbend_body(St s, const double* __restrict__ p) {
   return St{x_new, px_new, y_new, py_new, z_new, s.d};
 }
 
+// ---------------------------------------------------------------------------------------
+// Wiggler, src/tracking.jl:1224-1525.  Harmonic h (11 doubles): kx ky kz tz, then the
+// leading factors of the ax, axpy, ay, aypx terms, of Bx and By, and the sinc flag.
+// ---------------------------------------------------------------------------------------
+TP_DEV double sinc_taylor(double x) {  // _sinc_taylor, :1224-1227
+  const double x2 = x * x;
+  return 1.0 - x2 / 6.0 * (1.0 - x2 / 20.0 * (1.0 - x2 / 42.0 * (1.0 - x2 / 72.0)));
+}
+
+// _wig_ax_axpy (:1229-1279) and _wig_ay_aypx (:1281-1331) at (x, y, Zw)
+TP_DEV void wig_ax(const double* __restrict__ h, int nh, int nv, double x, double y, double Zw, double& ax,
+                   double& axpy) {
+  ax = 0.0;
+  axpy = 0.0;
+  for (int i = 0; i < nh; ++i, h += 11) {
+    const double kx = h[0], ky = h[1], kz = h[2], tz = h[3];
+    const double cx = cos(kx * x), chy = cosh(ky * y), sz = sin(kz * Zw + tz);
+    ax = ax + h[4] * cx * chy * sz;
+    const double shy = sinh(ky * y);
+    const double sxkx = h[10] != 0.0 ? sin(kx * x) / kx : x * sinc_taylor(kx * x);
+    axpy = axpy + h[5] * sxkx * shy * sz;
+  }
+  for (int i = 0; i < nv; ++i, h += 11) {
+    const double kx = h[0], ky = h[1], kz = h[2], tz = h[3];
+    const double shx = sinh(kx * x), sy = sin(ky * y), sz = sin(kz * Zw + tz);
+    ax = ax + h[4] * shx * sy * sz;
+    const double chx = cosh(kx * x), cy = cos(ky * y);
+    axpy = axpy + h[5] * chx * cy * sz;
+  }
+}
+
+TP_DEV void wig_ay(const double* __restrict__ h, int nh, int nv, double x, double y, double Zw, double& ay,
+                   double& aypx) {
+  ay = 0.0;
+  aypx = 0.0;
+  for (int i = 0; i < nh; ++i, h += 11) {
+    const double kx = h[0], ky = h[1], kz = h[2], tz = h[3];
+    const double sx = sin(kx * x), shy = sinh(ky * y), sz = sin(kz * Zw + tz);
+    ay = ay + h[6] * sx * shy * sz;
+    const double cx = cos(kx * x), chy = cosh(ky * y);
+    aypx = aypx + h[7] * cx * chy * sz;
+  }
+  for (int i = 0; i < nv; ++i, h += 11) {
+    const double kx = h[0], ky = h[1], kz = h[2], tz = h[3];
+    const double chx = cosh(kx * x), cy = cos(ky * y), sz = sin(kz * Zw + tz);
+    ay = ay + h[6] * chx * cy * sz;
+    const double shx = sinh(kx * x);
+    const double syky = h[10] != 0.0 ? sin(ky * y) / ky : y * sinc_taylor(ky * y);
+    aypx = aypx + h[7] * shx * syky * sz;
+  }
+}
+
+// _wig_map_2nd, :1333-1379 (Zw is advanced by the caller's copy)
+TP_DEV void wig_map_2nd(St& s, const double* __restrict__ h, int nh, int nv, double dl, double& Zw) {
+  const double delta = s.d;
+  const double inv1pd = 1.0 / (1.0 + delta);
+  const double dld = dl * inv1pd;
+  const double dl2 = dl / 2.0;
+  const double dl2d = dl2 * inv1pd;
+  Zw = Zw + dl2;
+  double x = s.x, px = s.px, y = s.y, py = s.py, z = s.z;
+  double a, b;
+  wig_ay(h, nh, nv, x, y, Zw, a, b);
+  px = px - b;
+  py = py - a;
+  y = y + dl2d * py;
+  z = z + (dl2d / 2.0) * (py * py) * inv1pd;
+  wig_ay(h, nh, nv, x, y, Zw, a, b);
+  px = px + b;
+  py = py + a;
+  wig_ax(h, nh, nv, x, y, Zw, a, b);
+  px = px - a;
+  py = py - b;
+  x = x + dld * px;
+  z = z + (dld / 2.0) * (px * px) * inv1pd;
+  wig_ax(h, nh, nv, x, y, Zw, a, b);
+  px = px + a;
+  py = py + b;
+  wig_ay(h, nh, nv, x, y, Zw, a, b);
+  px = px - b;
+  py = py - a;
+  y = y + dl2d * py;
+  z = z + (dl2d / 2.0) * (py * py) * inv1pd;
+  wig_ay(h, nh, nv, x, y, Zw, a, b);
+  px = px + b;
+  py = py + a;
+  Zw = Zw + dl2;
+  s = St{x, px, y, py, z, delta};
+}
+
+// _wig_B (Bx, By; :1401-1457) and _wig_radiation_kicks (:1459-1470), with the ax / ay
+// kicks around them as _wig_pass_4th_rad applies them (:1472-1507)
+TP_DEV void wig_rad_kick(St& s, const double* __restrict__ p, const double* __restrict__ h, int nh, int nv,
+                         double Zw) {
+  double ax, axpy, ay, aypx;
+  wig_ax(h, nh, nv, s.x, s.y, Zw, ax, axpy);
+  wig_ay(h, nh, nv, s.x, s.y, Zw, ay, aypx);
+  s.px = s.px - ax;
+  s.py = s.py - ay;
+  double Bx = 0.0, By = 0.0;
+  const double* q = h;
+  for (int i = 0; i < nh; ++i, q += 11) {
+    const double kx = q[0], ky = q[1], kz = q[2], tz = q[3];
+    const double sx = sin(kx * s.x), cx = cos(kx * s.x), chy = cosh(ky * s.y), shy = sinh(ky * s.y);
+    const double cz = cos(kz * Zw + tz);
+    Bx = Bx + q[8] * sx * shy * cz;
+    By = By - q[9] * cx * chy * cz;
+  }
+  for (int i = 0; i < nv; ++i, q += 11) {
+    const double kx = q[0], ky = q[1], kz = q[2], tz = q[3];
+    const double shx = sinh(kx * s.x), chx = cosh(kx * s.x), sy = sin(ky * s.y), cy = cos(ky * s.y);
+    const double cz = cos(kz * Zw + tz);
+    Bx = Bx + q[8] * chx * cy * cz;
+    By = By - q[9] * shx * sy * cz;
+  }
+  const double B2 = Bx * Bx + By * By;
+  if (B2 != 0.0) {
+    const double H = p[6];
+    const double irho2 = B2 / (H * H);
+    const double dFactor = (1.0 + s.d) * (1.0 + s.d);
+    const double dDelta = -p[7] * dFactor * irho2 * p[8];
+    const double scale = 1.0 + dDelta;
+    s.px = s.px * scale;
+    s.py = s.py * scale;
+    s.d = s.d + dDelta;
+  }
+  s.px = s.px + ax;
+  s.py = s.py + ay;
+}
+
+TP_DEV St wiggler(St s, int nstep, const double* __restrict__ p) {
+  const double dl1 = p[1], dl0 = p[2];
+  const int nh = (int)p[3], nv = (int)p[4];
+  const bool rad = p[5] != 0.0;
+  const double* h = p + 9;
+  double Zw = 0.0;
+  if (rad) wig_rad_kick(s, p, h, nh, nv, Zw);
+  for (int k = 0; k < nstep; ++k) {
+    wig_map_2nd(s, h, nh, nv, dl1, Zw);
+    wig_map_2nd(s, h, nh, nv, dl0, Zw);
+    wig_map_2nd(s, h, nh, nv, dl1, Zw);
+    if (rad) wig_rad_kick(s, p, h, nh, nv, Zw);
+  }
+  return s;
+}
+
 // One element of a family above; the caller rebuilds the delta-dependent hoists after it.
 template <bool EXH>
 __device__ __noinline__ St extra_map(St s, int tag, uint32_t fl, int n, const double* __restrict__ p,
@@ -261,6 +407,8 @@ __device__ __noinline__ St extra_map(St s, int tag, uint32_t fl, int n, const do
       edge_lin(s, 1, p + 9, irho, true);
       return s;
     }
+    case TP_WIGGLER:
+      return wiggler(s, n, p);
     default:  // TP_NAN
       return nan_state();
   }

@@ -12,7 +12,7 @@ from .elements import (  # noqa: F401
     ThinMultipole, TimeVaryingElement, get_length, timed,
     AccelCavity, Corrector, CrabCavity, DriftSC, HKicker, InvLorentzBoost, LBend, LongitudinalRFMap,
     LongitudinalRLCWake, LongitudinalWake, LorentzBoost, OctupoleSC, QuadrupoleSC, RBendSC, SBendSC, SextupoleSC,
-    Solenoid, SpaceCharge, StrongGaussianBeam, StrongThinGaussianBeam, Translation, VKicker, YRotation,
+    Solenoid, SpaceCharge, StrongGaussianBeam, StrongThinGaussianBeam, Translation, VKicker, Wiggler, YRotation,
 )
 from .lattice import (  # noqa: F401
     M_ELECTRON, M_PROTON, Beam, Lattice, beti, findelem, materialize_lattice, spos, total_length,

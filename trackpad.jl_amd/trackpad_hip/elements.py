@@ -21,6 +21,7 @@ constructors so that a lattice written for TrackPad reads the same here:
     LorentzBoost, InvLorentzBoost, StrongThinGaussianBeam, StrongGaussianBeam
                                                     src/elements/beambeam.jl:6-125
     LongitudinalRLCWake, LongitudinalWake           src/elements/wake.jl:6-90
+    Wiggler        src/elements/wiggler.jl:6-44
 
 Two deliberate extensions, both absent from the reference:
 
@@ -48,7 +49,7 @@ __all__ = [
     "CrabCavity", "AccelCavity", "LongitudinalRFMap", "Solenoid", "Corrector", "HKicker", "VKicker",
     "DriftSC", "QuadrupoleSC", "SextupoleSC", "OctupoleSC", "SBendSC", "RBendSC", "LBend", "SpaceCharge",
     "Translation", "YRotation", "LorentzBoost", "InvLorentzBoost", "StrongThinGaussianBeam",
-    "StrongGaussianBeam", "LongitudinalRLCWake", "LongitudinalWake",
+    "StrongGaussianBeam", "LongitudinalRLCWake", "LongitudinalWake", "Wiggler",
 ]
 
 # src/elements/rf.jl:349
@@ -603,6 +604,23 @@ class LongitudinalWake(AbstractElement):
         self.L = 0.0
         self.times, self.wakefields = t, w
         self.fliphalf, self.scale = float(fliphalf), float(scale)
+
+
+class Wiggler(AbstractMagnet, _Misaligned):
+    """Wiggler(L; lw=0, Bmax=0, Nsteps=10, By=[1,1,0,1,1,0], Bx=[], energy=1e9, rad=0, t1, t2,
+    r1, r2) -- src/elements/wiggler.jl:6-44.  By / Bx hold 6 integers per harmonic."""
+
+    def __init__(self, L, *, name="WIGGLER", lw=0.0, Bmax=0.0, Nsteps=10, By=(1, 1, 0, 1, 1, 0), Bx=(),
+                 energy=1.0e9, rad=0, t1=None, t2=None, r1=None, r2=None):
+        self.name = str(name)
+        self.L, self.lw, self.Bmax = float(L), float(lw), float(Bmax)
+        self.Nsteps = int(Nsteps)
+        self.By = [int(v) for v in By]
+        self.Bx = [int(v) for v in Bx]
+        self.energy = float(energy)
+        self.NHharm, self.NVharm = len(self.By) // 6, len(self.Bx) // 6
+        self.rad = int(rad)
+        self._init_misalign(t1, t2, r1, r2, None, None)
 
 
 class TimeVaryingElement(AbstractElement):

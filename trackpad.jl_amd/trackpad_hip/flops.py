@@ -23,6 +23,8 @@ they follow:
     Gaussian beam-beam          15 per slice  :1610-1641
     RLC / tabulated wake        12 / 6        :1647-1663
     LBend                       81 + edges    :1076-1170
+    Wiggler                     per step 3 x (90 per harmonic + 35), transcendentals not counted
+                                              :1224-1525
 
 so one Yoshida-4 step of a thick multipole is 4 x 19 + 3 x (8m + 4).  With these counts
 the 20-cell DBA ring of the benchmark is 313,587 flops per particle-turn and the FODO
@@ -35,7 +37,7 @@ from .elements import (AccelCavity, Corrector, CrabCavity, Drift, DriftSC, Exact
                        InvLorentzBoost, LBend, LongitudinalRFMap, LongitudinalRLCWake, LongitudinalWake, LorentzBoost,
                        Marker, Octupole, OctupoleSC, Quadrupole, QuadrupoleSC, RFCavity, SBend, SBendSC, Sextupole,
                        SextupoleSC, Solenoid, SpaceCharge, StrongGaussianBeam, StrongThinGaussianBeam, ThinMultipole,
-                       Translation, YRotation)
+                       Translation, Wiggler, YRotation)
 from .lattice import Lattice
 
 __all__ = ["elem_flops", "flops_per_turn"]
@@ -89,6 +91,14 @@ def elem_flops(e) -> int:
         return 6 if e.scale != 0 else 0
     if isinstance(e, LBend):
         return 81 + 2 * EDGE
+    if isinstance(e, Wiggler):
+        # per step three 2nd-order maps, each six potential evaluations of ~15 operations per
+        # harmonic and ~35 for the drifts; the radiation kick adds two potentials, the field
+        # (~12 per harmonic) and ~15
+        nhv = min(e.NHharm, len(e.By) // 6) + min(e.NVharm, len(e.Bx) // 6)
+        nstep = e.Nsteps * max(1, round(e.L / e.lw)) if e.lw else 0
+        rad = (42 * nhv + 15) if e.rad == 1 else 0
+        return nstep * (3 * (90 * nhv + 35) + rad) + rad
     if isinstance(e, ThinMultipole):
         return _kick(e.max_order)
     if isinstance(e, SBend):

@@ -21,7 +21,7 @@ from .elements import (
     ExactSBendLinearEdge, InvLorentzBoost, LBend, LongitudinalRFMap, LongitudinalRLCWake, LongitudinalWake,
     LorentzBoost, Marker, Octupole, OctupoleSC, Quadrupole, QuadrupoleSC, RFCavity, SBend, SBendSC, Sextupole,
     SextupoleSC, Solenoid, SpaceCharge, StrongGaussianBeam, StrongThinGaussianBeam, ThinMultipole,
-    TimeVaryingElement, Translation, YRotation,
+    TimeVaryingElement, Translation, Wiggler, YRotation,
 )
 from .lattice import Beam, Lattice, beti
 
@@ -36,7 +36,7 @@ TP_TABLE_VERSION = 1
 TAGS = {"MARKER": 0, "DRIFT": 1, "MPOLE": 2, "THINMP": 3, "SBEND": 4, "ESBEND": 5,
         "ESBEND_LE": 6, "RFCAV": 7, "SOLENOID": 8, "CORRECTOR": 9, "TRANSLATION": 10, "YROTATION": 11,
         "CRAB": 12, "ACCEL": 13, "LONGRF": 14, "BOOST": 15, "INVBOOST": 16, "GAUSSBB": 17, "RLCWAKE": 18,
-        "TABWAKE": 19, "LBEND": 20, "NAN": 21}
+        "TABWAKE": 19, "LBEND": 20, "NAN": 21, "WIGGLER": 22}
 EPS = 2.220446049250313e-16  # eps(Float64)
 
 F_T1, F_T2, F_R1, F_R2 = 0x1, 0x2, 0x4, 0x8
@@ -378,6 +378,8 @@ def _record_extra(e, bt: float):
         blk = [e.scale, e.fliphalf, float(e.times[0]) * e.fliphalf]
         blk += list(map(float, e.times)) + list(map(float, e.wakefields))
         return TAGS["TABWAKE"], n, 0, 0, -1, 0.0, blk, None
+    if isinstance(e, Wiggler):
+        return _wiggler_record(e)
     if isinstance(e, LBend):
         mflags, mblk = _misalign(e)
         if e.L == 0.0:                   # src/tracking.jl:1158-1160
@@ -458,3 +460,54 @@ def compile_table(lat: Lattice, beam: Beam, *, aperture: Optional[float] = None,
     if info is not None:
         info.n_seq, info.n_rec, info.pool_len = len(lat.elements), len(recs), len(pool)
     return hdr + seq_b + rec_b + pool_b
+
+
+M_ELECTRON = 0.51099895069e6  # src/lattice.jl:42
+
+
+def _julia_round(v: float) -> int:
+    """round(Int, v): nearest integer, ties to even (Python's round does the same)."""
+    return int(round(v))
+
+
+def _wiggler_record(e):
+    """TP_WIGGLER: the step plan of _wig_pass_4th / _wig_pass_4th_rad and the
+    particle-independent leading factors of every harmonic term (src/tracking.jl:1224-1525)."""
+    PN = int(e.Nsteps)
+    Nw = _julia_round(e.L / e.lw)
+    Nstep = PN * max(1, Nw)
+    dl = e.lw / float(PN)
+    dl1 = dl * 1.3512071919596573
+    dl0 = dl * -1.7024143839193146
+    gamma = e.energy / M_ELECTRON
+    Po = math.sqrt(gamma * gamma - 1.0)
+    Aw = 1e-9 * C_LIGHT / (M_ELECTRON * 1e-9) / TWO_PI * e.lw * e.Bmax
+    srCoef = (2.0 / 3.0) * 2.8179403205e-15 * (gamma * gamma * gamma)
+    H = Po * M_ELECTRON / C_LIGHT
+    kw = TWO_PI / e.lw
+    PB0 = e.Bmax
+    nh = min(e.NHharm, len(e.By) // 6)
+    nv = min(e.NVharm, len(e.Bx) // 6)
+    blk = [float(Nstep), dl1, dl0, float(nh), float(nv), float(e.rad), H, srCoef, dl]
+    for i in range(nh):
+        b = e.By[6 * i: 6 * i + 6]
+        HCw = float(b[1]) * Aw / Po
+        kx, ky, kz, tz = float(b[2]) * kw, float(b[3]) * kw, float(b[4]) * kw, float(b[5])
+        a = HCw * (kw / kz)
+        blk += [kx, ky, kz, tz,
+                a, a * ky,                                   # ax, axpy
+                a * (kx / ky), a * ((kx / ky) * (kx / ky)),  # ay, aypx
+                PB0 * float(b[1]) * (kx / ky), PB0 * float(b[1]),  # Bx, By of _wig_B
+                1.0 if abs(kx / kw) > 1e-6 else 0.0]
+    for i in range(nv):
+        b = e.Bx[6 * i: 6 * i + 6]
+        VCw = float(b[1]) * Aw / Po
+        kx, ky, kz, tz = float(b[2]) * kw, float(b[3]) * kw, float(b[4]) * kw, float(b[5])
+        a = VCw * (kw / kz)
+        blk += [kx, ky, kz, tz,
+                a * (ky / kx), a * ((ky / kx) * (ky / kx)),  # ax, axpy
+                a, a * kx,                                   # ay, aypx
+                PB0 * float(b[1]), PB0 * float(b[1]) * (ky / kx),
+                1.0 if abs(ky / kw) > 1e-6 else 0.0]
+    mflags, mblk = _misalign(e)
+    return TAGS["WIGGLER"], Nstep, 0, mflags, -1, 0.0, blk, mblk
