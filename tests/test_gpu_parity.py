@@ -216,6 +216,25 @@ def test_loss_semantics(ctx):
     np.testing.assert_array_equal(gc[[0, 1, 3, 4, 5]], oc[[0, 1, 3, 4, 5]])
     assert np.isnan(gc[2, 0]) and np.array_equal(gc[2, 1:], oc[2, 1:])
     assert np.array_equal(gc[0], c[0]) and np.array_equal(gc[4], c[4])
+    # the fast build: the same losses (its drift has no pz^2 test: the refined hardware
+    # reciprocal square root is NaN for pz^2 <= 0) and the same kept entry coordinates
+    fc, flf, flt, fle = gpu_track(ctx, lat, beam, c, 2, lost_flags=lf, fast=True)
+    assert np.array_equal(flf, olf) and np.array_equal(flt, olt) and np.array_equal(fle, ole)
+    np.testing.assert_array_equal(fc[[0, 1, 4]], oc[[0, 1, 4]])
+    for i in (3, 5):
+        assert np.max(np.abs(fc[i] - oc[i])) <= 1e-15
+    # pz^2 exactly 0 (px = 1) and -1.1e-16 (0.6^2 + 0.8^2 rounds above 1) at the first
+    # drift: lost there in both builds with the entry coordinates; px = py = 0.5 passes it
+    e = np.zeros((3, 6))
+    e[0] = [0.0, 1.0, 0.0, 0.0, 0.0, 0.0]
+    e[1] = [0.0, 0.6, 0.0, 0.8, 0.0, 0.0]
+    e[2] = [0.0, 0.5, 0.0, 0.5, 0.0, 0.0]
+    oe = O.ringpass(e, lat, beam, np.zeros(3, np.int64), 1)
+    assert list(oe[1][:2]) == [1, 1] and list(oe[3][:2]) == [0, 0]
+    for fast in (False, True):
+        ge = gpu_track(ctx, lat, beam, e, 1, fast=fast)
+        assert np.array_equal(ge[1], oe[1]) and np.array_equal(ge[2], oe[2]) and np.array_equal(ge[3], oe[3])
+        np.testing.assert_array_equal(ge[0][:2], e[:2])
 
 
 def test_empty_and_ragged(ctx):

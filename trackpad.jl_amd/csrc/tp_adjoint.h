@@ -36,7 +36,11 @@ TP_DEV void drift_back(St& s, Ad& a, const Cx& c, double L, double Lb) {
 #endif
   s.x = s.x - NormL * s.px;
   s.y = s.y - NormL * s.py;
+#if TP_FAST
+  s.z = s.z - __builtin_fma(NormL, c.B, -Lb);
+#else
   s.z = s.z - NormL * c.B + Lb;
+#endif
   // d NormL / d px = NormL px / pz2, / d py = NormL py / pz2, / d delta = -NormL B / pz2
   const double f = (a.x * s.px + a.y * s.py + a.z * c.B) * NormL * ipz2;
   a.px = a.px + a.x * NormL + f * s.px;

@@ -36,6 +36,24 @@ template <bool EXH, int KK, int PPL>
 TP_DEV void mpole_steps(St* s, const Cx* c, const double* __restrict__ p, int nsteps, int m) {
   const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3], L1b = p[4], L2b = p[5];
   const double* poly = p + 6;
+#if TP_FAST
+  if constexpr (KK == TP_KICK_SEXT) {
+    const double Kb1 = K1 * poly[4], Kb2 = K2 * poly[4];
+    for (int st = 0; st < nsteps; ++st) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        drift<EXH>(s[k], c[k], L1, L1b);
+        kick_sext_kb(s[k], Kb1);
+        drift<EXH>(s[k], c[k], L2, L2b);
+        kick_sext_kb(s[k], Kb2);
+        drift<EXH>(s[k], c[k], L2, L2b);
+        kick_sext_kb(s[k], Kb1);
+        drift<EXH>(s[k], c[k], L1, L1b);
+      }
+    }
+    return;
+  }
+#endif
   for (int st = 0; st < nsteps; ++st) {
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
