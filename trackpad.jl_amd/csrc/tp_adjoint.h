@@ -138,34 +138,45 @@ TP_DEV void bkick_back(St& s, Ad& a, const double* __restrict__ poly, int m, dou
 }
 
 // ---------------------------------------------------------------------------------------
-// Exact sector-bend body (Forest 12.18): forward value of x, px, y, z for signed L.
+// Exact sector-bend body (Forest 12.18, src/tracking.jl:858-894) backwards.  s enters as
+// the body's output and leaves as its input.  The inverse is the body of length -L (sn ->
+// -sn); its intermediates are the forward map's: the inverse's pz is the forward output's
+// pzn, its px_new is the input px and its pzn the input pz, and its path term is minus the
+// forward dasin.  The VJP at the input reuses them instead of re-evaluating the map.
+// rho = 1/irho and id2 = 1/d2 are formed by the caller once per element (d2 = pxyz(dp1, 0,
+// py) is constant through the body).
 // ---------------------------------------------------------------------------------------
-TP_DEV void ebody_eval(St& s, const Cx& c, double irho, double L, double Lb, double cs, double sn, double d2) {
-  const double pz = pxyz(c, s.px, s.py);
-  const double pzmx = pz - (1.0 + s.x * irho);
-  const double px_new = s.px * cs + pzmx * sn;
-  const double pzn = pxyz(c, px_new, s.py);
-  const double dasin = L + (asin_s(clamp1(s.px / d2)) - asin_s(clamp1(px_new / d2))) / irho;
-  const double x_new = (pzn - pzmx * cs + s.px * sn - 1.0) / irho;
-  s.y = s.y + s.py * dasin;
-  s.z = s.z + c.B * dasin - Lb;
-  s.x = x_new;
-  s.px = px_new;
-}
-
-// Exact body for length L: invert (body of -L) then apply the VJP at the input.
-TP_DEV void ebody_back(St& s, Ad& a, const Cx& c, double irho, double L, double Lb, double cs, double sn,
-                       double d2) {
-  ebody_eval(s, c, irho, -L, -Lb, cs, -sn, d2);
+TP_DEV void ebody_back(St& s, Ad& a, const Cx& c, double irho, double rho, double L, double Lb, double cs, double sn,
+                       double d2, double id2) {
   const double B = c.B;
-  const double px = s.px, py = s.py;
-  const double pz = pxyz(c, px, py);
-  const double pzmx = pz - (1.0 + s.x * irho);
-  const double pxn = px * cs + pzmx * sn;
-  const double pzn = pxyz(c, pxn, py);
-  const double ipz = 1.0 / pz, ipzn = 1.0 / pzn, id2 = 1.0 / d2, iirho = 1.0 / irho;
-  const double dasin = L + (asin_s(clamp1(px * id2)) - asin_s(clamp1(pxn * id2))) * iirho;
-  // partials with respect to (x, px, py, delta); y and z enter with coefficient 1
+  const double py = s.py;
+  const double pxn = s.px;                       // output px
+  const double pzn = pxyz(c, pxn, py);           // output pz
+  const double pzmx_o = pzn - (1.0 + s.x * irho);
+  const double px = pxn * cs - pzmx_o * sn;      // input px
+  const double pz = pxyz(c, px, py);             // input pz
+#if TP_FAST
+  // asin(px/d2) - asin(pxn/d2) as one series asin, as in the forward body
+  const double a1 = px * id2, a2 = pxn * id2;
+  double dif;
+  if (fabs(a1) < 0.5 && fabs(a2) < 0.5) {
+    dif = asin_f((px * pzn - pxn * pz) * (id2 * id2));
+  } else {
+    dif = asin_s(clamp1(a1)) - asin_s(clamp1(a2));
+  }
+  const double dasin = L + dif * rho;
+  const double x_in = (pz - pzmx_o * cs - pxn * sn - 1.0) * rho;
+  const double ipz = rcp_nr(pz), ipzn = rcp_nr(pzn);
+  (void)d2;
+#else
+  const double dasin = L + (asin_s(clamp1(px / d2)) - asin_s(clamp1(pxn / d2))) / irho;
+  const double x_in = (pz - pzmx_o * cs - pxn * sn - 1.0) / irho;
+  const double ipz = 1.0 / pz, ipzn = 1.0 / pzn;
+  (void)id2;
+#endif
+  const double iirho = rho;
+  // partials of the forward map at the input with respect to (x, px, py, delta); y and z
+  // enter the outputs with coefficient 1
   const double dpz_px = -px * ipz, dpz_py = -py * ipz, dpz_d = B * ipz;
   const double dpxn_x = -irho * sn, dpxn_px = cs + sn * dpz_px, dpxn_py = sn * dpz_py, dpxn_d = sn * dpz_d;
   const double dpzn_x = -pxn * dpxn_x * ipzn, dpzn_px = -pxn * dpxn_px * ipzn;
@@ -178,20 +189,28 @@ TP_DEV void ebody_back(St& s, Ad& a, const Cx& c, double irho, double L, double 
   // dasin = L + (asin(px/d2) - asin(pxn/d2)) / irho, d asin(u) = du / sqrt(1-u^2),
   // sqrt(1 - (px/d2)^2) = pz/d2 and sqrt(1 - (pxn/d2)^2) = pzn/d2; d2 = sqrt(B^2 - py^2)
   const double dd2_py = -py * id2, dd2_d = B * id2;
-  const double g1 = id2 * d2 * ipz, g2 = id2 * d2 * ipzn;  // = 1/pz, 1/pzn
-  const double da_x = (-(dpxn_x * g2)) * iirho;
-  const double da_px = (g1 - dpxn_px * g2) * iirho;
-  const double da_py = ((-px * dd2_py * id2) * g1 - (dpxn_py - pxn * dd2_py * id2) * g2) * iirho;
-  const double da_d = ((-px * dd2_d * id2) * g1 - (dpxn_d - pxn * dd2_d * id2) * g2) * iirho;
+  const double da_x = (-(dpxn_x * ipzn)) * iirho;
+  const double da_px = (ipz - dpxn_px * ipzn) * iirho;
+  const double da_py = ((-px * dd2_py * id2) * ipz - (dpxn_py - pxn * dd2_py * id2) * ipzn) * iirho;
+  const double da_d = ((-px * dd2_d * id2) * ipz - (dpxn_d - pxn * dd2_d * id2) * ipzn) * iirho;
   // outputs: x', px', y' = y + py dasin, z' = z + B dasin - L beti
-  const double ax = a.x * dx_x + a.px * dpxn_x + (a.y * py + a.z * B) * da_x;
-  const double apx = a.x * dx_px + a.px * dpxn_px + (a.y * py + a.z * B) * da_px;
-  const double apy = a.py + a.x * dx_py + a.px * dpxn_py + (a.y * py + a.z * B) * da_py + a.y * dasin;
-  const double ad = a.d + a.x * dx_d + a.px * dpxn_d + (a.y * py + a.z * B) * da_d + a.z * dasin;
+  const double ayz = a.y * py + a.z * B;
+  const double ax = a.x * dx_x + a.px * dpxn_x + ayz * da_x;
+  const double apx = a.x * dx_px + a.px * dpxn_px + ayz * da_px;
+  const double apy = a.py + a.x * dx_py + a.px * dpxn_py + ayz * da_py + a.y * dasin;
+  const double ad = a.d + a.x * dx_d + a.px * dpxn_d + ayz * da_d + a.z * dasin;
   a.x = ax;
   a.px = apx;
   a.py = apy;
   a.d = ad;
+  s.y = s.y - py * dasin;
+#if TP_FAST
+  s.z = s.z - __builtin_fma(B, dasin, -Lb);
+#else
+  s.z = s.z - B * dasin + Lb;
+#endif
+  s.x = x_in;
+  s.px = px;
 }
 
 // ---------------------------------------------------------------------------------------

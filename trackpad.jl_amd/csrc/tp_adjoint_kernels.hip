@@ -74,26 +74,43 @@ template <int KK>
 TP_DEV void esbend_body_back(St& s, Ad& a, const Cx& c, const double* __restrict__ p, int ns, int m) {
   const double irho = p[0];
   double gdummy = 0.0;
+  const double rho = 1.0 / irho;
+#if TP_FAST
+#define TP_ID2(d) rcp_nr(d)
+#else
+#define TP_ID2(d) (1.0 / (d))
+#endif
   double d2 = pxyz(c, 0.0, s.py);
+  double id2 = TP_ID2(d2);
   if (ns == 0) {
-    ebody_back(s, a, c, irho, p[1], p[5], p[7], p[8], d2);
+    ebody_back(s, a, c, irho, rho, p[1], p[5], p[7], p[8], d2, id2);
     return;
   }
   const double L1 = p[1], L2 = p[2], K1 = p[3], K2 = p[4], L1b = p[5], L2b = p[6];
   const double c1 = p[7], s1 = p[8], c2 = p[9], s2 = p[10];
   const double* poly = p + 24;
   for (int st = 0; st < ns; ++st) {
-    ebody_back(s, a, c, irho, L1, L1b, c1, s1, d2);
+    ebody_back(s, a, c, irho, rho, L1, L1b, c1, s1, d2, id2);
     kick_back<KK>(s, a, poly, m, K1, -1, gdummy);
-    if (KK != TP_KICK_NONE) d2 = pxyz(c, 0.0, s.py);
-    ebody_back(s, a, c, irho, L2, L2b, c2, s2, d2);
+    if (KK != TP_KICK_NONE) {
+      d2 = pxyz(c, 0.0, s.py);
+      id2 = TP_ID2(d2);
+    }
+    ebody_back(s, a, c, irho, rho, L2, L2b, c2, s2, d2, id2);
     kick_back<KK>(s, a, poly, m, K2, -1, gdummy);
-    if (KK != TP_KICK_NONE) d2 = pxyz(c, 0.0, s.py);
-    ebody_back(s, a, c, irho, L2, L2b, c2, s2, d2);
+    if (KK != TP_KICK_NONE) {
+      d2 = pxyz(c, 0.0, s.py);
+      id2 = TP_ID2(d2);
+    }
+    ebody_back(s, a, c, irho, rho, L2, L2b, c2, s2, d2, id2);
     kick_back<KK>(s, a, poly, m, K1, -1, gdummy);
-    if (KK != TP_KICK_NONE) d2 = pxyz(c, 0.0, s.py);
-    ebody_back(s, a, c, irho, L1, L1b, c1, s1, d2);
+    if (KK != TP_KICK_NONE) {
+      d2 = pxyz(c, 0.0, s.py);
+      id2 = TP_ID2(d2);
+    }
+    ebody_back(s, a, c, irho, rho, L1, L1b, c1, s1, d2, id2);
   }
+#undef TP_ID2
 }
 
 TP_DEV void sub6(St& s, const double* __restrict__ t) {
@@ -183,11 +200,21 @@ TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq, con
                        const double* __restrict__ pool, const int32_t* __restrict__ slot_of_pos, int32_t n_seq,
                        int32_t nturns, double beti, double* row, int lane) {
   for (int32_t t = 0; t < nturns; ++t) {
+    // scalar loads pipelined as in the forward element loop: the record and slot of
+    // element e-1 and the sequence entry of e-2 are requested while element e runs
+    int32_t ri_next = n_seq > 1 ? seq[n_seq - 2] : 0;
+    tp_rec rec_next = n_seq > 0 ? recs[seq[n_seq - 1]] : tp_rec{};
+    int32_t slot_next = (slot_of_pos && n_seq > 0) ? slot_of_pos[n_seq - 1] : -1;
     for (int32_t e = n_seq - 1; e >= 0; --e) {
-      const tp_rec rec = recs[seq[e]];
+      const tp_rec rec = rec_next;
+      const int32_t slot = slot_next;
+      if (e >= 1) {
+        rec_next = recs[ri_next];
+        slot_next = slot_of_pos ? slot_of_pos[e - 1] : -1;
+        if (e >= 2) ri_next = seq[e - 2];
+      }
       double gacc = 0.0;
       apply_elem_back<FEAT>(s, l, c, rec, pool, beti, gacc);
-      const int32_t slot = slot_of_pos ? slot_of_pos[e] : -1;
       if (slot >= 0) {
         const double w = wave_sum(gacc);
         if (lane == 0) row[slot] += w;
