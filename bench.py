@@ -23,7 +23,18 @@ graph launch: checkpointed forward, reverse pass, per-variant reduction) and its
 relative to the forward, and both against 256 plain per-variant launches on the same GPU
 (measured on every variant with the turn counts scaled by --compare-turn-scale).
 
-Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--workload dba|fodo|batched]
+Workload ``da``: the dynamic-aperture scan -- the 2,000-element ring (80 DBA cells, every
+magnet with order-10 normal and skew errors) on the 1024 x 1024 x 3 grid (3,145,728
+particles); one step is --turns turns (default 20: the full 10,000-turn scan is ~45 minutes
+of GPU time, and the rate is independent of the turn count).
+
+Workload ``diff``: the differentiation configuration -- dL/d(120 k1, 80 k2, x0) of
+L = mean(x^2 + y^2) after 100 turns of the DBA ring over 4,096 particles; one step is one
+tp_adjoint call (forward with checkpoints + reverse pass + reduction).  ``value`` is
+gradient particle-turns/s over the device time of the two passes; the line adds their cost
+relative to plain tracking of the same beam.
+
+Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--workload dba|fodo|da|diff|batched]
                        [--kernel fast|exact] [--turns-per-launch 100]
 """
 
@@ -73,6 +84,16 @@ def workload(name, n_override, t_override):
         n, T = 10_000, 1000
         mk = lambda n_: W.fodo_particles(n_, seed=1)
         aperture = 0.02
+    elif name == "da":
+        lat = W.da_ring(beam=beam)
+        n, T = 3 * 1024 * 1024, 20
+
+        def mk(n_):
+            g = W.da_grid()
+            # a sample of the grid spread over all three momenta and the whole (x, y) square
+            return g if n_ >= len(g) else g[np.linspace(0, len(g) - 1, n_).astype(np.int64)]
+
+        aperture = None
     else:
         raise SystemExit(f"unknown workload {name!r}")
     if n_override:
@@ -97,6 +118,74 @@ def live_turns_(lt, le, n_seq, T):
     """Particle-turns actually tracked: a particle lost at position e of turn t ran t turns
     and e + 1 elements of the next."""
     return float(np.where(lt >= 0, lt + (le + 1) / n_seq, T).sum())
+
+
+def bench_diff(args, fast):
+    from trackpad_hip import Beam, HIPContext, HIPParticles, compile_table, ringpass_grad, track_
+    from trackpad_hip import workloads as W
+    from trackpad_hip.flops import flops_per_turn
+
+    beam = Beam(W.ENERGY)
+    lat = W.dba_ring(beam=beam)
+    n = args.particles or 4096
+    T = args.turns or 100
+    c0 = W.dba_particles(n, seed=4)
+    ctx = HIPContext(0)
+    ctx.set_waves(args.fwd_waves, args.back_waves)
+    tab = ctx.table(compile_table(lat, beam))
+    init = HIPParticles.from_coords(c0, ctx=ctx)
+    work = HIPParticles(n, ctx)
+    print(f"[bench] diff: DBA ring, N={n} T={T} ckpt_stride={args.ckpt_stride} kernel={args.kernel}",
+          file=sys.stderr, flush=True)
+
+    def step():
+        g = ringpass_grad(c0, lat, beam, T, ckpt_stride=args.ckpt_stride, fast=fast, timed=True, ctx=ctx)
+        work.copy_from(init)
+        track_(work, tab, nturns=T, fast=fast, timed=True)
+        return g, ctx.timing()["kernel_ms"]
+
+    for i in range(args.warmup):
+        step()
+        print(f"[bench] warmup step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    fwd = bwd = plain = 0.0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        g, pl = step()
+        fwd += g.forward_ms
+        bwd += g.backward_ms
+        plain += pl
+        print(f"[bench] step {i + 1}/{args.steps}: forward {g.forward_ms:.1f} ms, reverse {g.backward_ms:.1f} ms, "
+              f"plain tracking {pl:.1f} ms", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    K = args.steps
+    grad_s = (fwd + bwd) / 1e3
+    fpt = flops_per_turn(lat)
+    out = {
+        "metric": "particle-turns/s FP64 (1 GPU, % FP64 peak); batched adjoint/forward cost ratio",
+        "value": float(n) * T * K / grad_s,
+        "unit": "gradient particle-turns/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": (fwd + bwd) / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Gaussian beam)",
+        "config": {"workload": f"diff_dba_{n}x{T}", "particles": n, "turns": T, "elements_per_turn": len(lat),
+                   "strengths": int((g.dk != 0).sum()), "kernel": args.kernel, "ckpt_stride": args.ckpt_stride,
+                   "flops_per_particle_turn": fpt},
+        "adjoint": {"forward_ms": fwd / K, "reverse_ms": bwd / K, "plain_tracking_ms": plain / K,
+                    "gradient_over_forward": (fwd + bwd) / plain, "wall_ms_per_step": wall * 1e3 / K,
+                    "loss": g.loss, "survivors": g.n_alive},
+        "roofline": {"bound": "mfma", "pipe": "vector FP64 VALU; 4,096 particles are 64 waves on 1,024 SIMDs: "
+                     "occupancy-bound, not at the roofline",
+                     "achieved": fpt * n * T / (plain / K / 1e3) / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": fpt * n * T / (plain / K / 1e3) / FP64_PEAK, "traffic": None,
+                     "avg_launch_ms": plain / K, "launches": K},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def bench_batched(args, fast):
@@ -276,12 +365,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 2; batched 1)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 1; batched 0)")
-    ap.add_argument("--workload", default="dba", choices=["dba", "fodo", "batched"])
+    ap.add_argument("--workload", default="dba", choices=["dba", "fodo", "da", "diff", "batched"])
     ap.add_argument("--variants", type=int, default=256, help="batched: number of lattice variants")
     ap.add_argument("--turn-scale", type=float, default=1.0, help="batched: scale every T_v (1 = the workload)")
     ap.add_argument("--compare-turn-scale", type=float, default=0.05,
                     help="batched: turn scale of the comparison with per-variant launches")
-    ap.add_argument("--ckpt-stride", type=int, default=10, help="batched adjoint: turns between checkpoints")
+    ap.add_argument("--ckpt-stride", type=int, default=None,
+                    help="adjoint: turns between checkpoints (default: batched 10, diff 1)")
     ap.add_argument("--kernel", default="fast", choices=["fast", "exact"])
     ap.add_argument("--particles", type=int, default=0, help="override N (default: the workload's)")
     ap.add_argument("--turns", type=int, default=0, help="override turns per step (default: the workload's)")
@@ -294,7 +384,8 @@ def main():
     ap.add_argument("--fwd-waves", type=int, default=0, help="register budget of the tracking kernels (0, 5, 6)")
     ap.add_argument("--back-waves", type=int, default=0, help="register budget of the reverse kernels (0, 3, 4)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", default="16384x20", help="particles x turns of the CPU baseline sample")
+    ap.add_argument("--cpu-sample", default=None,
+                    help="particles x turns of the CPU baseline sample (default 16384x20; da 2048x5)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -306,8 +397,15 @@ def main():
         args.steps = 1 if batched else 2
     if args.warmup is None:
         args.warmup = 0 if batched else 1
+    if args.ckpt_stride is None:
+        args.ckpt_stride = 10 if batched else 1
+    if args.cpu_sample is None:
+        args.cpu_sample = "2048x5" if args.workload == "da" else "16384x20"
     if batched:
         bench_batched(args, args.kernel == "fast")
+        return
+    if args.workload == "diff":
+        bench_diff(args, args.kernel == "fast")
         return
 
     from trackpad_hip import HIPContext, HIPParticles, compile_table, track_

@@ -183,6 +183,36 @@ TP_DEV void horner(const St& s, const double* __restrict__ poly, int m, double& 
   }
 }
 
+// Horner sum of a fixed order M on coefficients already in registers, cf = (b0, a0, ...,
+// bM, aM): the loop of horner() unrolled, so the 2M+2 coefficients are loaded once per
+// element instead of once per kick and per order.  The parity build keeps the reference's
+// expression; the fast build forms each line with two FMAs.
+template <int M>
+TP_DEV void horner_cf(const St& s, const double* cf, double& re, double& im) {
+  double ReSum = cf[2 * M], ImSum = cf[2 * M + 1];
+#pragma unroll
+  for (int i = M - 1; i >= 0; --i) {
+#if TP_FAST
+    const double t = __builtin_fma(ReSum, s.x, __builtin_fma(-ImSum, s.y, cf[2 * i]));
+    ImSum = __builtin_fma(ImSum, s.x, __builtin_fma(ReSum, s.y, cf[2 * i + 1]));
+#else
+    const double t = ReSum * s.x - ImSum * s.y + cf[2 * i];
+    ImSum = ImSum * s.x + ReSum * s.y + cf[2 * i + 1];
+#endif
+    ReSum = t;
+  }
+  re = ReSum;
+  im = ImSum;
+}
+
+template <int M>
+TP_DEV void kick_cf(St& s, const double* cf, double K) {
+  double re, im;
+  horner_cf<M>(s, cf, re, im);
+  s.px = s.px - K * re;
+  s.py = s.py + K * im;
+}
+
 // strthinkick, src/tracking.jl:284-305
 template <int KK>
 TP_DEV void kick(St& s, const double* __restrict__ poly, int m, double K) {

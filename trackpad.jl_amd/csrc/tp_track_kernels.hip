@@ -31,11 +31,30 @@
 namespace tpk {
 namespace TPK_BUILD {  // kernel names carry the build flavour (profiles tell them apart)
 
-// Thick multipole: Yoshida-4 with strthinkick, src/tracking.jl:317-341 / :386-491
-template <bool EXH, int KK, int PPL>
+// Thick multipole: Yoshida-4 with strthinkick, src/tracking.jl:317-341 / :386-491.
+// MM > 0: general kick of order MM with its coefficients held in registers (kick_cf).
+template <bool EXH, int KK, int PPL, int MM = 0>
 TP_DEV void mpole_steps(St* s, const Cx* c, const double* __restrict__ p, int nsteps, int m) {
   const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3], L1b = p[4], L2b = p[5];
   const double* poly = p + 6;
+  if constexpr (MM > 0) {
+    double cf[2 * MM + 2];
+#pragma unroll
+    for (int i = 0; i < 2 * MM + 2; ++i) cf[i] = poly[i];
+    for (int st = 0; st < nsteps; ++st) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        drift<EXH>(s[k], c[k], L1, L1b);
+        kick_cf<MM>(s[k], cf, K1);
+        drift<EXH>(s[k], c[k], L2, L2b);
+        kick_cf<MM>(s[k], cf, K2);
+        drift<EXH>(s[k], c[k], L2, L2b);
+        kick_cf<MM>(s[k], cf, K1);
+        drift<EXH>(s[k], c[k], L1, L1b);
+      }
+    }
+    return;
+  }
 #if TP_FAST
   if constexpr (KK == TP_KICK_SEXT) {
     const double Kb1 = K1 * poly[4], Kb2 = K2 * poly[4];
@@ -89,7 +108,7 @@ TP_DEV void sbend_steps(St* s, const Cx* c, const double* __restrict__ p, int ns
 }
 
 // Exact bend body with Yoshida-4 sub-kicks, src/tracking.jl:923-953
-template <int KK, int PPL>
+template <int KK, int PPL, int MM = 0>
 TP_DEV void esbend_body(St* s, const Cx* c, const double* __restrict__ p, int nsteps, int m, bool driftlimit) {
   const double irho = p[0];
   if (driftlimit) {
@@ -138,6 +157,36 @@ TP_DEV void esbend_body(St* s, const Cx* c, const double* __restrict__ p, int ns
   const double L1 = p[1], L2 = p[2], K1 = p[3], K2 = p[4], L1b = p[5], L2b = p[6];
   const double c1 = p[7], s1 = p[8], c2 = p[9], s2 = p[10];
   const double* poly = p + 24;
+  if constexpr (MM > 0) {
+    double cf[2 * MM + 2];
+#pragma unroll
+    for (int i = 0; i < 2 * MM + 2; ++i) cf[i] = poly[i];
+    for (int st = 0; st < nsteps; ++st) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        ebody(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k]);
+        kick_cf<MM>(s[k], cf, K1);
+        d2[k] = pxyz(c[k], 0.0, s[k].py);
+#if TP_FAST
+        id2[k] = rcp_nr(d2[k]);
+#endif
+        ebody(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k]);
+        kick_cf<MM>(s[k], cf, K2);
+        d2[k] = pxyz(c[k], 0.0, s[k].py);
+#if TP_FAST
+        id2[k] = rcp_nr(d2[k]);
+#endif
+        ebody(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k]);
+        kick_cf<MM>(s[k], cf, K1);
+        d2[k] = pxyz(c[k], 0.0, s[k].py);
+#if TP_FAST
+        id2[k] = rcp_nr(d2[k]);
+#endif
+        ebody(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k]);
+      }
+    }
+    return;
+  }
   for (int st = 0; st < nsteps; ++st) {
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
@@ -199,6 +248,43 @@ TP_DEV void mat6(St& s, const double* __restrict__ M) {
 
 // Kick dispatch on the record's specialisation; TP_KICK_GEN only in kernels built with
 // FT_GENKICK (the host never sends a record a kernel variant does not contain).
+// General kicks of order 3..10 run with their coefficients in registers (MM = m); other
+// orders take the loop over the table.
+#define TP_GEN_ORDERS(CALLM)                                                              \
+  switch (m) {                                                                           \
+    case 3: CALLM(3); return;                                                            \
+    case 4: CALLM(4); return;                                                            \
+    case 5: CALLM(5); return;                                                            \
+    case 6: CALLM(6); return;                                                            \
+    case 7: CALLM(7); return;                                                            \
+    case 8: CALLM(8); return;                                                            \
+    case 9: CALLM(9); return;                                                            \
+    case 10: CALLM(10); return;                                                          \
+    default: break;                                                                      \
+  }
+
+template <bool EXH, int KK, int PPL>
+TP_DEV void mpole_dispatch(St* s, const Cx* c, const double* __restrict__ p, int nsteps, int m) {
+  if constexpr (KK == TP_KICK_GEN) {
+#define TP_CALLM(MM) mpole_steps<EXH, KK, PPL, MM>(s, c, p, nsteps, m)
+    TP_GEN_ORDERS(TP_CALLM)
+#undef TP_CALLM
+  }
+  mpole_steps<EXH, KK, PPL>(s, c, p, nsteps, m);
+}
+
+template <int KK, int PPL>
+TP_DEV void esbend_dispatch(St* s, const Cx* c, const double* __restrict__ p, int nsteps, int m, bool driftlimit) {
+  if constexpr (KK == TP_KICK_GEN) {
+    if (!driftlimit) {
+#define TP_CALLM(MM) esbend_body<KK, PPL, MM>(s, c, p, nsteps, m, false)
+      TP_GEN_ORDERS(TP_CALLM)
+#undef TP_CALLM
+    }
+  }
+  esbend_body<KK, PPL>(s, c, p, nsteps, m, driftlimit);
+}
+
 #define TP_KICK_SWITCH(FEAT, CALL)                                         \
   switch (rec.kick) {                                                      \
     case TP_KICK_NONE: CALL(TP_KICK_NONE); break;                          \
@@ -232,7 +318,7 @@ TP_DEV void apply_elem(St* s, Cx* c, const tp_rec& rec, const double* __restrict
       for (int k = 0; k < PPL; ++k) drift<EXH>(s[k], c[k], p[0], p[1]);
       break;
     case TP_MPOLE: {
-#define TP_CALL(KK) mpole_steps<EXH, KK, PPL>(s, c, p, ns, m)
+#define TP_CALL(KK) mpole_dispatch<EXH, KK, PPL>(s, c, p, ns, m)
       TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
       break;
@@ -278,7 +364,7 @@ TP_DEV void apply_elem(St* s, Cx* c, const tp_rec& rec, const double* __restrict
 #pragma unroll
           for (int k = 0; k < PPL; ++k) s[k] = bend_edge(s[k], c[k], irho, p[13], p[14], p[15], p[16]);
         }
-#define TP_CALL(KK) esbend_body<KK, PPL>(s, c, p, ns, m, dl)
+#define TP_CALL(KK) esbend_dispatch<KK, PPL>(s, c, p, ns, m, dl)
         TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
         if (!dl) {
@@ -304,7 +390,7 @@ TP_DEV void apply_elem(St* s, Cx* c, const tp_rec& rec, const double* __restrict
 #pragma unroll
           for (int k = 0; k < PPL; ++k) edge_lin(s[k], fin, p + 11, irho, false);
         }
-#define TP_CALL(KK) esbend_body<KK, PPL>(s, c, p, ns, m, dl)
+#define TP_CALL(KK) esbend_dispatch<KK, PPL>(s, c, p, ns, m, dl)
         TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
         if (fout != 0) {
