@@ -144,14 +144,16 @@ TP_DEV void bkick_back(St& s, Ad& a, const double* __restrict__ poly, int m, dou
 // pzn, its px_new is the input px and its pzn the input pz, and its path term is minus the
 // forward dasin.  The VJP at the input reuses them instead of re-evaluating the map.
 // rho = 1/irho and id2 = 1/d2 are formed by the caller once per element (d2 = pxyz(dp1, 0,
-// py) is constant through the body).
+// py) is constant through the body).  pz_io enters as pxyz(dp1, px, py) of the output and
+// leaves as that of the input, which is the output pz of the body before it when the kick
+// between them is the identity.
 // ---------------------------------------------------------------------------------------
 TP_DEV void ebody_back(St& s, Ad& a, const Cx& c, double irho, double rho, double L, double Lb, double cs, double sn,
-                       double d2, double id2) {
+                       double d2, double id2, double& pz_io) {
   const double B = c.B;
   const double py = s.py;
   const double pxn = s.px;                       // output px
-  const double pzn = pxyz(c, pxn, py);           // output pz
+  const double pzn = pz_io;                      // output pz
   const double pzmx_o = pzn - (1.0 + s.x * irho);
   const double px = pxn * cs - pzmx_o * sn;      // input px
   const double pz = pxyz(c, px, py);             // input pz
@@ -211,6 +213,7 @@ TP_DEV void ebody_back(St& s, Ad& a, const Cx& c, double irho, double rho, doubl
 #endif
   s.x = x_in;
   s.px = px;
+  pz_io = pz;
 }
 
 // ---------------------------------------------------------------------------------------

@@ -132,6 +132,7 @@ TP_DEV void drift(St& s, const Cx& c, double L, double Lb) {
 #if TP_FAST
     // NaN when pz2 <= 0 (rsqrt_nr), so x, y, z become NaN and the particle is lost at
     // this element's exit with its entry coordinates, as after the reference's NaN map.
+    // (Yoshida steps use drift_xy and sum the path length per step.)
     // The path-length change NormL B - L beti is formed with one rounding.
     const double NormL = L * rsqrt_nr(pz2);
     s.x = s.x + NormL * s.px;
@@ -151,6 +152,21 @@ TP_DEV void drift(St& s, const Cx& c, double L, double Lb) {
     s.z = s.z + NormL * (s.px * s.px + s.py * s.py) / (2.0 * c.C);
   }
 }
+
+#if TP_FAST
+// drift6 (exact Hamiltonian) without its path-length update: returns NormL, NaN for
+// pz2 <= 0 (x and y become NaN).  A Yoshida step adds the four drifts' path lengths at once,
+// z += (beti + delta) sum(NormL) - sum(L beti), one FMA per step instead of two operations
+// per drift; the one rounding of the step's sum is below the reference's own rounding of
+// z + NormL (beti + delta) at every drift.
+TP_DEV double drift_xy(St& s, const Cx& c, double L) {
+  const double pz2 = c.A - s.px * s.px - s.py * s.py;
+  const double NormL = L * rsqrt_nr(pz2);
+  s.x = s.x + NormL * s.px;
+  s.y = s.y + NormL * s.py;
+  return NormL;
+}
+#endif
 
 // ---------------------------------------------------------------------------------------
 // Horner sums of strthinkick / bndthinkick, src/tracking.jl:251-259, 290-298.
@@ -276,8 +292,9 @@ TP_DEV void edge_lin(St& s, int method, const double* __restrict__ blk, double i
 TP_DEV double pxyz(const Cx& c, double px, double py) {
   double v = c.BB - px * px - py * py;
 #if TP_FAST
-  const double r = sqrt_nr(v);
-  return v > 0.0 ? r : 0.0;
+  // sqrt_nr is NaN for v <= 0 (and for NaN v), and fmax returns its other operand for a
+  // NaN: 0, as the reference's test v > 0 gives (one v_max instead of a compare and selects)
+  return __builtin_fmax(sqrt_nr(v), 0.0);
 #else
   return v > 0.0 ? sqrt(v) : 0.0;
 #endif
@@ -372,10 +389,13 @@ __device__ __noinline__ St bend_edge(St s, Cx c, double rhoinv, double co, doubl
 // exact_bend_body (Forest 12.18), src/tracking.jl:858-894, for |irho| >= 1e-6.
 // d2 = pxyz(dp1, 0, py) is passed in: py is unchanged by the body, and by the kicks
 // between bodies when the kick is the identity, so the caller may hoist it.
+// pz enters as pxyz(dp1, px, py) of the input and leaves as that of the output (pzn): the
+// next body of a Yoshida step with identity kicks starts from exactly these arguments, so
+// the caller passes it on instead of evaluating pxyz again (the same value bit for bit).
 // rho = 1/irho and id2 = 1/d2 are used by the fast build only.
 TP_DEV void ebody(St& s, const Cx& c, double irho, double rho, double L, double Lb, double cs, double sn,
-                  double d2, double id2) {
-  const double pz = pxyz(c, s.px, s.py);
+                  double d2, double id2, double& pz_io) {
+  const double pz = pz_io;
   const double pzmx = pz - (1.0 + s.x * irho);
   const double px_new = s.px * cs + pzmx * sn;
   const double pzn = pxyz(c, px_new, s.py);
@@ -410,6 +430,7 @@ TP_DEV void ebody(St& s, const Cx& c, double irho, double rho, double L, double 
 #endif
   s.x = x_new;
   s.px = px_new;
+  pz_io = pzn;
 }
 
 // Drift limit of exact_bend_body (|irho| < 1e-6), src/tracking.jl:862-872

@@ -37,6 +37,45 @@ template <bool EXH, int KK, int PPL, int MM = 0>
 TP_DEV void mpole_steps(St* s, const Cx* c, const double* __restrict__ p, int nsteps, int m) {
   const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3], L1b = p[4], L2b = p[5];
   const double* poly = p + 6;
+#if TP_FAST
+  if constexpr (EXH) {
+    // path length summed per Yoshida step (drift_xy)
+    const double Lsb = 2.0 * L1b + 2.0 * L2b;
+#define TP_YSTEP(k, KICK1, KICK2)                                 \
+  {                                                               \
+    double S_ = drift_xy(s[k], c[k], L1);                         \
+    KICK1;                                                        \
+    S_ = S_ + drift_xy(s[k], c[k], L2);                           \
+    KICK2;                                                        \
+    S_ = S_ + drift_xy(s[k], c[k], L2);                           \
+    KICK1;                                                        \
+    S_ = S_ + drift_xy(s[k], c[k], L1);                           \
+    s[k].z = s[k].z + __builtin_fma(S_, c[k].B, -Lsb);            \
+  }
+    if constexpr (MM > 0) {
+      double cf[2 * MM + 2];
+#pragma unroll
+      for (int i = 0; i < 2 * MM + 2; ++i) cf[i] = poly[i];
+      for (int st = 0; st < nsteps; ++st) {
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) TP_YSTEP(k, kick_cf<MM>(s[k], cf, K1), kick_cf<MM>(s[k], cf, K2))
+      }
+    } else if constexpr (KK == TP_KICK_SEXT) {
+      const double Kb1 = K1 * poly[4], Kb2 = K2 * poly[4];
+      for (int st = 0; st < nsteps; ++st) {
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) TP_YSTEP(k, kick_sext_kb(s[k], Kb1), kick_sext_kb(s[k], Kb2))
+      }
+    } else {
+      for (int st = 0; st < nsteps; ++st) {
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) TP_YSTEP(k, kick<KK>(s[k], poly, m, K1), kick<KK>(s[k], poly, m, K2))
+      }
+    }
+#undef TP_YSTEP
+    return;
+  }
+#endif
   if constexpr (MM > 0) {
     double cf[2 * MM + 2];
 #pragma unroll
@@ -55,24 +94,6 @@ TP_DEV void mpole_steps(St* s, const Cx* c, const double* __restrict__ p, int ns
     }
     return;
   }
-#if TP_FAST
-  if constexpr (KK == TP_KICK_SEXT) {
-    const double Kb1 = K1 * poly[4], Kb2 = K2 * poly[4];
-    for (int st = 0; st < nsteps; ++st) {
-#pragma unroll
-      for (int k = 0; k < PPL; ++k) {
-        drift<EXH>(s[k], c[k], L1, L1b);
-        kick_sext_kb(s[k], Kb1);
-        drift<EXH>(s[k], c[k], L2, L2b);
-        kick_sext_kb(s[k], Kb2);
-        drift<EXH>(s[k], c[k], L2, L2b);
-        kick_sext_kb(s[k], Kb1);
-        drift<EXH>(s[k], c[k], L1, L1b);
-      }
-    }
-    return;
-  }
-#endif
   for (int st = 0; st < nsteps; ++st) {
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
@@ -139,10 +160,13 @@ TP_DEV void esbend_body(St* s, const Cx* c, const double* __restrict__ p, int ns
   const double rho = 0.0;
 #endif
   // py is constant through the body and through identity kicks: hoist d2 = pxyz(dp1,0,py).
-  double d2[PPL], id2[PPL];
+  // pz = pxyz(dp1, px, py) is carried from one body to the next (ebody) and evaluated again
+  // only after a kick that moves px or py.
+  double d2[PPL], id2[PPL], pz[PPL];
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
     d2[k] = pxyz(c[k], 0.0, s[k].py);
+    pz[k] = pxyz(c[k], s[k].px, s[k].py);
 #if TP_FAST
     id2[k] = rcp_nr(d2[k]);
 #else
@@ -151,12 +175,22 @@ TP_DEV void esbend_body(St* s, const Cx* c, const double* __restrict__ p, int ns
   }
   if (nsteps == 0) {
 #pragma unroll
-    for (int k = 0; k < PPL; ++k) ebody(s[k], c[k], irho, rho, p[1], p[5], p[7], p[8], d2[k], id2[k]);
+    for (int k = 0; k < PPL; ++k) ebody(s[k], c[k], irho, rho, p[1], p[5], p[7], p[8], d2[k], id2[k], pz[k]);
     return;
   }
   const double L1 = p[1], L2 = p[2], K1 = p[3], K2 = p[4], L1b = p[5], L2b = p[6];
   const double c1 = p[7], s1 = p[8], c2 = p[9], s2 = p[10];
   const double* poly = p + 24;
+#if TP_FAST
+#define TP_AFTER_KICK(k)                        \
+  d2[k] = pxyz(c[k], 0.0, s[k].py);             \
+  id2[k] = rcp_nr(d2[k]);                       \
+  pz[k] = pxyz(c[k], s[k].px, s[k].py)
+#else
+#define TP_AFTER_KICK(k)                        \
+  d2[k] = pxyz(c[k], 0.0, s[k].py);             \
+  pz[k] = pxyz(c[k], s[k].px, s[k].py)
+#endif
   if constexpr (MM > 0) {
     double cf[2 * MM + 2];
 #pragma unroll
@@ -164,25 +198,16 @@ TP_DEV void esbend_body(St* s, const Cx* c, const double* __restrict__ p, int ns
     for (int st = 0; st < nsteps; ++st) {
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
-        ebody(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k]);
+        ebody(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k], pz[k]);
         kick_cf<MM>(s[k], cf, K1);
-        d2[k] = pxyz(c[k], 0.0, s[k].py);
-#if TP_FAST
-        id2[k] = rcp_nr(d2[k]);
-#endif
-        ebody(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k]);
+        TP_AFTER_KICK(k);
+        ebody(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k], pz[k]);
         kick_cf<MM>(s[k], cf, K2);
-        d2[k] = pxyz(c[k], 0.0, s[k].py);
-#if TP_FAST
-        id2[k] = rcp_nr(d2[k]);
-#endif
-        ebody(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k]);
+        TP_AFTER_KICK(k);
+        ebody(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k], pz[k]);
         kick_cf<MM>(s[k], cf, K1);
-        d2[k] = pxyz(c[k], 0.0, s[k].py);
-#if TP_FAST
-        id2[k] = rcp_nr(d2[k]);
-#endif
-        ebody(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k]);
+        TP_AFTER_KICK(k);
+        ebody(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k], pz[k]);
       }
     }
     return;
@@ -190,33 +215,25 @@ TP_DEV void esbend_body(St* s, const Cx* c, const double* __restrict__ p, int ns
   for (int st = 0; st < nsteps; ++st) {
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-      ebody(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k]);
+      ebody(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k], pz[k]);
       kick<KK>(s[k], poly, m, K1);
       if (KK != TP_KICK_NONE) {
-        d2[k] = pxyz(c[k], 0.0, s[k].py);
-#if TP_FAST
-        id2[k] = rcp_nr(d2[k]);
-#endif
+        TP_AFTER_KICK(k);
       }
-      ebody(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k]);
+      ebody(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k], pz[k]);
       kick<KK>(s[k], poly, m, K2);
       if (KK != TP_KICK_NONE) {
-        d2[k] = pxyz(c[k], 0.0, s[k].py);
-#if TP_FAST
-        id2[k] = rcp_nr(d2[k]);
-#endif
+        TP_AFTER_KICK(k);
       }
-      ebody(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k]);
+      ebody(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k], pz[k]);
       kick<KK>(s[k], poly, m, K1);
       if (KK != TP_KICK_NONE) {
-        d2[k] = pxyz(c[k], 0.0, s[k].py);
-#if TP_FAST
-        id2[k] = rcp_nr(d2[k]);
-#endif
+        TP_AFTER_KICK(k);
       }
-      ebody(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k]);
+      ebody(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k], pz[k]);
     }
   }
+#undef TP_AFTER_KICK
 }
 
 // Misalignment, src/tracking.jl:60-67 and the r1/r2 products of every magnet pass!.
