@@ -392,9 +392,13 @@ __device__ __noinline__ St bend_edge(St s, Cx c, double rhoinv, double co, doubl
 // pz enters as pxyz(dp1, px, py) of the input and leaves as that of the output (pzn): the
 // next body of a Yoshida step with identity kicks starts from exactly these arguments, so
 // the caller passes it on instead of evaluating pxyz again (the same value bit for bit).
-// rho = 1/irho and id2 = 1/d2 are used by the fast build only.
-TP_DEV void ebody(St& s, const Cx& c, double irho, double rho, double L, double Lb, double cs, double sn,
-                  double d2, double id2, double& pz_io) {
+// rho = 1/irho and id2 = 1/d2 are used by the fast build only.  Returns the path term
+// dasin; with YZ false the body leaves y and z to the caller (the fast build sums dasin over
+// a Yoshida step of identity kicks, where py is constant: y += py sum(dasin) and
+// z += (beti + delta) sum(dasin) - sum(L beti)).
+template <bool YZ = true>
+TP_DEV double ebody(St& s, const Cx& c, double irho, double rho, double L, double Lb, double cs, double sn,
+                    double d2, double id2, double& pz_io) {
   const double pz = pz_io;
   const double pzmx = pz - (1.0 + s.x * irho);
   const double px_new = s.px * cs + pzmx * sn;
@@ -422,15 +426,18 @@ TP_DEV void ebody(St& s, const Cx& c, double irho, double rho, double L, double 
   (void)rho;
   (void)id2;
 #endif
-  s.y = s.y + s.py * dasin;
+  if constexpr (YZ) {
+    s.y = s.y + s.py * dasin;
 #if TP_FAST
-  s.z = s.z + __builtin_fma(c.B, dasin, -Lb);
+    s.z = s.z + __builtin_fma(c.B, dasin, -Lb);
 #else
-  s.z = s.z + c.B * dasin - Lb;
+    s.z = s.z + c.B * dasin - Lb;
 #endif
+  }
   s.x = x_new;
   s.px = px_new;
   pz_io = pzn;
+  return dasin;
 }
 
 // Drift limit of exact_bend_body (|irho| < 1e-6), src/tracking.jl:862-872

@@ -191,6 +191,25 @@ TP_DEV void esbend_body(St* s, const Cx* c, const double* __restrict__ p, int ns
   d2[k] = pxyz(c[k], 0.0, s[k].py);             \
   pz[k] = pxyz(c[k], s[k].px, s[k].py)
 #endif
+#if TP_FAST
+  if constexpr (KK == TP_KICK_NONE) {
+    // identity kicks: py and delta stay fixed through the step, so y and z take the step's
+    // summed path term once (ebody<false>)
+    const double Lsb = 2.0 * L1b + 2.0 * L2b;
+    for (int st = 0; st < nsteps; ++st) {
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        double D = ebody<false>(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k], pz[k]);
+        D = D + ebody<false>(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k], pz[k]);
+        D = D + ebody<false>(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k], pz[k]);
+        D = D + ebody<false>(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k], pz[k]);
+        s[k].y = s[k].y + s[k].py * D;
+        s[k].z = s[k].z + __builtin_fma(c[k].B, D, -Lsb);
+      }
+    }
+    return;
+  }
+#endif
   if constexpr (MM > 0) {
     double cf[2 * MM + 2];
 #pragma unroll
