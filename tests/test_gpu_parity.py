@@ -166,6 +166,34 @@ def test_forest_exact_bend_ring(ctx):
     assert ok, err
 
 
+def test_exact_bend_large_angles(ctx):
+    """Exact bends at large sub-step angles and near-grazing momenta, where the fast build's
+    single-arcsine form of the body does not apply and it takes the reference's two clamped
+    arcsines: one pass, both builds, loss records identical with the oracle, coordinates
+    within 1e-12 (absolute; the body's x is a difference of O(1) terms over irho)."""
+    from trackpad_hip import ExactSBend, ExactSBendLinearEdge
+    beam = Beam(W.ENERGY)
+    c = np.zeros((8, 6))
+    c[0] = [1e-3, 0.3, 0.0, 0.1, 0.0, 0.0]
+    c[1] = [-2e-3, -0.6, 1e-3, 0.0, 0.0, 0.01]
+    c[2] = [0.0, 0.85, 0.0, 0.0, 0.0, -0.02]
+    c[3] = [0.0, 0.999, 0.0, 0.0, 0.0, 0.0]     # pz + pzn above d2/16 at small body angles
+    c[4] = [0.0, 0.9999, 0.0, 0.0, 0.0, 0.0]    # below it: the two-arcsine form
+    c[5] = [0.0, -0.9999, 0.0, 0.01, 0.0, 0.0]
+    c[6] = [1e-4, 2e-4, -1e-4, 1e-4, 0.0, 1e-3]
+    c[7] = [0.0, 0.5, 0.0, 0.5, 0.0, 0.0]
+    for lat in (Lattice([ExactSBendLinearEdge(0.5, 0.8, 0.1, 0.1, num_int_steps=1)]),
+                Lattice([ExactSBendLinearEdge(2.0, 0.02, num_int_steps=10)]),
+                Lattice([ExactSBend(1.0, 0.6, 0.2, 0.1)])):
+        oc, olf, olt, ole = O.ringpass(c, lat, beam, np.zeros(len(c), np.int64), 1)
+        for fast in (False, True):
+            gc, glf, glt, gle = gpu_track(ctx, lat, beam, c, 1, fast=fast)
+            assert np.array_equal(glf, olf) and np.array_equal(gle, ole), (lat.elements[0], fast)
+            alive = olf == 0
+            assert np.max(np.abs(gc - oc)[alive]) <= 1e-12, (lat.elements[0], fast, np.abs(gc - oc).max(axis=0))
+            np.testing.assert_array_equal(gc[~alive], oc[~alive])
+
+
 def test_da_grid_orders_to_10(ctx):
     """DA ring (order-10 normal/skew errors on every magnet) on a 16 x 16 x 3 grid, 10
     turns: loss records identical away from the boundary, survivors within 1e-9."""

@@ -159,17 +159,9 @@ TP_DEV void ebody_back(St& s, Ad& a, const Cx& c, double irho, double rho, doubl
   const double pz = pxyz(c, px, py);             // input pz
 #if TP_FAST
   // asin(px/d2) - asin(pxn/d2) as one series asin, as in the forward body
-  const double a1 = px * id2, a2 = pxn * id2;
-  double dif;
-  if (fabs(a1) < 0.5 && fabs(a2) < 0.5) {
-    dif = asin_f((px * pzn - pxn * pz) * (id2 * id2));
-  } else {
-    dif = asin_s(clamp1(a1)) - asin_s(clamp1(a2));
-  }
-  const double dasin = L + dif * rho;
+  const double dasin = L + asin_diff(px, pz, pxn, pzn, d2, id2) * rho;
   const double x_in = (pz - pzmx_o * cs - pxn * sn - 1.0) * rho;
   const double ipz = rcp_nr(pz), ipzn = rcp_nr(pzn);
-  (void)d2;
 #else
   const double dasin = L + (asin_s(clamp1(px / d2)) - asin_s(clamp1(pxn / d2))) / irho;
   const double x_in = (pz - pzmx_o * cs - pxn * sn - 1.0) / irho;

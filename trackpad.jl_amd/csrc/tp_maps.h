@@ -104,19 +104,26 @@ TP_DEV double asin_s(double u) {
   return r;
 }
 
+TP_DEV double clamp1(double v) { return v > 1.0 ? 1.0 : (v < -1.0 ? -1.0 : v); }
+
 #if TP_FAST
-// asin(u) for |u| <= 1/64 through the u^9 term of the same series (the first dropped term,
-// 63/2816 u^11, is below 2^-65 |u|); other arguments take asin_s.  The exact-bend sub-steps
-// of typical rings turn by well under 1/64 rad.
-TP_DEV double asin_f(double u) {
-  if (__builtin_expect(fabs(u) <= 0.015625, 1)) {
+// asin(a) - asin(b) of the exact bend body, a = px/d2, b = pxn/d2, whose cosines are
+// sqrt(1 - a^2) = pz/d2 and sqrt(1 - b^2) = pzn/d2 (pz, pzn are the body's pxyz values):
+// the difference has the sine u = (px pzn - pxn pz)/d2^2.  When |u| <= 1/64 and
+// pz + pzn > d2/16 the difference is asin(u) itself -- the other root, near +-pi, needs
+// both cosines below sin(asin(1/64)) = 1/64 -- and the series through u^9 gives it (first
+// dropped term 63/2816 u^11 < 2^-65 |u|).  Otherwise (large angles, clamped arguments) the
+// reference's two clamped arcsines.
+TP_DEV double asin_diff(double px, double pz, double pxn, double pzn, double d2, double id2) {
+  const double u = (px * pzn - pxn * pz) * (id2 * id2);
+  if (__builtin_expect(fabs(u) <= 0.015625 && pz + pzn > 0.0625 * d2, 1)) {
     const double w = u * u;
     double p = __builtin_fma(w, 35.0 / 1152.0, 5.0 / 112.0);
     p = __builtin_fma(p, w, 3.0 / 40.0);
     p = __builtin_fma(p, w, 1.0 / 6.0);
     return __builtin_fma(u * w, p, u);
   }
-  return asin_s(u);
+  return asin_s(clamp1(px * id2)) - asin_s(clamp1(pxn * id2));
 }
 #endif
 
@@ -300,7 +307,6 @@ TP_DEV double pxyz(const Cx& c, double px, double py) {
 #endif
 }
 
-TP_DEV double clamp1(double v) { return v > 1.0 ? 1.0 : (v < -1.0 ? -1.0 : v); }
 
 // yrot, src/tracking.jl:727-745 (caller skips phi == 0)
 __device__ __noinline__ St yrot(St s, Cx c, double co, double si) {
@@ -404,20 +410,10 @@ TP_DEV double ebody(St& s, const Cx& c, double irho, double rho, double L, doubl
   const double px_new = s.px * cs + pzmx * sn;
   const double pzn = pxyz(c, px_new, s.py);
 #if TP_FAST
-  // asin(a) - asin(b) = asin(a sqrt(1-b^2) - b sqrt(1-a^2)) with sqrt(1 - (px/d2)^2) = pz/d2:
-  // one series asin of (px*pzn - px_new*pz)/d2^2 instead of two library calls; the
-  // clamp of the reference is kept by falling back to the two-call form when either
-  // argument leaves (-1, 1).
-  const double a1 = s.px * id2, a2 = px_new * id2;
-  double dif;
-  if (fabs(a1) < 0.5 && fabs(a2) < 0.5) {
-    dif = asin_f((s.px * pzn - px_new * pz) * (id2 * id2));
-  } else {
-    dif = asin_s(clamp1(a1)) - asin_s(clamp1(a2));
-  }
+  // one series arcsine of the difference instead of two (asin_diff)
+  const double dif = asin_diff(s.px, pz, px_new, pzn, d2, id2);
   const double dasin = L + dif * rho;
   const double x_new = (pzn - pzmx * cs + s.px * sn - 1.0) * rho;
-  (void)d2;
 #else
   const double val1 = clamp1(s.px / d2);
   const double val2 = clamp1(px_new / d2);
