@@ -138,65 +138,94 @@ TP_DEV void bkick_back(St& s, Ad& a, const double* __restrict__ poly, int m, dou
 }
 
 // ---------------------------------------------------------------------------------------
+// pxyz(dp1, px, py) and its reciprocal (src/tracking.jl:717-720).  The fast build takes
+// both from one refined hardware estimate: y0 = rsq(v), e = 1 - v y0^2, then 1/sqrt(v) =
+// y0 (1 + e/2 + 3e^2/8) and sqrt(v) = v y0 (1 + e/2 + 3e^2/8), as rsqrt_nr / sqrt_nr.
+// ---------------------------------------------------------------------------------------
+TP_DEV double pxyz_r(const Cx& c, double px, double py, double& ipz) {
+  const double v = c.BB - px * px - py * py;
+#if TP_FAST
+  const double y = __builtin_amdgcn_rsq(v);
+  const double t = v * y;
+  const double e = __builtin_fma(-t, y, 1.0);
+  const double p = __builtin_fma(e, 0.375, 0.5);
+  ipz = __builtin_fma(y * e, p, y);
+  return __builtin_fmax(__builtin_fma(t * e, p, t), 0.0);
+#else
+  const double pz = v > 0.0 ? sqrt(v) : 0.0;
+  ipz = 1.0 / pz;
+  return pz;
+#endif
+}
+
+// ---------------------------------------------------------------------------------------
 // Exact sector-bend body (Forest 12.18, src/tracking.jl:858-894) backwards.  s enters as
 // the body's output and leaves as its input.  The inverse is the body of length -L (sn ->
 // -sn); its intermediates are the forward map's: the inverse's pz is the forward output's
 // pzn, its px_new is the input px and its pzn the input pz, and its path term is minus the
-// forward dasin.  The VJP at the input reuses them instead of re-evaluating the map.
+// forward dasin.  The VJP is then the reverse sweep over the forward body's operations
+//   pz = sqrt(B^2 - px^2 - py^2), pzmx = pz - 1 - x irho, pxn = px cs + pzmx sn,
+//   pzn = sqrt(B^2 - pxn^2 - py^2), d2 = sqrt(B^2 - py^2),
+//   dasin = L + (asin(px/d2) - asin(pxn/d2)) / irho  (d asin(px/d2)/d px = 1/pz),
+//   x' = (pzn - pzmx cs + px sn - 1) / irho, y' = y + py dasin, z' = z + B dasin - L beti,
+// B = beti + delta, each adjoint accumulated as its operation is undone.
 // rho = 1/irho and id2 = 1/d2 are formed by the caller once per element (d2 = pxyz(dp1, 0,
-// py) is constant through the body).  pz_io enters as pxyz(dp1, px, py) of the output and
-// leaves as that of the input, which is the output pz of the body before it when the kick
-// between them is the identity.
+// py) is constant through the body).  pz_io / ipz_io enter as pxyz(dp1, px, py) of the
+// output and its reciprocal and leave as those of the input, which are the output's of the
+// body before it when the kick between them is the identity.
 // ---------------------------------------------------------------------------------------
 TP_DEV void ebody_back(St& s, Ad& a, const Cx& c, double irho, double rho, double L, double Lb, double cs, double sn,
-                       double d2, double id2, double& pz_io) {
+                       double d2, double id2, double& pz_io, double& ipz_io) {
   const double B = c.B;
   const double py = s.py;
-  const double pxn = s.px;                       // output px
-  const double pzn = pz_io;                      // output pz
+  const double pxn = s.px;                        // output px
+  const double pzn = pz_io, ipzn = ipz_io;        // output pz and 1/pz
   const double pzmx_o = pzn - (1.0 + s.x * irho);
-  const double px = pxn * cs - pzmx_o * sn;      // input px
-  const double pz = pxyz(c, px, py);             // input pz
+  const double px = pxn * cs - pzmx_o * sn;       // input px
+  double ipz;
+  const double pz = pxyz_r(c, px, py, ipz);       // input pz and 1/pz
 #if TP_FAST
   // asin(px/d2) - asin(pxn/d2) as one series asin, as in the forward body
   const double dasin = L + asin_diff(px, pz, pxn, pzn, d2, id2) * rho;
   const double x_in = (pz - pzmx_o * cs - pxn * sn - 1.0) * rho;
-  const double ipz = rcp_nr(pz), ipzn = rcp_nr(pzn);
 #else
   const double dasin = L + (asin_s(clamp1(px / d2)) - asin_s(clamp1(pxn / d2))) / irho;
   const double x_in = (pz - pzmx_o * cs - pxn * sn - 1.0) / irho;
-  const double ipz = 1.0 / pz, ipzn = 1.0 / pzn;
-  (void)id2;
 #endif
-  const double iirho = rho;
-  // partials of the forward map at the input with respect to (x, px, py, delta); y and z
-  // enter the outputs with coefficient 1
-  const double dpz_px = -px * ipz, dpz_py = -py * ipz, dpz_d = B * ipz;
-  const double dpxn_x = -irho * sn, dpxn_px = cs + sn * dpz_px, dpxn_py = sn * dpz_py, dpxn_d = sn * dpz_d;
-  const double dpzn_x = -pxn * dpxn_x * ipzn, dpzn_px = -pxn * dpxn_px * ipzn;
-  const double dpzn_py = (-pxn * dpxn_py - py) * ipzn, dpzn_d = (B - pxn * dpxn_d) * ipzn;
-  // x' irho = pzn - pzmx cs + px sn - 1, with d pzmx = (-irho, dpz_px, dpz_py, dpz_d)
-  const double dx_x = (dpzn_x + cs * irho) * iirho;
-  const double dx_px = (dpzn_px - cs * dpz_px + sn) * iirho;
-  const double dx_py = (dpzn_py - cs * dpz_py) * iirho;
-  const double dx_d = (dpzn_d - cs * dpz_d) * iirho;
-  // dasin = L + (asin(px/d2) - asin(pxn/d2)) / irho, d asin(u) = du / sqrt(1-u^2),
-  // sqrt(1 - (px/d2)^2) = pz/d2 and sqrt(1 - (pxn/d2)^2) = pzn/d2; d2 = sqrt(B^2 - py^2)
-  const double dd2_py = -py * id2, dd2_d = B * id2;
-  const double da_x = (-(dpxn_x * ipzn)) * iirho;
-  const double da_px = (ipz - dpxn_px * ipzn) * iirho;
-  const double da_py = ((-px * dd2_py * id2) * ipz - (dpxn_py - pxn * dd2_py * id2) * ipzn) * iirho;
-  const double da_d = ((-px * dd2_d * id2) * ipz - (dpxn_d - pxn * dd2_d * id2) * ipzn) * iirho;
-  // outputs: x', px', y' = y + py dasin, z' = z + B dasin - L beti
-  const double ayz = a.y * py + a.z * B;
-  const double ax = a.x * dx_x + a.px * dpxn_x + ayz * da_x;
-  const double apx = a.x * dx_px + a.px * dpxn_px + ayz * da_px;
-  const double apy = a.py + a.x * dx_py + a.px * dpxn_py + ayz * da_py + a.y * dasin;
-  const double ad = a.d + a.x * dx_d + a.px * dpxn_d + ayz * da_d + a.z * dasin;
-  a.x = ax;
-  a.px = apx;
-  a.py = apy;
-  a.d = ad;
+  // reverse sweep; y and z pass through with coefficient 1
+  const double a_dasin = a.y * py + a.z * B;
+  double a_py = a.py + a.y * dasin;
+  double a_B = a.z * dasin;
+  const double g = a.x * rho;                     // x' = (...) rho
+  const double a_pzn0 = g;
+  double a_pzmx = -g * cs;
+  double a_px = g * sn;
+  const double h = a_dasin * rho;                 // dasin = L + dif rho
+  a_px = a_px + h * ipz;
+  double a_pxn = a.px - h * ipzn;
+  // d2 enters dif through both arguments: d(dif)/d(d2) = (pxn/pzn - px/pz) / d2
+  const double a_d2 = h * (pxn * ipzn - px * ipz) * id2;
+  a_py = a_py - a_d2 * py * id2;
+  a_B = a_B + a_d2 * B * id2;
+  // pzn = sqrt(B^2 - pxn^2 - py^2)
+  const double tn = a_pzn0 * ipzn;
+  a_pxn = a_pxn - tn * pxn;
+  a_py = a_py - tn * py;
+  a_B = a_B + tn * B;
+  // pxn = px cs + pzmx sn
+  a_px = a_px + a_pxn * cs;
+  a_pzmx = a_pzmx + a_pxn * sn;
+  // pzmx = pz - 1 - x irho
+  const double a_x = -a_pzmx * irho;
+  // pz = sqrt(B^2 - px^2 - py^2)
+  const double t0 = a_pzmx * ipz;
+  a_px = a_px - t0 * px;
+  a_py = a_py - t0 * py;
+  a_B = a_B + t0 * B;
+  a.x = a_x;
+  a.px = a_px;
+  a.py = a_py;
+  a.d = a.d + a_B;
   s.y = s.y - py * dasin;
 #if TP_FAST
   s.z = s.z - __builtin_fma(B, dasin, -Lb);
@@ -206,6 +235,7 @@ TP_DEV void ebody_back(St& s, Ad& a, const Cx& c, double irho, double rho, doubl
   s.x = x_in;
   s.px = px;
   pz_io = pz;
+  ipz_io = ipz;
 }
 
 // ---------------------------------------------------------------------------------------

@@ -82,37 +82,38 @@ TP_DEV void esbend_body_back(St& s, Ad& a, const Cx& c, const double* __restrict
 #endif
   double d2 = pxyz(c, 0.0, s.py);
   double id2 = TP_ID2(d2);
-  double pz = pxyz(c, s.px, s.py);
+  double ipz;
+  double pz = pxyz_r(c, s.px, s.py, ipz);
   if (ns == 0) {
-    ebody_back(s, a, c, irho, rho, p[1], p[5], p[7], p[8], d2, id2, pz);
+    ebody_back(s, a, c, irho, rho, p[1], p[5], p[7], p[8], d2, id2, pz, ipz);
     return;
   }
   const double L1 = p[1], L2 = p[2], K1 = p[3], K2 = p[4], L1b = p[5], L2b = p[6];
   const double c1 = p[7], s1 = p[8], c2 = p[9], s2 = p[10];
   const double* poly = p + 24;
   for (int st = 0; st < ns; ++st) {
-    ebody_back(s, a, c, irho, rho, L1, L1b, c1, s1, d2, id2, pz);
+    ebody_back(s, a, c, irho, rho, L1, L1b, c1, s1, d2, id2, pz, ipz);
     kick_back<KK>(s, a, poly, m, K1, -1, gdummy);
     if (KK != TP_KICK_NONE) {
       d2 = pxyz(c, 0.0, s.py);
       id2 = TP_ID2(d2);
-      pz = pxyz(c, s.px, s.py);
+      pz = pxyz_r(c, s.px, s.py, ipz);
     }
-    ebody_back(s, a, c, irho, rho, L2, L2b, c2, s2, d2, id2, pz);
+    ebody_back(s, a, c, irho, rho, L2, L2b, c2, s2, d2, id2, pz, ipz);
     kick_back<KK>(s, a, poly, m, K2, -1, gdummy);
     if (KK != TP_KICK_NONE) {
       d2 = pxyz(c, 0.0, s.py);
       id2 = TP_ID2(d2);
-      pz = pxyz(c, s.px, s.py);
+      pz = pxyz_r(c, s.px, s.py, ipz);
     }
-    ebody_back(s, a, c, irho, rho, L2, L2b, c2, s2, d2, id2, pz);
+    ebody_back(s, a, c, irho, rho, L2, L2b, c2, s2, d2, id2, pz, ipz);
     kick_back<KK>(s, a, poly, m, K1, -1, gdummy);
     if (KK != TP_KICK_NONE) {
       d2 = pxyz(c, 0.0, s.py);
       id2 = TP_ID2(d2);
-      pz = pxyz(c, s.px, s.py);
+      pz = pxyz_r(c, s.px, s.py, ipz);
     }
-    ebody_back(s, a, c, irho, rho, L1, L1b, c1, s1, d2, id2, pz);
+    ebody_back(s, a, c, irho, rho, L1, L1b, c1, s1, d2, id2, pz, ipz);
   }
 #undef TP_ID2
 }
