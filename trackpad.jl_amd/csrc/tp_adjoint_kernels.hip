@@ -91,29 +91,23 @@ TP_DEV void esbend_body_back(St& s, Ad& a, const Cx& c, const double* __restrict
   const double L1 = p[1], L2 = p[2], K1 = p[3], K2 = p[4], L1b = p[5], L2b = p[6];
   const double c1 = p[7], s1 = p[8], c2 = p[9], s2 = p[10];
   const double* poly = p + 24;
+  // The four bodies of a step (lengths L1 L2 L2 L1, a palindrome) one at a time: a rolled
+  // loop keeps one body's reverse sweep live instead of four interleaved ones.
   for (int st = 0; st < ns; ++st) {
-    ebody_back(s, a, c, irho, rho, L1, L1b, c1, s1, d2, id2, pz, ipz);
-    kick_back<KK>(s, a, poly, m, K1, -1, gdummy);
-    if (KK != TP_KICK_NONE) {
-      d2 = pxyz(c, 0.0, s.py);
-      id2 = TP_ID2(d2);
-      pz = pxyz_r(c, s.px, s.py, ipz);
+#pragma nounroll
+    for (int j = 0; j < 4; ++j) {
+      const bool outer = (j == 0) || (j == 3);
+      ebody_back(s, a, c, irho, rho, outer ? L1 : L2, outer ? L1b : L2b, outer ? c1 : c2, outer ? s1 : s2, d2, id2,
+                 pz, ipz);
+      if (j < 3) {
+        kick_back<KK>(s, a, poly, m, j == 1 ? K2 : K1, -1, gdummy);
+        if (KK != TP_KICK_NONE) {
+          d2 = pxyz(c, 0.0, s.py);
+          id2 = TP_ID2(d2);
+          pz = pxyz_r(c, s.px, s.py, ipz);
+        }
+      }
     }
-    ebody_back(s, a, c, irho, rho, L2, L2b, c2, s2, d2, id2, pz, ipz);
-    kick_back<KK>(s, a, poly, m, K2, -1, gdummy);
-    if (KK != TP_KICK_NONE) {
-      d2 = pxyz(c, 0.0, s.py);
-      id2 = TP_ID2(d2);
-      pz = pxyz_r(c, s.px, s.py, ipz);
-    }
-    ebody_back(s, a, c, irho, rho, L2, L2b, c2, s2, d2, id2, pz, ipz);
-    kick_back<KK>(s, a, poly, m, K1, -1, gdummy);
-    if (KK != TP_KICK_NONE) {
-      d2 = pxyz(c, 0.0, s.py);
-      id2 = TP_ID2(d2);
-      pz = pxyz_r(c, s.px, s.py, ipz);
-    }
-    ebody_back(s, a, c, irho, rho, L1, L1b, c1, s1, d2, id2, pz, ipz);
   }
 #undef TP_ID2
 }
