@@ -17,8 +17,9 @@ using TrackPad
 using TrackPad: AbstractElement, Beam, Lattice, Drift, Marker, Quadrupole, Sextupole, Octupole,
                 ThinMultipole, SBend, ExactSBend, RFCavity, TimeVaryingElement
 
-export HIPContext, HIPTable, HIPParticles, compile_table, track!, linepass_hip!, ringpass_hip!,
-       ringpass_grad, ExactSBendLinearEdge, download
+export HIPContext, HIPTable, HIPParticles, HIPBatch, compile_table, track!, linepass_hip!, ringpass_hip!,
+       ringpass_grad, ExactSBendLinearEdge, download, jacobian, autotune!, set_waves!, set_launch!, batch_grad,
+       LaunchCfg
 
 const LIBPATH = Ref{String}(get(ENV, "TRACKPAD_HIP_LIB",
                                 joinpath(@__DIR__, "..", "..", "..", "lib", "libtrackpad_hip.so")))
@@ -234,6 +235,160 @@ function track_loss(k::Vector{Float64}, coords0::Matrix{Float64}, lat, beam, ntu
     c, lf, _, _ = download(p)
     alive = lf .== 0
     return sum(c[alive, 1] .^ 2 .+ c[alive, 3] .^ 2) / count(alive)
+end
+
+# ----------------------------------------------------------------------------------------
+# Register budgets, autotuner, Jacobians, batches (include/trackpad_hip.h)
+# ----------------------------------------------------------------------------------------
+"""
+    set_waves!(ctx; fwd_waves = 0, back_waves = 0)
+
+Register budget (resident waves per SIMD) of the tracking kernels (0, 5, 6) and of the
+reverse kernels (0, 3, 4); 0 leaves it to the compiler.
+"""
+set_waves!(ctx::HIPContext; fwd_waves::Integer = 0, back_waves::Integer = 0) =
+    check(ccall((:tp_set_waves, libtp()), Cint, (Ptr{Cvoid}, Cint, Cint), ctx.handle, fwd_waves, back_waves))
+
+# tp_launch_cfg
+struct LaunchCfg
+    block::Int32
+    ppl::Int32
+    turns_per_launch::Int32
+    fwd_waves::Int32
+    back_waves::Int32
+    candidates::Int32
+    rate::Float64
+end
+LaunchCfg() = LaunchCfg(0, 0, 0, 0, 0, 0, 0.0)
+
+const TP_TUNE_ADJOINT = UInt32(0x100)
+
+"""
+    autotune!(p, tab; probe_turns = 2, fast = false, target_launch_ms = 0, cache = nothing) -> LaunchCfg
+
+Time every compiled launch shape on a device copy of `p` (left unchanged) and apply the
+fastest to the context, with turns per launch for launches of about `target_launch_ms`.
+"""
+function autotune!(p::HIPParticles, tab::HIPTable; probe_turns::Integer = 2, fast::Bool = false,
+                   target_launch_ms::Real = 0, cache::Union{Nothing, AbstractString} = nothing)
+    out = Ref(LaunchCfg())
+    check(ccall((:tp_autotune, libtp()), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, UInt32, Float64, Cstring, Ref{LaunchCfg}),
+                p.ctx.handle, tab.handle, p.handle, probe_turns, fast ? TP_TRACK_FAST : UInt32(0),
+                target_launch_ms, cache === nothing ? C_NULL : cache, out))
+    return out[]
+end
+
+"""
+    jacobian(p, tab; nturns = 1, turn = 0, ckpt_stride = 1, fast = false) -> Array{Float64,3}
+
+Exact `nturns`-turn Jacobian d(final)/d(initial) of every particle of `p` (N x 6 x 6,
+`J[q, i, j] = d out_i / d in_j`), by six reverse passes with unit seeds.  `p` is tracked.
+"""
+function jacobian(p::HIPParticles, tab::HIPTable; nturns::Integer = 1, turn::Integer = 0,
+                  ckpt_stride::Integer = 1, fast::Bool = false)
+    jac = zeros(Float64, 36 * p.n)
+    check(ccall((:tp_jacobian, libtp()), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Int32, Int32, UInt32, Ptr{Float64}),
+                p.ctx.handle, tab.handle, p.handle, turn, nturns, ckpt_stride, fast ? TP_TRACK_FAST : UInt32(0),
+                jac))
+    # row-major 6 x 6 per particle in C order -> J[q, i, j]
+    return permutedims(reshape(jac, 6, 6, p.n), (3, 2, 1))
+end
+
+"""
+    HIPBatch(ctx, tables, particles)
+
+Lattice variants with their particle sets, tracked (`track!`) or differentiated
+(`adjoint`) in one persistent graph launch.  The batch keeps its tables and particle sets
+alive.
+"""
+mutable struct HIPBatch
+    handle::Ptr{Cvoid}
+    ctx::HIPContext
+    tables::Vector{HIPTable}
+    parts::Vector{HIPParticles}
+    function HIPBatch(ctx::HIPContext, tables::Vector{HIPTable}, parts::Vector{HIPParticles})
+        length(tables) == length(parts) || throw(DimensionMismatch("one particle set per table"))
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        th = Ptr{Cvoid}[t.handle for t in tables]
+        ph = Ptr{Cvoid}[q.handle for q in parts]
+        check(ccall((:tp_batch_create, libtp()), Cint, (Ptr{Cvoid}, Int32, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
+                    Ptr{Ptr{Cvoid}}), ctx.handle, length(tables), th, ph, h))
+        b = new(h[], ctx, tables, parts)
+        finalizer(b) do x
+            ccall((:tp_batch_destroy, libtp()), Cint, (Ptr{Cvoid},), x.handle)
+        end
+        return b
+    end
+end
+
+set_launch!(b::HIPBatch; block::Integer = 0, fwd_waves::Integer = 0, back_waves::Integer = 0) =
+    check(ccall((:tp_batch_set_launch, libtp()), Cint, (Ptr{Cvoid}, Cint, Cint, Cint), b.handle, block,
+                fwd_waves, back_waves))
+
+"""
+    track!(b::HIPBatch, nturns::Vector{Int32}; turn0 = nothing, fast = false)
+
+Track variant v for `nturns[v]` turns from `turn0[v]`, all variants in one launch.
+"""
+function track!(b::HIPBatch, nturns::Vector{Int32}; turn0::Union{Nothing, Vector{Int32}} = nothing,
+                fast::Bool = false)
+    check(ccall((:tp_batch_track, libtp()), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int32}, Ptr{Int32}, UInt32),
+                b.ctx.handle, b.handle, turn0 === nothing ? C_NULL : turn0, nturns,
+                fast ? TP_TRACK_FAST : UInt32(0)))
+    return b
+end
+
+"""
+    batch_grad(b::HIPBatch, nturns; turn0 = nothing, ckpt_stride = 10, fast = false)
+        -> (loss, n_alive, dL_dk per variant, dL_dx0 per variant)
+
+`ringpass_grad` of every variant in one graph launch (forward with checkpoints, reverse
+pass, per-variant reduction in tile order).
+"""
+function batch_grad(b::HIPBatch, nturns::Vector{Int32}; turn0::Union{Nothing, Vector{Int32}} = nothing,
+                 ckpt_stride::Integer = 10, fast::Bool = false)
+    nv = length(b.tables)
+    loss = zeros(Float64, nv)
+    na = zeros(Int64, nv)
+    nseq = [t.n_seq for t in b.tables]
+    ns = [q.n for q in b.parts]
+    dk = zeros(Float64, sum(nseq))
+    dx = zeros(Float64, 6 * sum(ns))
+    check(ccall((:tp_batch_adjoint, libtp()), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int32}, Ptr{Int32}, Int32, UInt32, Ptr{Float64}, Ptr{Int64},
+                 Ptr{Float64}, Ptr{Float64}),
+                b.ctx.handle, b.handle, turn0 === nothing ? C_NULL : turn0, nturns, ckpt_stride,
+                fast ? TP_TRACK_FAST : UInt32(0), loss, na, dk, dx))
+    dks = Vector{Vector{Float64}}(undef, nv)
+    dxs = Vector{Matrix{Float64}}(undef, nv)
+    ok = 0
+    ox = 0
+    for v in 1:nv
+        dks[v] = dk[ok + 1:ok + nseq[v]]
+        dxs[v] = reshape(dx[ox + 1:ox + 6 * ns[v]], ns[v], 6)   # column-major N_v x 6 per variant
+        ok += nseq[v]
+        ox += 6 * ns[v]
+    end
+    return loss, na, dks, dxs
+end
+
+"""
+    autotune!(b::HIPBatch, probe_turns; ckpt_stride = 10, adjoint = false, fast = false,
+              cache = nothing) -> LaunchCfg
+
+Block size and register budgets of the persistent kernels, timed on the batch itself with
+turn counts `probe_turns[v]` (particle sets restored after every probe).
+"""
+function autotune!(b::HIPBatch, probe_turns::Vector{Int32}; ckpt_stride::Integer = 10, adjoint::Bool = false,
+                   fast::Bool = false, cache::Union{Nothing, AbstractString} = nothing)
+    out = Ref(LaunchCfg())
+    fl = (fast ? TP_TRACK_FAST : UInt32(0)) | (adjoint ? TP_TUNE_ADJOINT : UInt32(0))
+    check(ccall((:tp_batch_autotune, libtp()), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int32}, Int32, UInt32, Cstring, Ref{LaunchCfg}),
+                b.ctx.handle, b.handle, probe_turns, ckpt_stride, fl, cache === nothing ? C_NULL : cache, out))
+    return out[]
 end
 
 include("table.jl")
