@@ -194,6 +194,30 @@ def test_exact_bend_large_angles(ctx):
             np.testing.assert_array_equal(gc[~alive], oc[~alive])
 
 
+def test_survivor_compaction_between_launches(ctx):
+    """Tracking in several launches compacts the survivors into a particle list between
+    launches (lost particles stop occupying lanes); lanes map to particles one to one, so
+    coordinates and loss records are the same bits as in one launch.  DA ring on a wide
+    24 x 24 x 3 grid: most particles are lost within the 12 turns."""
+    beam = Beam(W.ENERGY)
+    lat = W.da_ring(ncell=80, beam=beam)
+    grid = W.da_grid(24, 24, half=0.018)
+    for fast in (False, True):
+        one = gpu_track(ctx, lat, beam, grid, 12, fast=fast)
+        for tpl, ppl in ((1, 1), (5, 2), (3, 1)):
+            many = gpu_track(ctx, lat, beam, grid, 12, fast=fast, tpl=tpl, ppl=ppl)
+            for a, b in zip(one, many):
+                assert np.array_equal(a, b), (fast, tpl, ppl)
+        lost = one[1].astype(bool)
+        assert 0.3 < lost.mean() < 0.95, lost.mean()
+        assert (one[2][lost] >= 0).all() and (one[2][lost] < 12).all()
+    # all particles lost early: the remaining launches are skipped, records unchanged
+    wide = W.da_grid(4, 4, half=0.5)
+    one = gpu_track(ctx, lat, beam, wide, 6)
+    many = gpu_track(ctx, lat, beam, wide, 6, tpl=1)
+    assert one[1].all() and all(np.array_equal(a, b) for a, b in zip(one, many))
+
+
 def test_da_grid_orders_to_10(ctx):
     """DA ring (order-10 normal/skew errors on every magnet) on a 16 x 16 x 3 grid, 10
     turns: loss records identical away from the boundary, survivors within 1e-9."""

@@ -124,6 +124,7 @@ void particles_release(tp_particles* p) {
   (void)hipFree(p->d_coords);
   (void)hipFree(p->d_lt);
   (void)hipFree(p->d_le);
+  (void)hipFree(p->d_order);
   delete p;
   ctx_release(ctx);
 }
@@ -504,6 +505,14 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
                 (long long)turn0 + nturns, a.row_turn0, (long long)a.row_turn0 + a.n_rows);
   const bool timed = (flags & TP_TRACK_TIMED) != 0;
   const int chunk = ctx->tpl > 0 ? ctx->tpl : (nturns > 0 ? nturns : 1);
+  // Between launches the survivors are compacted into a particle list (compact_alive), so
+  // lanes of lost particles stop costing issue slots in the following launches.  It needs a
+  // count read back per launch, so asynchronous calls and single launches skip it.
+  const bool compact = !(flags & TP_TRACK_ASYNC) && nturns > chunk && p->n <= 0x7fffffffLL;
+  if (compact && !p->d_order) TP_HIP(hipMalloc(&p->d_order, (size_t)(2 * p->n + 1) * sizeof(int32_t)));
+  int32_t* lists[2] = {p->d_order, p->d_order ? p->d_order + p->n : nullptr};
+  int32_t* d_count = p->d_order ? p->d_order + 2 * p->n : nullptr;
+  int cur = -1;  // list in use (-1: all particles)
   double ms_total = 0.0;
   int launches = 0;
   if (p->n > 0 && a.n_seq > 0) {
@@ -521,6 +530,23 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
         float ms = 0.f;
         TP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ms_total += ms;
+      }
+      if (compact && done + chunk < nturns) {
+        const int nxt = cur == 1 ? 0 : 1;
+        e = tp_launch_fill_i32(d_count, 0, 1, ctx->stream);
+        if (e == hipSuccess)
+          e = tp_launch_compact_alive(cur < 0 ? nullptr : lists[cur], (int32_t)a.n, p->d_lt, lists[nxt], d_count,
+                                      ctx->stream);
+        if (e != hipSuccess) return fail(TP_EHIP, "tp_track: compaction launch failed: %s", hipGetErrorName(e));
+        int32_t alive = 0;
+        TP_HIP(hipMemcpyAsync(&alive, d_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        TP_HIP(hipStreamSynchronize(ctx->stream));
+        if (alive == 0) break;  // nothing left to track
+        if (alive < a.n) {      // switch to the shorter list (else keep the current one)
+          cur = nxt;
+          a.order = lists[cur];
+          a.n = alive;
+        }
       }
     }
   }

@@ -58,6 +58,9 @@ struct TpKArgs {
   int32_t waves;    // register budget of the kernel: 0 (compiler's), 5 or 6 waves per SIMD
   int32_t n_rows;   // sequence rows (1: static lattice; > 1: one row per turn from row_turn0)
   int32_t row_turn0;
+  // Particle list of the launch: lane i tracks particle order[i] (i < n); null = particle i.
+  // The host compacts the list of survivors between launches (compact_alive).
+  const int32_t* order;
 };
 
 // One lattice variant of a batch: where its element table sits in the concatenated
@@ -147,6 +150,10 @@ hipError_t tp_launch_batch_reduce(const TpVarDesc* desc, const int32_t* vfirst, 
                                   const double* item_partial, int32_t max_slots, const double* item_loss, double* grad,
                                   double* vloss, hipStream_t stream);
 hipError_t tp_launch_fill_i32(int32_t* p, int32_t v, int32_t n, hipStream_t stream);
+// order_out <- the entries of order_in (null: 0..n_in-1) whose particle is still alive
+// (lost_turn == -1), in no particular order; *count (cleared by the caller) += their number.
+hipError_t tp_launch_compact_alive(const int32_t* order_in, int32_t n_in, const int32_t* lost_turn, int32_t* order_out,
+                                   int32_t* count, hipStream_t stream);
 hipError_t tp_launch_adjoint_exact(const TpAdjArgs& a, hipStream_t stream);
 hipError_t tp_launch_adjoint_fast(const TpAdjArgs& a, hipStream_t stream);
 hipError_t tp_launch_grad_reduce(const double* partial, int nblocks, int n_slots, double* grad, hipStream_t stream);

@@ -54,6 +54,7 @@ struct tp_particles {
   double* d_coords = nullptr;
   int32_t* d_lt = nullptr;
   int32_t* d_le = nullptr;
+  int32_t* d_order = nullptr;  // 2 x n survivor lists + 1 counter, allocated on first use (tp_track)
 };
 
 // ---------------------------------------------------------------------------------------

@@ -580,8 +580,9 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void track_kernel(const int32_t*
   bool valid[PPL];
 #pragma unroll
   for (int k = 0; k < PPL; ++k) {
-    idx[k] = base + (int64_t)k * blockDim.x;
-    valid[k] = idx[k] < a.n;
+    const int64_t i = base + (int64_t)k * blockDim.x;
+    valid[k] = i < a.n;
+    idx[k] = (a.order && valid[k]) ? (int64_t)a.order[i] : i;
   }
   St s[PPL];
   Cx c[PPL];
@@ -662,6 +663,28 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_kernel(const TpVarDes
 __global__ void fill_i32(int32_t* __restrict__ p, int32_t v, int32_t n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// Survivor list for the next launch: each wave ballots its entries whose particle is alive,
+// lane 0 reserves that many slots with one atomicAdd, and every kept entry is written at
+// its rank within the wave's ballot.  Lanes map to particles one to one and no particle's
+// arithmetic depends on its lane, so tracking with the list gives the same bits.
+__global__ void compact_alive(const int32_t* __restrict__ order_in, int32_t n_in,
+                              const int32_t* __restrict__ lost_turn, int32_t* __restrict__ order_out,
+                              int32_t* __restrict__ count) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int lane = threadIdx.x & 63;
+  int32_t id = 0;
+  bool keep = false;
+  if (i < n_in) {
+    id = order_in ? order_in[i] : i;
+    keep = lost_turn[id] == -1;
+  }
+  const unsigned long long m = __ballot(keep);
+  int32_t at = 0;
+  if (lane == 0 && m != 0ull) at = atomicAdd(count, (int32_t)__popcll(m));
+  at = __shfl(at, 0, 64);
+  if (keep) order_out[at + (int32_t)__popcll(m & ((1ull << lane) - 1ull))] = id;
 }
 
 }  // namespace TPK_BUILD
@@ -781,6 +804,14 @@ hipError_t TP_LAUNCH(const TpKArgs& a, int block, int ppl, hipStream_t stream) {
 hipError_t tp_launch_fill_i32(int32_t* p, int32_t v, int32_t n, hipStream_t stream) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(tpk::TPK_BUILD::fill_i32, dim3((n + 255) / 256), dim3(256), 0, stream, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t tp_launch_compact_alive(const int32_t* order_in, int32_t n_in, const int32_t* lost_turn, int32_t* order_out,
+                                   int32_t* count, hipStream_t stream) {
+  if (n_in <= 0) return hipSuccess;
+  hipLaunchKernelGGL(tpk::TPK_BUILD::compact_alive, dim3((unsigned)((n_in + 255) / 256)), dim3(256), 0, stream,
+                     order_in, n_in, lost_turn, order_out, count);
   return hipGetLastError();
 }
 #endif
