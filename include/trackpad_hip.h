@@ -137,7 +137,10 @@ int tp_particles_device_ptrs(tp_particles* p, double** coords, int32_t** lost_tu
  * nturns == 1 is linepass! (src/lattice.jl:260-297); nturns > 1 is ringpass!
  * (src/lattice.jl:305-317). Loss is checked at every element exit with check_lost
  * (src/tracking.jl:49-52) plus the table's aperture; a lost particle keeps its
- * element-entry coordinates and is skipped afterwards (src/lattice.jl:269-284). */
+ * element-entry coordinates and is skipped afterwards (src/lattice.jl:269-284).
+ * When the turns take several launches (tp_set_launch turns_per_launch) and the call is
+ * synchronous, the survivors are compacted into a particle list between launches, so lost
+ * particles stop occupying lanes; results are the same bits either way. */
 int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns,
              uint32_t flags);
 
