@@ -194,6 +194,24 @@ def test_exact_bend_large_angles(ctx):
             np.testing.assert_array_equal(gc[~alive], oc[~alive])
 
 
+def test_persistent_launch_large_beam(ctx):
+    """A beam with more waves than the GPU holds at once runs as a persistent grid whose waves
+    take 64-particle tiles from a queue: 400,000 particles of the FODO workload (with 2 % of
+    them started outside the aperture), 3 turns in one launch and in three (queue plus
+    survivor compaction), parity build bit for bit with the oracle, both PPL."""
+    beam = Beam(W.ENERGY)
+    lat = W.fodo_lattice()
+    n = 400_000
+    c0 = W.fodo_particles(n, seed=21)
+    c0[::50, 0] = 0.021
+    oc, olf, olt, ole = O.ringpass(c0, lat, beam, np.zeros(n, np.int64), 3, aperture=0.02, nthreads=8)
+    assert olf.sum() >= n // 50
+    for block, ppl, tpl in ((256, 1, 0), (64, 2, 1), (128, 1, 1)):
+        gc, glf, glt, gle = gpu_track(ctx, lat, beam, c0, 3, aperture=0.02, block=block, ppl=ppl, tpl=tpl)
+        assert np.array_equal(glf, olf) and np.array_equal(glt, olt) and np.array_equal(gle, ole), (block, ppl, tpl)
+        assert np.array_equal(gc, oc), (block, ppl, tpl)
+
+
 def test_survivor_compaction_between_launches(ctx):
     """Tracking in several launches compacts the survivors into a particle list between
     launches (lost particles stop occupying lanes); lanes map to particles one to one, so

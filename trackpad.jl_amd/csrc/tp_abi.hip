@@ -113,6 +113,7 @@ void ctx_release(tp_ctx* ctx) {
   (void)hipEventDestroy(ctx->ev0);
   (void)hipEventDestroy(ctx->ev1);
   (void)hipStreamDestroy(ctx->stream);
+  (void)hipFree(ctx->d_counter);
   delete ctx;
 }
 
@@ -158,7 +159,11 @@ int tp_ctx_create(int device, tp_ctx** out) {
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e == hipSuccess) e = hipMalloc(&c->d_counter, 256);
   if (e != hipSuccess) {
+    if (c->d_counter) (void)hipFree(c->d_counter);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -513,15 +518,26 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
   int32_t* lists[2] = {p->d_order, p->d_order ? p->d_order + p->n : nullptr};
   int32_t* d_count = p->d_order ? p->d_order + 2 * p->n : nullptr;
   int cur = -1;  // list in use (-1: all particles)
+  // A launch with more waves than the device holds at once runs persistent: a resident
+  // grid whose waves take 64-particle tiles from a queue (TpKArgs::counter).
+  const bool fast = (flags & TP_TRACK_FAST) != 0;
+  const int occ = fast ? tp_track_occupancy_fast(a.variant, a.exact_h, ctx->block, ctx->ppl, ctx->fwd_waves)
+                       : tp_track_occupancy_exact(a.variant, a.exact_h, ctx->block, ctx->ppl, ctx->fwd_waves);
+  const int max_grid = occ > 0 ? occ * ctx->cus : 0;
   double ms_total = 0.0;
   int launches = 0;
   if (p->n > 0 && a.n_seq > 0) {
     for (int32_t done = 0; done < nturns; done += chunk) {
       a.turn0 = turn0 + done;
       a.nturns = (nturns - done < chunk) ? (nturns - done) : chunk;
+      const int64_t blocks = (a.n + (int64_t)ctx->block * ctx->ppl - 1) / ((int64_t)ctx->block * ctx->ppl);
+      a.counter = (max_grid > 0 && blocks > max_grid) ? ctx->d_counter : nullptr;
+      hipError_t e = hipSuccess;
+      if (a.counter) e = tp_launch_fill_i32(a.counter, 0, 1, ctx->stream);
+      if (e != hipSuccess) return fail(TP_EHIP, "tp_track: queue reset failed: %s", hipGetErrorName(e));
       if (timed) TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-      hipError_t e = (flags & TP_TRACK_FAST) ? tp_launch_track_fast(a, ctx->block, ctx->ppl, ctx->stream)
-                                             : tp_launch_track_exact(a, ctx->block, ctx->ppl, ctx->stream);
+      e = fast ? tp_launch_track_fast(a, ctx->block, ctx->ppl, max_grid, ctx->stream)
+               : tp_launch_track_exact(a, ctx->block, ctx->ppl, max_grid, ctx->stream);
       if (e != hipSuccess) return fail(TP_EHIP, "tp_track: kernel launch failed: %s", hipGetErrorName(e));
       ++launches;
       if (timed) {

@@ -61,6 +61,9 @@ struct TpKArgs {
   // Particle list of the launch: lane i tracks particle order[i] (i < n); null = particle i.
   // The host compacts the list of survivors between launches (compact_alive).
   const int32_t* order;
+  // Persistent launch: waves take tiles of 64 x PPL list entries from this queue counter
+  // (cleared before the launch) until it runs dry; null = one tile per wave of the grid.
+  int32_t* counter;
 };
 
 // One lattice variant of a batch: where its element table sits in the concatenated
@@ -135,11 +138,15 @@ struct TpAdjArgs {
 };
 
 // Host launchers exported by the kernel translation units (one per build flavour).
-hipError_t tp_launch_track_exact(const TpKArgs& a, int block, int ppl, hipStream_t stream);
-hipError_t tp_launch_track_fast(const TpKArgs& a, int block, int ppl, hipStream_t stream);
+// max_grid: blocks of a persistent launch (a.counter set; ignored otherwise)
+hipError_t tp_launch_track_exact(const TpKArgs& a, int block, int ppl, int max_grid, hipStream_t stream);
+hipError_t tp_launch_track_fast(const TpKArgs& a, int block, int ppl, int max_grid, hipStream_t stream);
 hipError_t tp_launch_batch_exact(const TpBatchArgs& b, bool ckpt, hipStream_t stream);
 hipError_t tp_launch_batch_fast(const TpBatchArgs& b, bool ckpt, hipStream_t stream);
 int tp_batch_occupancy_exact(int variant, int exact_h, bool ckpt, int block, int waves);
+// Resident blocks per CU of the tracking kernel that a launch shape selects.
+int tp_track_occupancy_exact(int variant, int exact_h, int block, int ppl, int waves);
+int tp_track_occupancy_fast(int variant, int exact_h, int block, int ppl, int waves);
 int tp_batch_occupancy_fast(int variant, int exact_h, bool ckpt, int block, int waves);
 hipError_t tp_launch_batch_back_exact(const TpBatchArgs& b, hipStream_t stream);
 hipError_t tp_launch_batch_back_fast(const TpBatchArgs& b, hipStream_t stream);

@@ -26,6 +26,8 @@ struct tp_ctx {
   int ppl = 1;
   int tpl = 0;  // turns per launch, 0 = all
   int fwd_waves = 0, back_waves = 0;  // register budgets (0 = the compiler's)
+  int cus = 0;                         // compute units of the device
+  int32_t* d_counter = nullptr;        // work-queue counter of persistent tracking launches
   tp_timing_t timing{};
 };
 

@@ -18,11 +18,13 @@
 
 #if TP_FAST
 #define TP_LAUNCH tp_launch_track_fast
+#define TP_TRACK_OCC tp_track_occupancy_fast
 #define TP_BATCH_LAUNCH tp_launch_batch_fast
 #define TP_BATCH_OCC tp_batch_occupancy_fast
 #define TPK_BUILD fast
 #else
 #define TP_LAUNCH tp_launch_track_exact
+#define TP_TRACK_OCC tp_track_occupancy_exact
 #define TP_BATCH_LAUNCH tp_launch_batch_exact
 #define TP_BATCH_OCC tp_batch_occupancy_exact
 #define TPK_BUILD exact
@@ -575,23 +577,44 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void track_kernel(const int32_t*
                                                     const double* __restrict__ pool, double* __restrict__ coords,
                                                     int32_t* __restrict__ lost_turn, int32_t* __restrict__ lost_elem,
                                                     TpKArgs a) {
-  const int64_t base = (int64_t)blockIdx.x * blockDim.x * PPL + threadIdx.x;
-  int64_t idx[PPL];
-  bool valid[PPL];
+  // Classic launch: the wave's lanes are list entries base + k blockDim.  Persistent launch
+  // (a.counter): the wave takes tiles of 64 PPL consecutive entries from the queue until
+  // it is empty; waves then start their tiles at different times and drift apart, so the
+  // scalar parameter loads of different waves do not all wait at the same element.
+  const bool persist = a.counter != nullptr;
+  const int lane = threadIdx.x & 63;
+  const int64_t n_tiles = (a.n + 64 * PPL - 1) / (64 * PPL);
+  for (;;) {
+    int64_t i0, stride;
+    if (persist) {
+      int32_t it = 0;
+      if (lane == 0) it = atomicAdd(a.counter, 1);
+      it = __builtin_amdgcn_readfirstlane(it);
+      if (it >= n_tiles) break;
+      i0 = (int64_t)it * 64 * PPL + lane;
+      stride = 64;
+    } else {
+      i0 = (int64_t)blockIdx.x * blockDim.x * PPL + threadIdx.x;
+      stride = blockDim.x;
+    }
+    int64_t idx[PPL];
+    bool valid[PPL];
 #pragma unroll
-  for (int k = 0; k < PPL; ++k) {
-    const int64_t i = base + (int64_t)k * blockDim.x;
-    valid[k] = i < a.n;
-    idx[k] = (a.order && valid[k]) ? (int64_t)a.order[i] : i;
+    for (int k = 0; k < PPL; ++k) {
+      const int64_t i = i0 + (int64_t)k * stride;
+      valid[k] = i < a.n;
+      idx[k] = (a.order && valid[k]) ? (int64_t)a.order[i] : i;
+    }
+    St s[PPL];
+    Cx c[PPL];
+    bool alive[PPL];
+    load_lanes<PPL>(s, c, alive, coords, lost_turn, a.ld, a.beti, idx, valid);
+    run_turns<EXH, FEAT, PPL>(s, c, alive, seq, recs, pool, coords, lost_turn, lost_elem, a, idx, 0, a.nturns);
+#pragma unroll
+    for (int k = 0; k < PPL; ++k)
+      if (alive[k]) store_state(coords + idx[k], a.ld, s[k]);
+    if (!persist) break;
   }
-  St s[PPL];
-  Cx c[PPL];
-  bool alive[PPL];
-  load_lanes<PPL>(s, c, alive, coords, lost_turn, a.ld, a.beti, idx, valid);
-  run_turns<EXH, FEAT, PPL>(s, c, alive, seq, recs, pool, coords, lost_turn, lost_elem, a, idx, 0, a.nturns);
-#pragma unroll
-  for (int k = 0; k < PPL; ++k)
-    if (alive[k]) store_state(coords + idx[k], a.ld, s[k]);
 }
 
 // Batched persistent kernel: every wave takes work items -- (lattice variant, tile of 64
@@ -722,6 +745,19 @@ static void launch_v(const TpKArgs& a, int ppl, unsigned grid, int block, hipStr
   }
 }
 
+template <bool EXH, unsigned FEAT>
+static const void* track_fn(int ppl, int waves) {
+  const void* fn = nullptr;
+  if (ppl == 2) {
+    fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::track_kernel<EXH, FEAT, 2, 0>);
+  } else {
+#define TP_CALL(W) fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::track_kernel<EXH, FEAT, 1, W>)
+    TP_WPE_SWITCH(waves, TP_CALL)
+#undef TP_CALL
+  }
+  return fn;
+}
+
 template <bool EXH, unsigned FEAT, bool CKPT>
 static const void* batch_fn(int waves) {
   const void* fn = nullptr;
@@ -782,10 +818,20 @@ int TP_BATCH_OCC(int variant, int exact_h, bool ckpt, int block, int waves) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) == hipSuccess ? nb : 0;
 }
 
-hipError_t TP_LAUNCH(const TpKArgs& a, int block, int ppl, hipStream_t stream) {
+int TP_TRACK_OCC(int variant, int exact_h, int block, int ppl, int waves) {
+  const void* fn = nullptr;
+#define TP_CALL(E, F) fn = track_fn<E, F>(ppl, waves)
+  TP_BATCH_DISPATCH(TP_CALL)
+#undef TP_CALL
+  int nb = 0;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) == hipSuccess ? nb : 0;
+}
+
+hipError_t TP_LAUNCH(const TpKArgs& a, int block, int ppl, int max_grid, hipStream_t stream) {
   if (a.n <= 0) return hipSuccess;
   const int64_t per_block = (int64_t)block * ppl;
-  const int64_t grid64 = (a.n + per_block - 1) / per_block;
+  int64_t grid64 = (a.n + per_block - 1) / per_block;
+  if (a.counter && max_grid > 0 && grid64 > max_grid) grid64 = max_grid;  // persistent grid
   if (grid64 > 0x7fffffff) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)grid64;
   switch (a.variant) {
