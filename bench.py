@@ -55,10 +55,11 @@ FP64_PEAK = 78.6e12
 HBM_PEAK = 8.0e12
 
 # HBM bytes per particle of one tracking-kernel launch, measured with rocprofv3 --pmc on the
-# 1e6-particle DBA launch: FETCH_SIZE 25,553.6 KiB x 1.998 (the counter reads 23,456.9 KiB for a
-# plain 48 MB device copy) + WRITE_SIZE 46,875.0 KiB x 1.00 (exact on the same copy), per 1e6
-# particles.  Independent of the turns per launch (the state stays in registers in between).
-MEASURED_BYTES_PER_PARTICLE_LAUNCH = {"dba": (25553.6 * 1.998 + 46875.0) * 1024 / 1e6}
+# 1e6-particle DBA launch: FETCH_SIZE 25,562.4 KiB x 1.998 (the counter reads 23,457.1 KiB for a
+# plain 48 MB device copy) + WRITE_SIZE 47,491.3 KiB x 1.00 (exact on the same copy; 616 KiB
+# above the 46,875 KiB of coordinates are the tile queue's atomics), per 1e6 particles.
+# Independent of the turns per launch (the state stays in registers in between).
+MEASURED_BYTES_PER_PARTICLE_LAUNCH = {"dba": (25562.4 * 1.998 + 47491.3) * 1024 / 1e6}
 
 
 def host_cores() -> int:
