@@ -526,10 +526,17 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
   const int max_grid = occ > 0 ? occ * ctx->cus : 0;
   double ms_total = 0.0;
   int launches = 0;
+  // With compaction the launches start short and double up to `chunk` turns (1, 2, 4, ...)
+  // while they lose particles: a lossy beam (a dynamic-aperture grid loses most of its
+  // particles in the first turns) is compacted early; a launch that loses under 1 % of its
+  // particles sends the rest straight to `chunk`, so a beam that loses nothing pays one
+  // extra launch.
+  int32_t len = compact ? 1 : chunk;
   if (p->n > 0 && a.n_seq > 0) {
-    for (int32_t done = 0; done < nturns; done += chunk) {
+    for (int32_t done = 0; done < nturns; done += a.nturns) {
       a.turn0 = turn0 + done;
-      a.nturns = (nturns - done < chunk) ? (nturns - done) : chunk;
+      a.nturns = (nturns - done < len) ? (nturns - done) : len;
+      if (len < chunk) len = (len > chunk / 2) ? chunk : 2 * len;
       const int64_t blocks = (a.n + (int64_t)ctx->block * ctx->ppl - 1) / ((int64_t)ctx->block * ctx->ppl);
       a.counter = (max_grid > 0 && blocks > max_grid) ? ctx->d_counter : nullptr;
       hipError_t e = hipSuccess;
@@ -547,7 +554,7 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
         TP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ms_total += ms;
       }
-      if (compact && done + chunk < nturns) {
+      if (compact && done + a.nturns < nturns) {
         const int nxt = cur == 1 ? 0 : 1;
         e = tp_launch_fill_i32(d_count, 0, 1, ctx->stream);
         if (e == hipSuccess)
@@ -558,6 +565,7 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
         TP_HIP(hipMemcpyAsync(&alive, d_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         TP_HIP(hipStreamSynchronize(ctx->stream));
         if (alive == 0) break;  // nothing left to track
+        if (alive >= a.n - a.n / 100) len = chunk;
         if (alive < a.n) {      // switch to the shorter list (else keep the current one)
           cur = nxt;
           a.order = lists[cur];
