@@ -15,6 +15,9 @@ The JSON line carries:
                  FP64 peak of 78.6 TFLOP/s
   cpu_baseline   the oracle (the C++ restatement of the reference's CPU path, same loop
                  order) on this host's cores, on a bounded sample of the same workload
+  streamed       after the timed region: the streamed-state variant (one turn per launch,
+                 the beam read from and written to HBM every turn) -- its HBM GB/s against
+                 the 8 TB/s peak and its FP64 rate (--streamed-turns, 0 skips it)
 
 Workload ``batched``: 256 variants of the DBA ring (strengths x (1 + N(0, 1%)), seeded),
 N_v log-uniform in [1k, 64k], T_v uniform in [100, 2000]; one step is one graph launch of
@@ -361,6 +364,39 @@ def batched_line(args, nvar, ns, ts, sched, wall, n_seq, info_f, fpt, live, avg_
     }
 
 
+def streamed(ctx, tab, init, work, args, fast, fpt, n, n_seq):
+    """The streamed-state variant, measured after the timed region: one turn per launch, so
+    every launch reads the beam from HBM and writes it back (48 B in, 48 B out and 8 B of
+    loss record per particle: 104 B algorithmic).  Reports the launches' HBM rate against
+    the 8 TB/s peak next to their FP64 rate."""
+    from trackpad_hip import track_
+
+    T = args.streamed_turns
+    ctx.set_launch(args.block, args.ppl, 1)
+    work.copy_from(init)
+    track_(work, tab, nturns=1, fast=fast)  # untimed: first launch at this shape
+    work.copy_from(init)
+    ctx.sync()
+    track_(work, tab, nturns=T, fast=fast, timed=True)
+    tm = ctx.timing()
+    ctx.set_launch(args.block, args.ppl, args.turns_per_launch)
+    _, _, lt, le = work.download()
+    live = float(live_turns_(lt, le, n_seq, T))
+    launches = max(tm["launches"], 1)
+    avg_s = tm["kernel_ms"] / 1e3 / launches
+    alive_per_launch = live / launches
+    algo = 104.0 * alive_per_launch
+    bpp = MEASURED_BYTES_PER_PARTICLE_LAUNCH.get(args.workload)
+    flops = fpt * alive_per_launch / avg_s
+    return {"turns_per_launch": 1, "turns": T, "launches": tm["launches"], "avg_launch_ms": avg_s * 1e3,
+            "bytes_per_launch": algo, "achieved_GBps": algo / avg_s / 1e9, "peak_GBps": HBM_PEAK / 1e9,
+            "hbm_frac": algo / avg_s / HBM_PEAK,
+            "traffic": None if bpp is None else bpp * n,
+            "fp64_TFLOPs": flops / 1e12, "fp64_frac": flops / FP64_PEAK,
+            "note": "104 B per live particle per launch (48 in + 48 out + 8 loss record); rates over the "
+                    "launches' HIP-event time; compute-bound even at one turn per launch"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -384,6 +420,8 @@ def main():
     ap.add_argument("--ppl", type=int, default=1)
     ap.add_argument("--fwd-waves", type=int, default=0, help="register budget of the tracking kernels (0, 5, 6)")
     ap.add_argument("--back-waves", type=int, default=0, help="register budget of the reverse kernels (0, 3, 4)")
+    ap.add_argument("--streamed-turns", type=int, default=10,
+                    help="turns of the streamed-state measurement (1 turn per launch; 0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", default=None,
                     help="particles x turns of the CPU baseline sample (default 16384x20; da 2048x5)")
@@ -491,6 +529,8 @@ def main():
                      "frac": achieved / FP64_PEAK, "traffic": traffic,
                      "avg_launch_ms": avg_launch_s * 1e3, "launches": launches},
     }
+    if args.streamed_turns > 0:
+        out["streamed"] = streamed(ctx, tab, init, work, args, fast, fpt, n, n_seq)
     if not args.no_cpu_baseline:
         ns, ts = (int(v) for v in args.cpu_sample.lower().split("x"))
         cores = host_cores()

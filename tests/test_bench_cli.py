@@ -30,6 +30,7 @@ def test_help_lists_the_workloads():
     assert r.returncode == 0
     for w in ("dba", "fodo", "da", "diff", "batched"):
         assert w in r.stdout
+    assert "--streamed-turns" in r.stdout
 
 
 def test_workload_sizes_and_flops():

@@ -267,7 +267,7 @@ def linepass_(coords: np.ndarray, lat: Lattice, beam: Beam, lost_flags: np.ndarr
 def track_(particles: HIPParticles, lat, beam: Optional[Beam] = None, nturns: int = 1, *,
            turn: int = 0, fast: bool = False, timed: bool = False, aperture: Optional[float] = None,
            time: float = 0.0, dt_turn: float = 0.0) -> None:
-    """Track device-resident particles for ``nturns`` turns (the north-star ``track!``).
+    """Track device-resident particles for ``nturns`` turns (``track!`` of the Julia package).
 
     ``lat`` is a :class:`Lattice` (compiled with ``beam``) or an already uploaded
     :class:`DeviceTable`.
