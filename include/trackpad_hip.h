@@ -166,9 +166,10 @@ int tp_track_host(tp_ctx* ctx, const tp_table* t, double* coords, int64_t n, int
  * lattice position's main strength (k1 of a Quadrupole, k2 of a Sextupole, k3 of an
  * Octupole; 0 for other positions), dL_dx0 (N x 6 column-major host array) = dL/d initial
  * coordinates.  Particles lost during tracking do not enter L and get zero gradient.
- * Supported: drifts, thick multipoles, thin multipoles, SBend, exact bends with linear
- * edges, RF cavities, Marker, t1/t2 translations, exact Hamiltonian; other tables return
- * TP_EUNSUPPORTED.  flags: TP_TRACK_FAST / TP_TRACK_TIMED as for tp_track. */
+ * Supported: drifts, thick multipoles, thin multipoles, SBend, the reference's ExactSBend
+ * (yrot, Forest fringe, wedges; not in its drift limit |angle/L| < 1e-6), exact bends with
+ * linear edges, RF cavities, Marker, t1/t2 translations, exact Hamiltonian; other tables
+ * return TP_EUNSUPPORTED.  flags: TP_TRACK_FAST / TP_TRACK_TIMED as for tp_track. */
 int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns,
                int32_t ckpt_stride, uint32_t flags, double* loss, int64_t* n_alive, double* dL_dk,
                double* dL_dx0);

@@ -60,3 +60,17 @@ def element_cases():
 # Elements whose map uses only + - * / sqrt on particle coordinates: the parity build
 # must match the oracle bit for bit.  The rest call asin/sin/tan/atan/cos per particle.
 BITEXACT = {"Marker", "Drift", "Quadrupole", "Sextupole", "Octupole", "ThinMultipole", "SBend", "RBend"}
+
+
+def forest_ring(energy):
+    """Four cells of the reference's ExactSBend with yrot (e1, e2 != 0), Forest fringe (gK != 0)
+    and wedges, one bend with a sextupole component (non-identity kicks between the body's
+    sub-steps), quads, a sextupole, and an RF cavity (h = 400)."""
+    cell = [Drift(1.0), Quadrupole(0.4, 1.2), Drift(0.5), ExactSBend(1.0, 0.1, 0.03, 0.02, gk=0.01), Drift(0.3),
+            Sextupole(0.2, 10.0), Drift(0.2), Quadrupole(0.4, -1.2), Drift(0.5),
+            ExactSBend(1.0, 0.1, 0.0, 0.05, gk=0.02, polynom_b=[0.0, 0.0, 3.0, 0.0], max_order=2, num_int_steps=4),
+            Drift(1.0)]
+    els = cell * 4
+    circ = sum(e.L for e in els)
+    els.append(RFCavity(0.0, 1e6, 400 * 2.99792458e8 / circ, energy=energy))
+    return Lattice(els)

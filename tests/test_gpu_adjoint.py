@@ -22,6 +22,8 @@ from trackpad_hip import (Beam, Drift, ExactSBend, Lattice, Quadrupole, Sextupol
                           set_strengths, strength_positions)
 from trackpad_hip import workloads as W
 
+import lattices as LT
+
 pytestmark = pytest.mark.gpu
 
 
@@ -146,10 +148,53 @@ def test_trackloss_autograd(ctx):
 
 
 def test_adjoint_refuses_unsupported(ctx):
+    from trackpad_hip import Solenoid
     beam = Beam(W.ENERGY)
-    lat = Lattice([Drift(1.0), ExactSBend(1.0, 0.1, 0.02, 0.02), Quadrupole(0.4, 1.0)])
+    lat = Lattice([Drift(1.0), Solenoid(0.5, 0.1), Quadrupole(0.4, 1.0)])
     with pytest.raises(TrackPadHIPError, match="TP_EUNSUPPORTED"):
         ringpass_grad(W.dba_particles(8), lat, beam, 2, ctx=ctx)
+    # an exact bend in its drift limit (|angle / L| < 1e-6)
+    lat = Lattice([Drift(1.0), ExactSBend(1.0, 1e-8), Quadrupole(0.4, 1.0)])
+    with pytest.raises(TrackPadHIPError, match="TP_EUNSUPPORTED"):
+        ringpass_grad(W.dba_particles(8), lat, beam, 2, ctx=ctx)
+
+
+@pytest.mark.parametrize("fast", [False, True])
+def test_adjoint_forest_exact_bends(ctx, fast):
+    """Reverse mode through the reference's ExactSBend (yrot, bend_fringe, bend_edge undone by
+    their inverses, VJPs by dual numbers at the reconstructed inputs): gradients against the
+    oracle's forward-mode derivatives along random directions and per strength, relative
+    1e-9, both builds, 5 turns of 64 particles."""
+    beam = Beam(W.ENERGY)
+    lat = LT.forest_ring(W.ENERGY)
+    n, T = 64, 5
+    c0 = W.gaussian_particles(n, 31, 2e-4, 2e-5, 2e-4, 2e-5, 1e-3)
+    g = ringpass_grad(c0, lat, beam, T, fast=fast, ctx=ctx)
+    assert g.n_alive == n
+    rng = np.random.default_rng(8)
+    pos = strength_positions(lat)
+    assert len(pos) == 12
+    for trial in range(3):
+        dk = np.zeros(len(lat))
+        dk[pos] = rng.normal(0, 1, len(pos))
+        dr0 = rng.normal(0, 1, (n, 6)) * np.array([5e-5, 2e-5, 1e-5, 2e-6, 1e-4, 1e-3])
+        loss, dl, na = O.loss_dir(lat, beam, c0, T, dk=dk, dr0=dr0)
+        assert na == n
+        ref = dl / na
+        got = _directional(g, dk, dr0)
+        assert abs(got - ref) <= 1e-9 * abs(ref), (trial, got, ref)
+    for p in pos[::3]:
+        dk = np.zeros(len(lat))
+        dk[p] = 1.0
+        _, dl, na = O.loss_dir(lat, beam, c0, T, dk=dk)
+        assert abs(g.dk[p] - dl / na) <= 1e-9 * abs(dl / na) + 1e-22, (p, g.dk[p], dl / na)
+    # every initial coordinate of two particles, against unit directions
+    for q in (0, 37):
+        for j in range(6):
+            dr0 = np.zeros((n, 6))
+            dr0[q, j] = 1.0
+            _, dl, na = O.loss_dir(lat, beam, c0, T, dr0=dr0)
+            assert abs(g.dx0[q, j] - dl / na) <= 1e-9 * abs(dl / na) + 1e-20, (q, j, g.dx0[q, j], dl / na)
 
 
 def test_adjoint_diff_workload(ctx):

@@ -181,6 +181,30 @@ def test_batch_adjoint_matches_per_variant(ctx, fast):
     assert g.n_alive[0] == 98
 
 
+@pytest.mark.parametrize("fast", [False, True])
+def test_batch_adjoint_forest_variant(ctx, fast):
+    """A batch holding a ring of the reference's ExactSBend (Forest yrot / fringe / wedge) next
+    to a DBA variant runs the reverse kernel variant with the Forest maps for both; each
+    variant's results equal its own tp_adjoint call as in the test above."""
+    import lattices as LT
+    beam = _beam()
+    lats = [LT.forest_ring(W.ENERGY), _variants(1, seed=10)[0]]
+    ns = [70, 64]
+    ts = np.array([5, 3], np.int32)
+    coords = [W.gaussian_particles(ns[0], 33, 2e-4, 2e-5, 2e-4, 2e-5, 1e-3), _coords(ns[1], seed=80)]
+    tabs = [ctx.table(compile_table(l, beam)) for l in lats]
+    parts = [HIPParticles.from_coords(c, None, ctx) for c in coords]
+    b = HIPBatch(tabs, parts)
+    g = b.adjoint(ts, ckpt_stride=2, fast=fast)
+    for v, (lat, c, t) in enumerate(zip(lats, coords, ts)):
+        r = ringpass_grad(c, lat, beam, int(t), ckpt_stride=2, fast=fast, ctx=ctx)
+        assert g.n_alive[v] == r.n_alive == ns[v]
+        assert abs(g.loss[v] - r.loss) <= 1e-13 * abs(r.loss), v
+        _same(g.dx0[v], r.dx0, fast, v)
+        scale = np.max(np.abs(r.dk))
+        assert np.all(np.abs(g.dk[v] - r.dk) <= 1e-12 * scale), (v, np.max(np.abs(g.dk[v] - r.dk)) / scale)
+
+
 def test_batch_adjoint_vs_quad_precision_fd(ctx):
     """One variant of a batch (DBA, 8 particles, 30 turns, checkpoints every 7 turns):
     central differences with h = 1e-7 in quad precision agree to 1e-7 with the batched
@@ -223,9 +247,10 @@ def test_batch_adjoint_vs_quad_precision_fd(ctx):
 def test_batch_refusals_and_lifetime(ctx):
     beam = _beam()
     ring = _variants(1, seed=12)[0]
-    forest = Lattice([Drift(1.0), ExactSBend(1.0, 0.1, 0.02, 0.02), Quadrupole(0.4, 1.0)])
+    from trackpad_hip import Solenoid
+    other = Lattice([Drift(1.0), Solenoid(0.5, 0.1), Quadrupole(0.4, 1.0)])
     c = _coords(20, seed=90)
-    tabs = [ctx.table(compile_table(ring, beam)), ctx.table(compile_table(forest, beam))]
+    tabs = [ctx.table(compile_table(ring, beam)), ctx.table(compile_table(other, beam))]
     parts = [HIPParticles.from_coords(c, None, ctx) for _ in range(2)]
     b = HIPBatch(tabs, parts)
     b.track([2, 2])  # tracking is supported

@@ -47,6 +47,27 @@ def test_jacobian_determinant_is_one(ctx):
         assert np.max(np.abs(det - 1.0)) <= tol, (T, np.max(np.abs(det - 1.0)))
 
 
+@pytest.mark.parametrize("fast", [False, True])
+def test_jacobian_forest_exact_bends(ctx, fast):
+    """Jacobians through the reference's ExactSBend (yrot, Forest fringe with gK != 0, wedges):
+    against the oracle's dual numbers after 3 turns (1e-10 of the largest entry; 1e-9 fast),
+    and unit determinant to 1e-11 for 32 particles."""
+    import lattices as LT
+    beam = Beam(W.ENERGY)
+    lat = LT.forest_ring(W.ENERGY)
+    c0 = W.gaussian_particles(32, 44, 1e-3, 1e-4, 1e-3, 1e-4, 1e-3)
+    J, fin, lf = jacobians(c0, lat, beam, 3, fast=fast, ctx=ctx)
+    assert not lf.any()
+    tol = 1e-9 if fast else 1e-10
+    for q in (0, 13, 31):
+        ro, Jo, lost = O.jacobian(lat, beam, c0[q], 3)
+        assert not lost
+        scale = np.max(np.abs(Jo))
+        assert np.max(np.abs(J[q] - Jo)) <= tol * scale, (q, np.max(np.abs(J[q] - Jo)) / scale)
+    det = np.linalg.det(J)
+    assert np.max(np.abs(det - 1.0)) <= 1e-11, np.max(np.abs(det - 1.0))
+
+
 def test_jacobian_of_lost_particle_is_zero(ctx):
     beam, lat = _dba()
     c0 = W.dba_particles(4, seed=43)

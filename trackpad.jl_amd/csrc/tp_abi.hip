@@ -631,6 +631,14 @@ struct DevBuf {
 
 namespace {
 
+// Variant of the reverse kernels: the largest one (TP_FEAT_ADJ_ALL) only for Forest exact
+// bends; a table whose forward variant is ALL for its misalignments alone runs the C reverse
+// kernel, which undoes t1/t2 for every variant at run time.
+int adj_variant(unsigned features, int variant) {
+  if (features & FT_ESBEND) return TP_VARIANT_ALL;
+  return variant > TP_VARIANT_C ? TP_VARIANT_C : variant;
+}
+
 // Checks shared by the reverse-mode entry points.
 int adj_check(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t nturns, int32_t ckpt_stride, uint32_t flags,
               const char* who) {
@@ -640,11 +648,14 @@ int adj_check(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t nturns, i
   if (nturns < 1) return fail(TP_EINVAL, "%s: nturns = %d", who, nturns);
   if (ckpt_stride < 0) return fail(TP_EINVAL, "%s: ckpt_stride = %d", who, ckpt_stride);
   if (flags & ~(TP_TRACK_FAST | TP_TRACK_TIMED)) return fail(TP_EINVAL, "%s: unsupported flags 0x%x", who, flags);
-  if (t->features & (FT_ESBEND | FT_LINH | FT_EXTRA))
-    return fail(TP_EUNSUPPORTED, "%s: Forest exact bends, the linearised drift and the element families beyond "
-                "drifts, multipoles, bends and RF cavities have no adjoint here", who);
-  for (const tp_rec& r : t->recs)
+  if (t->features & (FT_LINH | FT_EXTRA))
+    return fail(TP_EUNSUPPORTED, "%s: the linearised drift and the element families beyond drifts, multipoles, "
+                "bends and RF cavities have no adjoint here", who);
+  for (const tp_rec& r : t->recs) {
     if (r.flags & (TP_F_R1 | TP_F_R2)) return fail(TP_EUNSUPPORTED, "%s: r1/r2 rotations have no adjoint here", who);
+    if (r.tag == TP_ESBEND && (r.flags & TP_F_DRIFTLIMIT))
+      return fail(TP_EUNSUPPORTED, "%s: exact bends with |angle/L| < 1e-6 (drift limit) have no adjoint here", who);
+  }
   if (t->hdr.n_rows > 1) return fail(TP_EUNSUPPORTED, "%s: time-varying tables have no adjoint here", who);
   return set_dev(ctx);
 }
@@ -745,7 +756,7 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
   a.ld = n;
   a.n_seq = (int32_t)t->hdr.n_seq;
   a.n_slots = n_slots;
-  a.variant = t->variant;
+  a.variant = adj_variant(t->features, t->variant);
   a.waves = ctx->back_waves;
   a.beti = t->hdr.beti;
   // the reverse pass seeds dL/dx = 2x (the gradient of the loss sum); the 1/N_alive of the
@@ -830,7 +841,7 @@ int tp_jacobian(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, 
   a.ld = n;
   a.n_seq = (int32_t)t->hdr.n_seq;
   a.n_slots = 0;
-  a.variant = t->variant;
+  a.variant = adj_variant(t->features, t->variant);
   a.waves = ctx->back_waves;
   a.beti = t->hdr.beti;
   std::vector<double> row(6 * plane);
@@ -1012,6 +1023,7 @@ int batch_plan(tp_batch* b, int mode, const int32_t* turn0, const int32_t* nturn
   a.item_partial = (double*)b->d_item_partial;
   a.max_slots = b->max_slots;
   a.variant = b->variant;
+  a.back_variant = adj_variant(b->features, b->variant);
   a.exact_h = b->exact_h;
   a.block = b->block;
   const int occ_f = fast ? tp_batch_occupancy_fast(b->variant, b->exact_h, adj, b->block, b->fwd_waves)
@@ -1019,8 +1031,8 @@ int batch_plan(tp_batch* b, int mode, const int32_t* turn0, const int32_t* nturn
   if (occ_f < 1) return fail(TP_EHIP, "batch: the forward kernel does not fit a CU with %d threads", b->block);
   b->grid_f = pick_grid(occ_f, b->cus, wpb, n_items);
   if (adj) {
-    const int occ_b = fast ? tp_batch_back_occupancy_fast(b->variant, b->block, b->max_slots, b->back_waves)
-                           : tp_batch_back_occupancy_exact(b->variant, b->block, b->max_slots, b->back_waves);
+    const int occ_b = fast ? tp_batch_back_occupancy_fast(a.back_variant, b->block, b->max_slots, b->back_waves)
+                           : tp_batch_back_occupancy_exact(a.back_variant, b->block, b->max_slots, b->back_waves);
     if (occ_b < 1) return fail(TP_EHIP, "batch: the reverse kernel does not fit a CU with %d threads", b->block);
     b->grid_b = pick_grid(occ_b, b->cus, wpb, n_items);
   }
@@ -1150,11 +1162,14 @@ int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_part
     b->max_slots = std::max(b->max_slots, ns);
     if (b->no_adjoint.empty()) {
       if (t->hdr.n_rows > 1) b->no_adjoint = "time-varying tables have no adjoint here";
-      if (t->features & (FT_ESBEND | FT_LINH | FT_EXTRA))
-        b->no_adjoint = "Forest exact bends, the linearised drift and the element families beyond drifts, "
-                        "multipoles, bends and RF cavities have no adjoint here";
-      for (const tp_rec& r : t->recs)
+      if (t->features & (FT_LINH | FT_EXTRA))
+        b->no_adjoint = "the linearised drift and the element families beyond drifts, multipoles, bends and RF "
+                        "cavities have no adjoint here";
+      for (const tp_rec& r : t->recs) {
         if (r.flags & (TP_F_R1 | TP_F_R2)) b->no_adjoint = "r1/r2 rotations have no adjoint here";
+        if (r.tag == TP_ESBEND && (r.flags & TP_F_DRIFTLIMIT))
+          b->no_adjoint = "exact bends with |angle/L| < 1e-6 (drift limit) have no adjoint here";
+      }
     }
   }
   const unsigned feat = b->features;

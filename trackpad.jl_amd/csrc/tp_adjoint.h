@@ -298,6 +298,234 @@ TP_DEV void rf_kick_back(St& s, Ad& a, const double* __restrict__ p) {
   a.z = a.z - a.d * nv * cph * (w / 2.99792458e8) / bb;
 }
 
+// ---------------------------------------------------------------------------------------
+// Forest maps of ExactSBend (src/tracking.jl:727-851) backwards.
+// Inverses: yrot(-phi) undoes yrot(phi) and bend_edge(-theta) undoes bend_edge(theta)
+// exactly in exact arithmetic (rotations of the reference frame, in free space and in the
+// dipole field, compose additively in the angle; checked to 1e-40 at 40 digits).
+// bend_fringe changes y, x, py and z with coefficients phi, dpx, dpy, dd that depend on the
+// input (px, py, delta); px and delta pass through.  Its inverse solves
+// py_out = py - phi(px, py, delta) yf for the input py by fixed-point steps (a contraction
+// with factor |d phi/d py| |yf| ~ b0 |px py| |y|, below 1e-6 in any lattice that keeps the
+// particle), then y = yf - dpy yf^2 / 2 (yf = 2y / (1 + sqrt(1 - 2 dpy y)) is the root of
+// dpy yf^2 / 2 - yf + y = 0), x = x_out - dpx yf^2 / 2 and z = z_out + dd yf^2 / 2.
+// VJPs: the Jacobian of each map at the reconstructed input, one input direction at a time,
+// by forward-mode dual numbers (D1) through the same formulas; the maps run twice per bend
+// per turn at most, so four evaluations cost little and keep the register need small.
+// ---------------------------------------------------------------------------------------
+struct D1 {
+  double v, d;
+  D1() = default;
+  TP_DEV D1(double x) : v(x), d(0.0) {}
+  TP_DEV D1(double x, double dx) : v(x), d(dx) {}
+};
+TP_DEV D1 operator+(D1 a, D1 b) { return D1(a.v + b.v, a.d + b.d); }
+TP_DEV D1 operator-(D1 a, D1 b) { return D1(a.v - b.v, a.d - b.d); }
+TP_DEV D1 operator-(D1 a) { return D1(-a.v, -a.d); }
+TP_DEV D1 operator*(D1 a, D1 b) { return D1(a.v * b.v, a.d * b.v + a.v * b.d); }
+TP_DEV D1 operator/(D1 a, D1 b) {
+  const double q = a.v / b.v;
+  return D1(q, (a.d - q * b.d) / b.v);
+}
+TP_DEV D1 operator+(double a, D1 b) { return D1(a + b.v, b.d); }
+TP_DEV D1 operator+(D1 a, double b) { return D1(a.v + b, a.d); }
+TP_DEV D1 operator-(double a, D1 b) { return D1(a - b.v, -b.d); }
+TP_DEV D1 operator-(D1 a, double b) { return D1(a.v - b, a.d); }
+TP_DEV D1 operator*(double a, D1 b) { return D1(a * b.v, a * b.d); }
+TP_DEV D1 operator*(D1 a, double b) { return D1(a.v * b, a.d * b); }
+TP_DEV D1 operator/(D1 a, double b) { return D1(a.v / b, a.d / b); }
+TP_DEV D1 operator/(double a, D1 b) {
+  const double q = a / b.v;
+  return D1(q, -q * b.d / b.v);
+}
+TP_DEV double val(double a) { return a; }
+TP_DEV double val(D1 a) { return a.v; }
+TP_DEV double gsqrt(double a) { return sqrt(a); }
+TP_DEV D1 gsqrt(D1 a) {
+  const double r = sqrt(a.v);
+  return D1(r, 0.5 * a.d / r);
+}
+TP_DEV double gtan(double a) { return tan_lib(a); }
+TP_DEV D1 gtan(D1 a) {
+  const double t = tan_lib(a.v);
+  return D1(t, a.d * (1.0 + t * t));
+}
+TP_DEV double gatan(double a) { return atan(a); }
+TP_DEV D1 gatan(D1 a) { return D1(atan(a.v), a.d / (1.0 + a.v * a.v)); }
+TP_DEV double gcos(double a) { return cos_lib(a); }
+TP_DEV D1 gcos(D1 a) { return D1(cos_lib(a.v), -sin_lib(a.v) * a.d); }
+// asin of the argument clamped to [-1, 1] (src/tracking.jl:839-840); zero slope where clamped
+TP_DEV double gasin_c(double a) { return asin_s(clamp1(a)); }
+TP_DEV D1 gasin_c(D1 a) {
+  if (!(fabs(a.v) < 1.0)) return D1(asin_s(clamp1(a.v)), 0.0);
+  return D1(asin_s(a.v), a.d / sqrt(1.0 - a.v * a.v));
+}
+// pxyz (src/tracking.jl:717-720): sqrt(B^2 - px^2 - py^2), 0 where that is not positive
+template <class T>
+TP_DEV T pxyz_g(T B, T px, T py) {
+  const T v = B * B - px * px - py * py;
+  return val(v) > 0.0 ? gsqrt(v) : T(0.0);
+}
+
+// yrot (src/tracking.jl:727-745) as functions of (x, px, py, B = beti + delta): the new x
+// and px and the increments of y and z
+template <class T>
+TP_DEV void yrot_g(T x, T px, T py, T B, double co, double si, T& xo, T& pxo, T& dy, T& dz) {
+  const T pz = pxyz_g(B, px, py);
+  const T p = co * pz - si * px;
+  pxo = si * pz + co * px;
+  xo = x * pz / p;
+  dy = x * py * si / p;
+  dz = B * x * si / p;
+}
+
+// bend_edge (src/tracking.jl:822-851), same outputs
+template <class T>
+TP_DEV void edge_g(T x, T px, T py, T B, double rhoinv, double co, double si, double s2t, double theta, T& xo,
+                   T& pxo, T& dy, T& dz) {
+  const T pz = pxyz_g(B, px, py);
+  const T d2 = pxyz_g(B, T(0.0), py);
+  pxo = px * co + (pz - rhoinv * x) * si;
+  const T dasin = gasin_c(px / d2) - gasin_c(pxo / d2);
+  const T num = x * (px * s2t + si * si * (2.0 * pz - rhoinv * x));
+  const T den = pxyz_g(B, pxo, py) + pz * co - px * si;
+  xo = x * co + num / den;
+  dy = py * (theta / rhoinv + dasin / rhoinv);
+  dz = B / rhoinv * (theta + dasin);
+}
+
+// The coefficients of bend_fringe (src/tracking.jl:752-815) as functions of (px, py, B)
+template <class T>
+TP_DEV void fringe_coef_g(T px, T py, T B, double b0, double gK, T& phi, T& dpx, T& dpy, T& dd) {
+  const T pz = pxyz_g(B, px, py);
+  const T xp = px / pz;
+  const T yp = py / pz;
+  const T term1 = 1.0 + xp * xp * (2.0 + yp * yp);
+  const T term2 = xp / (1.0 + yp * yp);
+  phi = -b0 * gtan(b0 * gK * term1 * pz - gatan(term2));
+  const T px2 = px * px, py2 = py * py, pz2 = pz * pz;
+  const T pz4 = pz2 * pz2;
+  const T py2z2 = py2 + pz2;
+  const T arg_sec = (b0 * gK * (pz4 + px2 * (py2 + 2.0 * pz2))) / (pz2 * pz) - gatan(px * pz / (py2 + pz2));
+  const T secv = 1.0 / gcos(arg_sec);
+  const T powsec = secv * secv;
+  const T py4 = py2 * py2, py6 = py4 * py2, pz5 = pz4 * pz, pz6 = pz4 * pz2;
+  const T py2z2_sq = py2z2 * py2z2;
+  const T denom = pz5 * (py4 + px2 * pz2 + 2.0 * py2 * pz2 + pz4);
+  dpx = -(b0 * (px2 * pz4 * (py2 - pz2) - pz6 * (py2 + pz2) +
+                b0 * gK * px *
+                    (pz2 * py2z2_sq * (2.0 * py2 + 3.0 * pz2) + px2 * px2 * (3.0 * py2 * pz2 + 2.0 * pz4) +
+                     px2 * (3.0 * py6 + 8.0 * py4 * pz2 + 9.0 * py2 * pz4 + 5.0 * pz6)))) *
+        powsec / denom;
+  dpy = -(b0 * py *
+          (px * pz4 * (py2 + pz2) +
+           b0 * gK * (-(pz4 * py2z2_sq) + px2 * px2 * (3.0 * py2 * pz2 + 4.0 * pz4) +
+                      px2 * (3.0 * py6 + 10.0 * py4 * pz2 + 11.0 * py2 * pz4 + 3.0 * pz6)))) *
+        powsec / denom;
+  dd = (b0 * B *
+        (px * pz4 * (py2 - pz2) + b0 * gK *
+                                      (-(pz4 * py2z2_sq) + px2 * px2 * (3.0 * py2 * pz2 + 2.0 * pz4) +
+                                       px2 * (3.0 * py6 + 8.0 * py4 * pz2 + 7.0 * py2 * pz4 + pz6)))) *
+       powsec / denom;
+}
+
+// bend_fringe as functions of (y, px, py, B): the increment of x, the new y, the decrement
+// of py and the decrement of z
+template <class T>
+TP_DEV void fringe_g(T y, T px, T py, T B, double b0, double gK, T& dxf, T& yf, T& dpyf, T& dct) {
+  T phi, dpx, dpy, dd;
+  fringe_coef_g(px, py, B, b0, gK, phi, dpx, dpy, dd);
+  yf = (2.0 * y) / (1.0 + gsqrt(1.0 - 2.0 * dpy * y));
+  dxf = 0.5 * dpx * (yf * yf);
+  dct = 0.5 * dd * (yf * yf);
+  dpyf = phi * yf;
+}
+
+// State and adjoint of one point, returned together by the out-of-line maps below (they are
+// called at most a few times per bend and turn; out of line, their temporaries do not raise
+// the register allocation of the element loop).
+struct SA {
+  St s;
+  Ad a;
+};
+
+// yrot backwards: s enters as the output and leaves as the input (yrot with -phi).  x and px
+// are replaced, y and z incremented, py and delta pass through.
+__device__ __noinline__ SA yrot_back(St s, Ad a, Cx c, double co, double si) {
+  s = yrot(s, c, co, -si);
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+#pragma nounroll
+  for (int k = 0; k < 4; ++k) {
+    D1 xo, pxo, dy, dz;
+    yrot_g(D1(s.x, k == 0), D1(s.px, k == 1), D1(s.py, k == 2), D1(c.B, k == 3), co, si, xo, pxo, dy, dz);
+    const double gk = a.x * xo.d + a.px * pxo.d + a.y * dy.d + a.z * dz.d;
+    g0 = k == 0 ? gk : g0;
+    g1 = k == 1 ? gk : g1;
+    g2 = k == 2 ? gk : g2;
+    g3 = k == 3 ? gk : g3;
+  }
+  a.x = g0;
+  a.px = g1;
+  a.py = a.py + g2;
+  a.d = a.d + g3;
+  return SA{s, a};
+}
+
+// bend_edge backwards (bend_edge with -theta), same structure as yrot
+__device__ __noinline__ SA wedge_back(St s, Ad a, Cx c, double rhoinv, double co, double si, double s2t, double theta) {
+  s = bend_edge(s, c, rhoinv, co, -si, -s2t, -theta);
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+#pragma nounroll
+  for (int k = 0; k < 4; ++k) {
+    D1 xo, pxo, dy, dz;
+    edge_g(D1(s.x, k == 0), D1(s.px, k == 1), D1(s.py, k == 2), D1(c.B, k == 3), rhoinv, co, si, s2t, theta, xo,
+           pxo, dy, dz);
+    const double gk = a.x * xo.d + a.px * pxo.d + a.y * dy.d + a.z * dz.d;
+    g0 = k == 0 ? gk : g0;
+    g1 = k == 1 ? gk : g1;
+    g2 = k == 2 ? gk : g2;
+    g3 = k == 3 ? gk : g3;
+  }
+  a.x = g0;
+  a.px = g1;
+  a.py = a.py + g2;
+  a.d = a.d + g3;
+  return SA{s, a};
+}
+
+// bend_fringe backwards: y is replaced, x incremented, py and z decremented; px and delta
+// pass through.  Five fixed-point steps for the input py (each gains ~6 digits or more).
+__device__ __noinline__ SA fringe_back(St s, Ad a, Cx c, double b0, double gK) {
+  const double yf = s.y;
+  double py = s.py, phi, dpx, dpy, dd;
+#pragma nounroll
+  for (int it = 0; it < 5; ++it) {
+    fringe_coef_g(s.px, py, c.B, b0, gK, phi, dpx, dpy, dd);
+    py = s.py + phi * yf;
+  }
+  fringe_coef_g(s.px, py, c.B, b0, gK, phi, dpx, dpy, dd);
+  s.x = s.x - 0.5 * dpx * (yf * yf);
+  s.y = yf - 0.5 * dpy * (yf * yf);
+  s.py = py;
+  s.z = s.z + 0.5 * dd * (yf * yf);
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+#pragma nounroll
+  for (int k = 0; k < 4; ++k) {
+    D1 dxf, yo, dpyf, dct;
+    fringe_g(D1(s.y, k == 0), D1(s.px, k == 1), D1(s.py, k == 2), D1(c.B, k == 3), b0, gK, dxf, yo, dpyf, dct);
+    const double gk = a.x * dxf.d + a.y * yo.d - a.py * dpyf.d - a.z * dct.d;
+    g0 = k == 0 ? gk : g0;
+    g1 = k == 1 ? gk : g1;
+    g2 = k == 2 ? gk : g2;
+    g3 = k == 3 ? gk : g3;
+  }
+  a.y = g0;
+  a.px = a.px + g1;
+  a.py = a.py + g2;
+  a.d = a.d + g3;
+  return SA{s, a};
+}
+
 }  // namespace tpk
 
 #endif  // TP_ADJOINT_H

@@ -170,6 +170,28 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
         if (fin != 0) edge_back(s, a, fin, p + 11, irho, false);
       }
       break;
+    case TP_ESBEND:
+      // ExactSBend with Forest's yrot / fringe / wedge maps, src/tracking.jl:901-973, undone
+      // in reverse order (the host refuses the drift-limit case |irho| < 1e-6)
+      if constexpr ((FEAT & FT_ESBEND) != 0) {
+        const int fin = (fl >> TP_F_FIN_SHIFT) & 0xF, fout = (fl >> TP_F_FOUT_SHIFT) & 0xF;
+        const double irho = p[0];
+        SA r{s, a};
+        if (p[20] != 0.0) r = yrot_back(r.s, r.a, c, p[21], p[22]);  // p[20] = -e2
+        if (fout != 0) r = fringe_back(r.s, r.a, c, -irho, p[23]);
+        r = wedge_back(r.s, r.a, c, irho, p[17], p[18], p[19], p[20]);
+        s = r.s;
+        a = r.a;
+#define TP_CALL(KK) esbend_body_back<KK>(s, a, c, p, ns, m)
+        TP_KICK_SWITCH(FEAT, TP_CALL)
+#undef TP_CALL
+        r = wedge_back(s, a, c, irho, p[13], p[14], p[15], p[16]);
+        if (fin != 0) r = fringe_back(r.s, r.a, c, irho, p[23]);
+        if (p[16] != 0.0) r = yrot_back(r.s, r.a, c, p[11], p[12]);  // p[16] = -e1
+        s = r.s;
+        a = r.a;
+      }
+      break;
     case TP_RFCAV:
       if constexpr ((FEAT & FT_RF) != 0) {
         const bool dr = (fl & TP_F_RF_DRIFT) != 0;
@@ -381,6 +403,8 @@ __global__ void reduce_kernel(const double* __restrict__ partial, int nblocks, i
     if ((waves) == 3) { CALL(3); }                            \
     else if ((waves) == 4) { CALL(4); }                       \
     else { CALL(0); }                                         \
+  } else if constexpr (FEAT == TP_FEAT_ADJ_ALL) {            \
+    CALL(3);                                                  \
   } else {                                                    \
     CALL(0);                                                  \
   }
@@ -419,10 +443,11 @@ static const void* back_fn(int waves) {
 
 hipError_t TP_BATCH_BACK_LAUNCH(const TpBatchArgs& b, hipStream_t stream) {
   if (b.n_items <= 0) return hipSuccess;
-  switch (b.variant) {
+  switch (b.back_variant) {
     case TP_VARIANT_A: tpk::TPK_BUILD::launch_back_v<TP_FEAT_A>(b, stream); break;
     case TP_VARIANT_B: tpk::TPK_BUILD::launch_back_v<TP_FEAT_B>(b, stream); break;
-    default: tpk::TPK_BUILD::launch_back_v<TP_FEAT_C>(b, stream); break;
+    case TP_VARIANT_C: tpk::TPK_BUILD::launch_back_v<TP_FEAT_C>(b, stream); break;
+    default: tpk::TPK_BUILD::launch_back_v<TP_FEAT_ADJ_ALL>(b, stream); break;
   }
   return hipGetLastError();
 }
@@ -433,7 +458,8 @@ int TP_BATCH_BACK_OCC(int variant, int block, int max_slots, int waves) {
   switch (variant) {
     case TP_VARIANT_A: fn = tpk::TPK_BUILD::back_fn<TP_FEAT_A>(waves); break;
     case TP_VARIANT_B: fn = tpk::TPK_BUILD::back_fn<TP_FEAT_B>(waves); break;
-    default: fn = tpk::TPK_BUILD::back_fn<TP_FEAT_C>(waves); break;
+    case TP_VARIANT_C: fn = tpk::TPK_BUILD::back_fn<TP_FEAT_C>(waves); break;
+    default: fn = tpk::TPK_BUILD::back_fn<TP_FEAT_ADJ_ALL>(waves); break;
   }
   int nb = 0;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, lds) == hipSuccess ? nb : 0;
@@ -447,7 +473,8 @@ hipError_t TP_ADJ_LAUNCH(const TpAdjArgs& a, hipStream_t stream) {
   switch (a.variant) {
     case TP_VARIANT_A: tpk::TPK_BUILD::launch_v<TP_FEAT_A>(a, grid, stream); break;
     case TP_VARIANT_B: tpk::TPK_BUILD::launch_v<TP_FEAT_B>(a, grid, stream); break;
-    default: tpk::TPK_BUILD::launch_v<TP_FEAT_C>(a, grid, stream); break;
+    case TP_VARIANT_C: tpk::TPK_BUILD::launch_v<TP_FEAT_C>(a, grid, stream); break;
+    default: tpk::TPK_BUILD::launch_v<TP_FEAT_ADJ_ALL>(a, grid, stream); break;
   }
   return hipGetLastError();
 }

@@ -35,6 +35,9 @@ enum : unsigned {
 #define TP_FEAT_B (FT_ESBEND_LE | FT_RF)                          // rings of exact bends + RF
 #define TP_FEAT_C (FT_ESBEND_LE | FT_RF | FT_SBEND | FT_GENKICK)  // + high-order multipoles
 #define TP_FEAT_ALL 255u
+// The reverse kernels' largest variant: every family the adjoint covers (no linearised
+// drift, no tags beyond TP_RFCAV's group); the host selects it only for Forest exact bends.
+#define TP_FEAT_ADJ_ALL (TP_FEAT_C | FT_ESBEND | FT_MISALIGN)
 enum { TP_VARIANT_A = 0, TP_VARIANT_B = 1, TP_VARIANT_C = 2, TP_VARIANT_ALL = 3 };
 
 struct TpKArgs {
@@ -107,7 +110,8 @@ struct TpBatchArgs {
   double* item_loss;     // per item: loss sum, survivors (adjoint)
   double* item_partial;  // per item: max_slots gradient sums (adjoint)
   int32_t max_slots;
-  int32_t variant;
+  int32_t variant;       // forward kernel variant
+  int32_t back_variant;  // reverse kernel variant
   int32_t exact_h;
   int32_t grid;
   int32_t block;
