@@ -169,7 +169,10 @@ int tp_track_host(tp_ctx* ctx, const tp_table* t, double* coords, int64_t n, int
  * Supported: drifts, thick multipoles, thin multipoles, SBend, the reference's ExactSBend
  * (yrot, Forest fringe, wedges; not in its drift limit |angle/L| < 1e-6), exact bends with
  * linear edges, RF cavities, Marker, t1/t2 translations, exact Hamiltonian; other tables
- * return TP_EUNSUPPORTED.  flags: TP_TRACK_FAST / TP_TRACK_TIMED as for tp_track. */
+ * return TP_EUNSUPPORTED.  A time-varying table runs each turn on its row (turns outside the
+ * rows: TP_EINVAL); dL_dk of a position is then the derivative with respect to an offset
+ * added to its strength at every turn.  flags: TP_TRACK_FAST / TP_TRACK_TIMED as for
+ * tp_track. */
 int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns,
                int32_t ckpt_stride, uint32_t flags, double* loss, int64_t* n_alive, double* dL_dk,
                double* dL_dx0);

@@ -100,7 +100,8 @@ def test_dba_ring_with_ramped_sextupoles(ctx):
     assert np.max(np.abs(sc[alive] - gc[alive])) > 1e-9
 
 
-def test_rows_bound_the_turn_range_and_adjoint_refuses(ctx):
+def test_rows_bound_the_turn_range(ctx):
+    """Tracking, the adjoint and Jacobians refuse turns outside the table's rows."""
     beam = Beam(3.0e9)
     lat = _ref_quad_line()
     tab = ctx.table(compile_table(lat, beam, turn=4, nturns=3, time=0.0, dt_turn=0.1))
@@ -108,13 +109,78 @@ def test_rows_bound_the_turn_range_and_adjoint_refuses(ctx):
     track_(p, tab, nturns=3, turn=4)  # inside the rows
     with pytest.raises(TrackPadHIPError, match="TP_EINVAL"):
         track_(p, tab, nturns=3, turn=5)
-    from trackpad_hip import _lib as L
     import ctypes as C
 
     q = HIPParticles.from_coords(np.zeros((8, 6)), ctx=ctx)
     loss, na = C.c_double(), C.c_int64()
+    rc = ctx._lib.tp_adjoint(ctx.handle, tab.handle, q.handle, 5, 3, 1, 0, C.byref(loss), C.byref(na), None, None)
+    assert rc == -1 and b"time-varying rows" in ctx._lib.tp_last_error()
     rc = ctx._lib.tp_adjoint(ctx.handle, tab.handle, q.handle, 4, 3, 1, 0, C.byref(loss), C.byref(na), None, None)
-    assert rc == -2 and b"time-varying" in ctx._lib.tp_last_error()
+    assert rc == 0 and na.value == 8
+
+
+def _ramped_ring():
+    """FODO ring (8 cells, 4 Yoshida steps) whose focusing quads ramp with the turn, whose
+    first defocusing quad wobbles in time (with a step count that changes at t = 0.3), with
+    a sextupole and an octupole ramped in time."""
+    from trackpad_hip import Octupole
+    base = W.fodo_lattice()
+    els = list(base.elements)
+    for i, e in enumerate(els):
+        if e.name == "QF":
+            els[i] = timed(e, k1=lambda t, n, k0=e.k1: k0 * (1.0 + 0.004 * n))
+    q = els[2]
+    els[2] = timed(q, k1=lambda t, n, k0=q.k1: k0 * (1.0 + 0.01 * math.sin(7.0 * t)),
+                   num_int_steps=lambda t, n: 4 if t < 0.3 else 6)
+    els[5] = timed(Sextupole(0.2, 5.0, name="S"), k2=lambda t, n: 5.0 + 20.0 * t)
+    els[9] = timed(Octupole(0.2, 50.0, name="O"), k3=lambda t, n: 50.0 * math.cos(t))
+    return Lattice(els)
+
+
+@pytest.mark.parametrize("fast", [False, True])
+def test_adjoint_time_varying(ctx, fast):
+    """Reverse mode over a time-varying table (one row per turn): the gradient equals that of
+    the static lattice made of the T materialised turns in a row, tracked for one turn, with
+    each position's strength gradient summed over its T copies -- the derivative with
+    respect to an offset added to the element's strength at every turn -- and the oracle's
+    dual-number derivatives of that unrolled lattice along random directions (relative
+    1e-9).  With one checkpoint segment both runs undo the same maps from the same end state,
+    so dL/dx0 agrees bit for bit in the parity build; with segments of 4 turns the reverse
+    pass restarts from stored states (reconstruction differs in the last bits: 1e-12)."""
+    from trackpad_hip import materialize_lattice, ringpass_grad, strength_positions
+    beam = Beam(W.ENERGY)
+    lat = _ramped_ring()
+    n, T, turn, time, dt = 48, 6, 3, 0.2, 0.05
+    c0 = W.gaussian_particles(n, 53, 1e-3, 1e-4, 1e-3, 1e-4, 1e-3)
+    g = ringpass_grad(c0, lat, beam, T, turn=turn, time=time, dt_turn=dt, ckpt_stride=4, fast=fast, ctx=ctx)
+    assert g.n_alive == n
+    ns = len(lat)
+    els, t = [], time
+    for k in range(T):  # time accumulated per turn, as ringpass! and the table writer do
+        els += materialize_lattice(lat, time=t, turn=turn + k).elements
+        t += dt
+    unrolled = Lattice(els)
+    u = ringpass_grad(c0, unrolled, beam, 1, fast=fast, ctx=ctx)
+    assert u.n_alive == n and abs(g.loss - u.loss) <= 1e-14 * abs(u.loss)
+    assert np.max(np.abs(g.dx0 - u.dx0)) <= 1e-12 * np.max(np.abs(u.dx0))
+    g1 = ringpass_grad(c0, lat, beam, T, turn=turn, time=time, dt_turn=dt, ckpt_stride=T, fast=fast, ctx=ctx)
+    if fast:
+        assert np.max(np.abs(g1.dx0 - u.dx0)) <= 1e-12 * np.max(np.abs(u.dx0))
+    else:
+        assert np.array_equal(g1.dx0, u.dx0)
+    summed = u.dk.reshape(T, ns).sum(axis=0)
+    pos = strength_positions(lat)
+    assert len(pos) == 18 and np.all(g.dk[np.setdiff1d(np.arange(ns), pos)] == 0.0)
+    assert np.max(np.abs(g.dk - summed)) <= 1e-12 * np.max(np.abs(summed))
+    rng = np.random.default_rng(9)
+    for trial in range(2):
+        dk = np.zeros(ns)
+        dk[pos] = rng.normal(0, 1, len(pos))
+        dr0 = rng.normal(0, 1, (n, 6)) * np.array([5e-5, 2e-5, 1e-5, 2e-6, 1e-4, 1e-3])
+        _, dl, na = O.loss_dir(unrolled, beam, c0, 1, dk=np.tile(dk, T), dr0=dr0)
+        ref = dl / na
+        got = float(np.dot(g.dk, dk) + np.sum(g.dx0 * dr0))
+        assert abs(got - ref) <= 1e-9 * abs(ref), (trial, got, ref)
 
 
 def test_batch_with_time_varying_variant(ctx):
@@ -133,5 +199,12 @@ def test_batch_with_time_varying_variant(ctx):
         assert np.array_equal(r.download()[0], p.download()[0])
     with pytest.raises(TrackPadHIPError, match="TP_EINVAL"):
         b.track([5, 5], [0, 0])  # variant 0 has rows for turns 4..8 only
-    with pytest.raises(TrackPadHIPError, match="TP_EUNSUPPORTED"):
-        b.adjoint([5, 5], [4, 0])
+    # the batched adjoint against the per-variant calls (parity build: dL/dx0 bit for bit)
+    from trackpad_hip import ringpass_grad
+    parts = [HIPParticles.from_coords(c0, ctx=ctx) for _ in range(2)]
+    b = HIPBatch(tabs, parts)
+    gb = b.adjoint([5, 5], [4, 0], ckpt_stride=2)
+    for v, (l, t0) in enumerate(((lat, 4), (static, 0))):
+        r = ringpass_grad(c0, l, beam, 5, turn=t0, time=0.0, dt_turn=0.1, ckpt_stride=2, ctx=ctx)
+        assert np.array_equal(gb.dx0[v], r.dx0)
+        assert np.max(np.abs(gb.dk[v] - r.dk)) <= 1e-12 * np.max(np.abs(r.dk))

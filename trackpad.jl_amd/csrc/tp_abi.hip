@@ -639,9 +639,21 @@ int adj_variant(unsigned features, int variant) {
   return variant > TP_VARIANT_C ? TP_VARIANT_C : variant;
 }
 
+// A time-varying table's strength slots are those of row 0 (per position); the reverse pass
+// adds each row's kick derivative to its position's slot, i.e. the gradient is that of an
+// offset added to the element's strength at every turn.  That needs the main polynomial
+// order of every position to be the same in every row.
+bool rows_share_slots(const tp_table* t) {
+  const size_t n_seq = t->hdr.n_seq, rows = t->hdr.n_rows > 1 ? t->hdr.n_rows : 1;
+  for (size_t r = 1; r < rows; ++r)
+    for (size_t e = 0; e < n_seq; ++e)
+      if (t->recs[t->seq[r * n_seq + e]].gorder != t->recs[t->seq[e]].gorder) return false;
+  return true;
+}
+
 // Checks shared by the reverse-mode entry points.
-int adj_check(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t nturns, int32_t ckpt_stride, uint32_t flags,
-              const char* who) {
+int adj_check(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, int32_t ckpt_stride,
+              uint32_t flags, const char* who) {
   if (!ctx || !t || !p) return fail(TP_EINVAL, "%s: NULL argument", who);
   if (t->ctx != ctx || p->ctx != ctx) return fail(TP_EINVAL, "%s: table and particles belong to another context", who);
   if (ctx->closed) return fail(TP_EINVAL, "%s: context destroyed", who);
@@ -656,7 +668,14 @@ int adj_check(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t nturns, i
     if (r.tag == TP_ESBEND && (r.flags & TP_F_DRIFTLIMIT))
       return fail(TP_EUNSUPPORTED, "%s: exact bends with |angle/L| < 1e-6 (drift limit) have no adjoint here", who);
   }
-  if (t->hdr.n_rows > 1) return fail(TP_EUNSUPPORTED, "%s: time-varying tables have no adjoint here", who);
+  if (t->hdr.n_rows > 1) {
+    const int64_t r0 = t->hdr.row_turn0, r1 = r0 + (int64_t)t->hdr.n_rows;
+    if ((int64_t)turn0 < r0 || (int64_t)turn0 + nturns > r1)
+      return fail(TP_EINVAL, "%s: turns [%d, %lld) outside the table's time-varying rows [%lld, %lld)", who, turn0,
+                  (long long)turn0 + nturns, (long long)r0, (long long)r1);
+    if (!rows_share_slots(t))
+      return fail(TP_EUNSUPPORTED, "%s: a position changes its main multipole order between rows", who);
+  }
   return set_dev(ctx);
 }
 
@@ -683,7 +702,7 @@ extern "C" {
 
 int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, int32_t ckpt_stride,
                uint32_t flags, double* loss, int64_t* n_alive, double* dL_dk, double* dL_dx0) {
-  int rc = adj_check(ctx, t, p, nturns, ckpt_stride, flags, "tp_adjoint");
+  int rc = adj_check(ctx, t, p, turn0, nturns, ckpt_stride, flags, "tp_adjoint");
   if (rc) return rc;
   tp_table* tm = const_cast<tp_table*>(t);
   if (!tm->d_slots) {
@@ -758,6 +777,8 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
   a.n_slots = n_slots;
   a.variant = adj_variant(t->features, t->variant);
   a.waves = ctx->back_waves;
+  a.n_rows = t->hdr.n_rows > 1 ? (int32_t)t->hdr.n_rows : 1;
+  a.row_turn0 = t->hdr.row_turn0;
   a.beti = t->hdr.beti;
   // the reverse pass seeds dL/dx = 2x (the gradient of the loss sum); the 1/N_alive of the
   // mean is applied to the results below
@@ -766,6 +787,7 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
     for (int32_t j = nseg - 1; j >= 0; --j) {
       a.end_state = (const double*)ckpt.p + (size_t)j * 6 * plane;
       a.nturns = (j == nseg - 1) ? (nturns - j * K) : K;
+      a.turn_last = turn0 + j * K + a.nturns - 1;
       a.init_lambda = (j == nseg - 1) ? 1 : 0;
       hipError_t le = fast ? tp_launch_adjoint_fast(a, st) : tp_launch_adjoint_exact(a, st);
       if (le != hipSuccess) return fail(TP_EHIP, "tp_adjoint: reverse kernel launch failed: %s", hipGetErrorName(le));
@@ -807,7 +829,7 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
 
 int tp_jacobian(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, int32_t ckpt_stride,
                 uint32_t flags, double* jac) {
-  int rc = adj_check(ctx, t, p, nturns, ckpt_stride, flags, "tp_jacobian");
+  int rc = adj_check(ctx, t, p, turn0, nturns, ckpt_stride, flags, "tp_jacobian");
   if (rc) return rc;
   const int64_t n = p->n;
   const int32_t K = ckpt_stride > 0 ? ckpt_stride : 1;
@@ -843,6 +865,8 @@ int tp_jacobian(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, 
   a.n_slots = 0;
   a.variant = adj_variant(t->features, t->variant);
   a.waves = ctx->back_waves;
+  a.n_rows = t->hdr.n_rows > 1 ? (int32_t)t->hdr.n_rows : 1;
+  a.row_turn0 = t->hdr.row_turn0;
   a.beti = t->hdr.beti;
   std::vector<double> row(6 * plane);
   float bwd_ms = 0.f;
@@ -852,6 +876,7 @@ int tp_jacobian(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, 
     for (int32_t j = nseg - 1; j >= 0; --j) {
       a.end_state = (const double*)ckpt.p + (size_t)j * 6 * plane;
       a.nturns = (j == nseg - 1) ? (nturns - j * K) : K;
+      a.turn_last = turn0 + j * K + a.nturns - 1;
       a.init_lambda = (j == nseg - 1) ? 2 + i : 0;
       hipError_t le = fast ? tp_launch_adjoint_fast(a, st) : tp_launch_adjoint_exact(a, st);
       if (le != hipSuccess) return fail(TP_EHIP, "tp_jacobian: reverse kernel launch failed: %s", hipGetErrorName(le));
@@ -1161,7 +1186,7 @@ int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_part
     b->total_slots += ns;
     b->max_slots = std::max(b->max_slots, ns);
     if (b->no_adjoint.empty()) {
-      if (t->hdr.n_rows > 1) b->no_adjoint = "time-varying tables have no adjoint here";
+      if (!rows_share_slots(t)) b->no_adjoint = "a position changes its main multipole order between rows";
       if (t->features & (FT_LINH | FT_EXTRA))
         b->no_adjoint = "the linearised drift and the element families beyond drifts, multipoles, bends and RF "
                         "cavities have no adjoint here";

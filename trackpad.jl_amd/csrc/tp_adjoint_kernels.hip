@@ -215,12 +215,17 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
 // nturns whole turns backwards for one lane: s / l enter as the state and adjoint at the
 // end and leave as those at the start.  The strength gradient of every position with a
 // slot is summed over the wave and added to the wave's LDS row by lane 0 (no strength
-// gradients when slot_of_pos is null).
+// gradients when slot_of_pos is null).  A time-varying table (n_rows > 1) holds the
+// lattice of turn r + row_turn0 in sequence row r; the turns undone here are turn_last,
+// turn_last - 1, ..., each with its own row.
 template <unsigned FEAT>
-TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
+TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq_rows, const tp_rec* __restrict__ recs,
                        const double* __restrict__ pool, const int32_t* __restrict__ slot_of_pos, int32_t n_seq,
-                       int32_t nturns, double beti, double* row, int lane) {
+                       int32_t nturns, int32_t turn_last, int32_t n_rows, int32_t row_turn0, double beti, double* row,
+                       int lane) {
   for (int32_t t = 0; t < nturns; ++t) {
+    const int32_t* __restrict__ seq =
+        seq_rows + (n_rows > 1 ? (int64_t)(turn_last - t - row_turn0) * n_seq : 0);
     // scalar loads pipelined as in the forward element loop: the record and slot of
     // element e-1 and the sequence entry of e-2 are requested while element e runs
     int32_t ri_next = n_seq > 1 ? seq[n_seq - 2] : 0;
@@ -285,7 +290,8 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void adjoint_kernel(const int32_
   }
   Cx c;
   mkctx(s, c, a.beti);
-  back_turns<FEAT>(s, l, c, seq, recs, pool, slot_of_pos, a.n_seq, a.nturns, a.beti, lds + wave * a.n_slots, lane);
+  back_turns<FEAT>(s, l, c, seq, recs, pool, slot_of_pos, a.n_seq, a.nturns, a.turn_last, a.n_rows, a.row_turn0,
+                   a.beti, lds + wave * a.n_slots, lane);
   if (i < a.n) {
     double* r = lam + i;
     r[0] = l.x;
@@ -346,7 +352,7 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_back_kernel(const TpV
       mkctx(s, c, d.beti);
       const int32_t nt = (j == d.nseg - 1) ? d.nturns - j * d.ckpt_stride : d.ckpt_stride;
       back_turns<FEAT>(s, l, c, seq + d.seq_off, recs + d.rec_off, pool + d.pool_off, slots + d.pos_off, d.n_seq, nt,
-                       d.beti, row, lane);
+                       d.turn0 + j * d.ckpt_stride + nt - 1, d.n_rows, d.row_turn0, d.beti, row, lane);
     }
     if (li < d.n) {
       double* r = d.lam + li;

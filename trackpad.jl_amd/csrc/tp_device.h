@@ -138,6 +138,9 @@ struct TpAdjArgs {
                                // loss sum, 2x / 2y; 2 + i: start from the unit vector e_i
   int32_t variant;
   int32_t waves;               // register budget: 0 (compiler's), 3 or 4 waves per SIMD
+  int32_t turn_last;           // turn number of the segment's last turn
+  int32_t n_rows;              // sequence rows of a time-varying table (1 = static)
+  int32_t row_turn0;           // turn of row 0
   double beti;
 };
 

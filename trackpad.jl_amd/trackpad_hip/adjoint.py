@@ -5,7 +5,10 @@
 returns ``L = mean over surviving particles of (x^2 + y^2)`` after ``nturns`` turns and
 ``grads.dk`` (dL/dk of each lattice position's main strength: k1 of a Quadrupole, k2 of a
 Sextupole, k3 of an Octupole, 0 elsewhere) and ``grads.dx0`` (dL/d initial coordinates,
-N x 6).  The reference has no adjoint (its Enzyme extension is a stub,
+N x 6).  A lattice with time-varying elements is resolved per turn as in tracking (the
+table holds one row per turn); the gradient of a time-varying magnet's strength is that of
+an offset added to its strength at every turn.  The reference has no adjoint (its Enzyme
+extension is a stub,
 ext/TrackPadEnzymeExt.jl:14-17); this is the rule a Julia ChainRulesCore ``rrule`` /
 Enzyme custom rule wraps, and :class:`TrackLoss` wraps it as a ``torch.autograd.Function``.
 """
@@ -19,7 +22,7 @@ from typing import Optional
 import numpy as np
 
 from . import _lib as L
-from .elements import Octupole, Quadrupole, Sextupole
+from .elements import Octupole, Quadrupole, Sextupole, TimeVaryingElement
 from .lattice import Beam, Lattice
 from .table import compile_table
 from .tracking import HIPContext, HIPParticles, default_context
@@ -40,9 +43,12 @@ class Gradients:
 
 
 def strength_positions(lat: Lattice) -> np.ndarray:
-    """0-based positions whose element has a strength gradient (quads, sextupoles, octupoles)."""
-    return np.array([i for i, e in enumerate(lat.elements) if isinstance(e, (Quadrupole, Sextupole, Octupole))],
-                    dtype=np.int64)
+    """0-based positions whose element has a strength gradient (quads, sextupoles, octupoles,
+    static or time-varying)."""
+    def magnet(e):
+        e = e.base if isinstance(e, TimeVaryingElement) else e
+        return isinstance(e, (Quadrupole, Sextupole, Octupole))
+    return np.array([i for i, e in enumerate(lat.elements) if magnet(e)], dtype=np.int64)
 
 
 def _strength_attr(e) -> str:
@@ -63,9 +69,10 @@ def set_strengths(lat: Lattice, positions, values) -> Lattice:
 
 def ringpass_grad(coords0, lat: Lattice, beam: Beam, nturns: int, *, turn: int = 0, ckpt_stride: int = 1,
                   fast: bool = False, timed: bool = False, ctx: Optional[HIPContext] = None,
-                  aperture: Optional[float] = None) -> Gradients:
+                  aperture: Optional[float] = None, time: float = 0.0, dt_turn: float = 0.0) -> Gradients:
     ctx = ctx or default_context()
-    tab = ctx.table(compile_table(lat, beam, aperture=aperture))
+    tab = ctx.table(compile_table(lat, beam, aperture=aperture, turn=turn, nturns=nturns, time=time,
+                                  dt_turn=dt_turn))
     p = HIPParticles.from_coords(coords0, ctx=ctx)
     n = p.n
     loss = C.c_double()

@@ -36,11 +36,14 @@ _PD = C.POINTER(C.c_double)
 
 
 def jacobians(coords, lat: Lattice, beam: Beam, nturns: int = 1, *, turn: int = 0, ckpt_stride: int = 1,
-              fast: bool = False, ctx: Optional[HIPContext] = None, aperture: Optional[float] = None):
+              fast: bool = False, ctx: Optional[HIPContext] = None, aperture: Optional[float] = None,
+              time: float = 0.0, dt_turn: float = 0.0):
     """(J, final, lost_flags): J[q] = d(coordinates after nturns) / d(initial coordinates) of
-    particle q (N x 6 x 6; zero for lost particles), with the tracked beam and loss flags."""
+    particle q (N x 6 x 6; zero for lost particles), with the tracked beam and loss flags.
+    Time-varying elements are resolved per turn, as in tracking."""
     ctx = ctx or default_context()
-    tab = ctx.table(compile_table(lat, beam, aperture=aperture))
+    tab = ctx.table(compile_table(lat, beam, aperture=aperture, turn=turn, nturns=nturns, time=time,
+                                  dt_turn=dt_turn))
     c0 = np.asarray(coords, dtype=np.float64)
     if c0.ndim == 1:
         c0 = c0.reshape(1, 6)
