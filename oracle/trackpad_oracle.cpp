@@ -18,9 +18,11 @@
 //   (straight-line drift geometry, complex power sums, circle-arc geometry) in
 //   tests/golden/, 4th-order convergence of the Yoshida composition, det(J)=1 and
 //   reversibility invariants.
-//   bend_fringe (Forest 13.13) and yrot: pinned only by the 50-digit restatement of the
-//   same formula plus the det(J)=1 invariant; the JuTrack values the reference tests
-//   compare against are not available ("parity unpinned" for last-ulp agreement).
+//   bend_fringe (Forest 13.13), yrot and bend_edge: the 50-digit restatement of the same
+//   formulas, J^T S J = S of the whole exact bend (dual numbers), and the fringe's x/y/z
+//   updates as the derivatives of its py kick at 50 digits; the JuTrack values the
+//   reference tests compare against are not available (sign conventions relative to
+//   JuTrack: parity unpinned).
 //   r1/r2 rotation summation order (StaticArrays SMatrix*SVector): parity unpinned.
 //   Julia Base.Math sin/cos/tan/asin/atan last-ulp values: parity unpinned (glibc here).
 //

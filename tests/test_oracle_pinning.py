@@ -11,8 +11,10 @@
 2. Closed forms: Yoshida coefficients, Beam, 4th-order convergence of the Yoshida thick
    quad, Horner kicks vs complex evaluation up to order 10, exact dipole body n=0 vs n=10,
    drift composition.
-3. Invariants: det of the one-turn Jacobian = 1, reversibility of the symmetric Yoshida
-   step, on-axis particles stay on axis, k = 0 quad == drift.
+3. Invariants: det of the one-turn Jacobian = 1, J^T S J = S for the reference's exact bend
+   with Forest's yrot / fringe / wedge maps and for a ring of them, the fringe's updates as
+   the derivatives of its kick, reversibility of the symmetric Yoshida step, on-axis
+   particles stay on axis, k = 0 quad == drift.
 4. Loss-boundary semantics of check_lost (src/tracking.jl:49-52) and the aperture.
 """
 
@@ -194,6 +196,81 @@ def test_symplectic_det(name):
     _, J, lost = O.jacobian(lat, beam, r0, 1)
     assert not lost
     assert abs(np.linalg.det(J) - 1.0) < 1e-12, np.linalg.det(J) - 1.0
+
+
+# (x, px), (y, py) and (delta, z) are the canonical pairs of the reference's coordinates:
+# its drift advances z by NormL (beti + delta) - L beti, i.e. z is conjugate to -delta.
+_S6 = np.zeros((6, 6))
+for _i, (_a, _b) in enumerate(((0, 1), (2, 3), (5, 4))):
+    _S6[_a, _b], _S6[_b, _a] = 1.0, -1.0
+
+
+@pytest.mark.parametrize("gk,e1,e2,r0", [
+    (0.0, 0.05, 0.03, [1e-3, 2e-3, -5e-4, 1e-3, 0.0, 1e-3]),
+    (0.02, 0.05, 0.03, [1e-3, 2e-3, -5e-4, 1e-3, 0.0, 1e-3]),
+    (0.1, -0.04, 0.07, [-2e-3, -4e-3, 3e-3, -2e-3, 1e-4, -5e-3]),
+    (0.05, 0.1, 0.0, [5e-3, 1e-2, -4e-3, 6e-3, 0.0, 2e-2]),
+])
+def test_forest_exact_bend_symplectic(gk, e1, e2, r0):
+    """The oracle's ExactSBend -- yrot, Forest fringe, wedge, exact body with Yoshida kicks,
+    mirrored exit -- is a symplectic map: J^T S J = S for its dual-number Jacobian, to 1e-13.
+    This pins yrot, bend_fringe and bend_edge beyond det J = 1 (15 independent conditions
+    instead of one): e.g. the fringe's x and z updates must carry exactly the derivatives of
+    its py kick with respect to px and delta (checked at 50 digits below)."""
+    beam = Beam(3e9)
+    lat = Lattice([ExactSBend(1.0, 0.2, e1, e2, gk=gk, polynom_b=[0.0, 0.1, 2.0, 0.0], max_order=2)])
+    _, J, lost = O.jacobian(lat, beam, np.array(r0), 1)
+    assert not lost
+    d = J.T @ _S6 @ J - _S6
+    assert np.max(np.abs(d)) <= 1e-13 * np.max(np.abs(J)) ** 2, np.max(np.abs(d))
+    # the map is not the identity's neighbour trivially: the fringe and wedges act
+    assert np.max(np.abs(J - np.eye(6))) > 0.1
+
+
+def test_forest_ring_one_turn_map_symplectic():
+    """The one-turn map of a ring of Forest bends, quads, a sextupole and RF: J^T S J = S."""
+    import lattices as LT
+    beam = Beam(3e9)
+    lat = LT.forest_ring(3e9)
+    for r0 in ([1e-3, 1e-4, -1e-3, 2e-4, 0.0, 1e-3], [0.0] * 6):
+        _, J, lost = O.jacobian(lat, beam, np.array(r0), 3)
+        assert not lost
+        d = J.T @ _S6 @ J - _S6
+        assert np.max(np.abs(d)) <= 1e-12 * np.max(np.abs(J)) ** 2, np.max(np.abs(d))
+
+
+def test_fringe_updates_are_derivatives_of_its_kick():
+    """Forest's fringe (src/tracking.jl:752-815) moves py by -phi yf and x, z by dpx yf^2/2 and
+    -dd yf^2/2 with y -> yf = 2y / (1 + sqrt(1 - 2 dpy y)).  The map is symplectic exactly when
+    dpx, dpy, dd are the partial derivatives of phi(px, py, dp1) with respect to px, py and
+    dp1 = beti + delta; the reference's long closed forms for them are checked against
+    50-digit derivatives of its phi, as restated in the golden-vector writer (phi, dpx, dpy,
+    dd are read back from the map's outputs).  A single wrong coefficient in any of the four
+    closed forms breaks the identity by many orders of magnitude."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "make_golden_mpmath", os.path.join(os.path.dirname(__file__), "golden", "make_golden_mpmath.py"))
+    G = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(G)
+    mp.mp.dps = 50
+    bt = mp.mpf(1) + mp.mpf("1.4e-8")
+    y0 = mp.mpf("2e-3")
+
+    def coeffs(px, py, d, b0, gk):
+        x1, _, yf, py1, z1, _ = G.bend_fringe([mp.mpf(0), px, y0, py, mp.mpf(0), d], b0, gk, bt)
+        phi = (py - py1) / yf
+        return phi, 2 * x1 / yf ** 2, 2 * (yf - y0) / yf ** 2, -2 * z1 / yf ** 2
+
+    for px, py, d, b0, gk in ((mp.mpf("3e-3"), mp.mpf("-2e-3"), mp.mpf("1e-3"), mp.mpf("0.1"), mp.mpf("0.02")),
+                              (mp.mpf("-1e-2"), mp.mpf("5e-3"), mp.mpf("-2e-2"), mp.mpf("-0.3"), mp.mpf("0.1")),
+                              (mp.mpf("2e-3"), mp.mpf("1e-3"), mp.mpf("0"), mp.mpf("0.05"), mp.mpf("0"))):
+        phi, dpx, dpy, dd = coeffs(px, py, d, b0, gk)
+        f = lambda a, b, c: coeffs(a, b, c, b0, gk)[0]
+        gx = mp.diff(lambda a: f(a, py, d), px)
+        gy = mp.diff(lambda b: f(px, b, d), py)
+        gd = mp.diff(lambda c: f(px, py, c), d)
+        for got, ref in ((dpx, gx), (dpy, gy), (dd, gd)):
+            assert abs(got - ref) <= mp.mpf("1e-30") * (abs(ref) + mp.mpf("1e-30")), (got, ref)
 
 
 def test_on_axis_stays_on_axis():
