@@ -205,6 +205,40 @@ def test_batch_adjoint_forest_variant(ctx, fast):
         assert np.all(np.abs(g.dk[v] - r.dk) <= 1e-12 * scale), (v, np.max(np.abs(g.dk[v] - r.dk)) / scale)
 
 
+@pytest.mark.parametrize("fast", [False, True])
+def test_batch_adjoint_every_family(ctx, fast):
+    """A batch holding the ring of every family (rolled quadrupoles among them: the batch
+    carries the rotation inverses at the concatenated pool offsets), a DBA variant and the
+    Forest ring: each variant's gradients equal its own call."""
+    import lattices as LT
+    beam = _beam()
+    lats = [_variants(1, seed=11)[0], LT.families_ring(), LT.forest_ring(W.ENERGY)]
+    ns = [64, 70, 40]
+    ts = np.array([3, 4, 2], np.int32)
+    rng = np.random.default_rng(91)
+    coords = [_coords(ns[0], seed=81),
+              rng.normal(0, 1, (ns[1], 6)) * np.array([3e-4, 3e-5, 3e-4, 3e-5, 1e-3, 5e-4]),
+              W.gaussian_particles(ns[2], 34, 2e-4, 2e-5, 2e-4, 2e-5, 1e-3)]
+    tabs = [ctx.table(compile_table(l, beam)) for l in lats]
+    parts = [HIPParticles.from_coords(c, None, ctx) for c in coords]
+    b = HIPBatch(tabs, parts)
+    g = b.adjoint(ts, ckpt_stride=2, fast=fast)
+    for v, (lat, c, t) in enumerate(zip(lats, coords, ts)):
+        r = ringpass_grad(c, lat, beam, int(t), ckpt_stride=2, fast=fast, ctx=ctx)
+        assert g.n_alive[v] == r.n_alive == ns[v]
+        assert abs(g.loss[v] - r.loss) <= 1e-13 * abs(r.loss), v
+        if fast:
+            # the DBA variant runs the all-families reverse kernel in the batch and the B kernel
+            # alone: FMA contraction differs between the two, within the adjoint's 1e-9
+            scale = np.max(np.abs(r.dx0), axis=0, keepdims=True)
+            assert np.all(np.abs(g.dx0[v] - r.dx0) <= 1e-9 * scale), v
+        else:
+            assert np.array_equal(g.dx0[v], r.dx0), v
+        scale = np.max(np.abs(r.dk))
+        tol = 1e-9 if fast else 1e-12
+        assert np.all(np.abs(g.dk[v] - r.dk) <= tol * scale), (v, np.max(np.abs(g.dk[v] - r.dk)) / scale)
+
+
 def test_batch_adjoint_vs_quad_precision_fd(ctx):
     """One variant of a batch (DBA, 8 particles, 30 turns, checkpoints every 7 turns):
     central differences with h = 1e-7 in quad precision agree to 1e-7 with the batched
@@ -247,8 +281,8 @@ def test_batch_adjoint_vs_quad_precision_fd(ctx):
 def test_batch_refusals_and_lifetime(ctx):
     beam = _beam()
     ring = _variants(1, seed=12)[0]
-    from trackpad_hip import Solenoid
-    other = Lattice([Drift(1.0), Solenoid(0.5, 0.1), Quadrupole(0.4, 1.0)])
+    from trackpad_hip import Wiggler
+    other = Lattice([Drift(1.0), Wiggler(0.4, lw=0.2, Bmax=0.8, Nsteps=4), Quadrupole(0.4, 1.0)])
     c = _coords(20, seed=90)
     tabs = [ctx.table(compile_table(ring, beam)), ctx.table(compile_table(other, beam))]
     parts = [HIPParticles.from_coords(c, None, ctx) for _ in range(2)]

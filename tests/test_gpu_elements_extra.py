@@ -22,6 +22,7 @@ from trackpad_hip import (M_ELECTRON, AccelCavity, Beam, Corrector, CrabCavity, 
                           track_)
 
 from conftest import ENERGY_VAL, PARITY_ATOL, PARTICLES_INITIAL
+import lattices as LT
 
 pytestmark = pytest.mark.gpu
 E = ENERGY_VAL
@@ -113,8 +114,7 @@ def test_ring_of_all_families(ctx, nturns):
     amplifies the last-place differences of the per-particle transcendentals between the
     device and the host library; the reference algorithm itself is only ~2e-8 accurate there
     against the same algorithm in long double (measured), so the device is held to accuracy
-    parity: its error against the long-double run at most 3x the oracle's, per plane.  The
-    adjoint refuses the table."""
+    parity: its error against the long-double run at most 3x the oracle's, per plane."""
     beam = Beam(3.0e9)
     q = lambda k: Quadrupole(0.4, k, num_int_steps=8)
     els = [q(1.2), Drift(1.0), Solenoid(0.4, 0.3), Drift(0.5), q(-1.2), Drift(1.0),
@@ -147,8 +147,70 @@ def test_ring_of_all_families(ctx, nturns):
         eg = np.max(np.abs(gc - ld)[alive], axis=0) / rms
         eo = np.max(np.abs(oc - ld)[alive], axis=0) / rms
         assert np.all(eg <= 3.0 * eo + 1e-14), (eg, eo)
-    with pytest.raises(TrackPadHIPError, match="TP_EUNSUPPORTED"):
-        ringpass_grad(c0[8:16], lat, beam, 2, ctx=ctx)
+
+
+@pytest.mark.parametrize("fast", [False, True])
+def test_adjoint_through_every_family(ctx, fast):
+    """Reverse mode through the families above (inverse maps, VJPs by dual numbers at the
+    reconstructed inputs, the crab's phase by fixed-point steps, rotations by their inverses):
+    gradients of 48 particles over 4 turns against the oracle's forward-mode derivatives along
+    random directions, per quadrupole strength and per initial coordinate: relative 1e-9."""
+    from trackpad_hip import strength_positions
+    beam = Beam(3.0e9)
+    lat = LT.families_ring()
+    n, T = 48, 4
+    rng = np.random.default_rng(62)
+    c0 = rng.normal(0, 1, (n, 6)) * np.array([3e-4, 3e-5, 3e-4, 3e-5, 1e-3, 5e-4])
+    g = ringpass_grad(c0, lat, beam, T, ckpt_stride=3, fast=fast, ctx=ctx)
+    assert g.n_alive == n
+    pos = strength_positions(lat)
+    assert len(pos) == 3  # the SC wrappers carry no strength gradient (their k is not a slot)
+    for trial in range(3):
+        dk = np.zeros(len(lat))
+        dk[pos] = rng.normal(0, 1, len(pos))
+        dr0 = rng.normal(0, 1, (n, 6)) * np.array([5e-5, 2e-5, 1e-5, 2e-6, 1e-4, 1e-3])
+        _, dl, na = O.loss_dir(lat, beam, c0, T, dk=dk, dr0=dr0)
+        assert na == n
+        ref = dl / na
+        got = float(np.dot(g.dk, dk) + np.sum(g.dx0 * dr0))
+        assert abs(got - ref) <= 1e-9 * abs(ref), (trial, got, ref)
+    for p in pos:
+        dk = np.zeros(len(lat))
+        dk[p] = 1.0
+        _, dl, na = O.loss_dir(lat, beam, c0, T, dk=dk)
+        assert abs(g.dk[p] - dl / na) <= 1e-9 * abs(dl / na) + 1e-22, (p, g.dk[p], dl / na)
+    for q_ in (0, 29):
+        for j in range(6):
+            dr0 = np.zeros((n, 6))
+            dr0[q_, j] = 1.0
+            _, dl, na = O.loss_dir(lat, beam, c0, T, dr0=dr0)
+            assert abs(g.dx0[q_, j] - dl / na) <= 1e-9 * abs(dl / na) + 1e-20, (q_, j, g.dx0[q_, j], dl / na)
+
+
+@pytest.mark.parametrize("fast", [False, True])
+def test_adjoint_linearised_drift(ctx, fast):
+    """USE_EXACT_HAMILTONIAN = false (src/tracking.jl:100-107) in every drift of the DBA ring and
+    of the families ring: gradients against the oracle run with the linearised drift, 1e-9."""
+    from trackpad_hip import strength_positions
+    from trackpad_hip import workloads as W
+    beam = Beam(3.0e9)
+    rng = np.random.default_rng(63)
+    for lat, T in ((W.dba_ring(beam=beam), 3), (LT.families_ring(), 3)):
+        n = 32
+        c0 = rng.normal(0, 1, (n, 6)) * np.array([2e-4, 2e-5, 2e-4, 2e-5, 1e-3, 5e-4])
+        g = ringpass_grad(c0, lat, beam, T, fast=fast, ctx=ctx, exact_hamiltonian=False)
+        assert g.n_alive == n
+        pos = strength_positions(lat)
+        dk = np.zeros(len(lat))
+        dk[pos] = rng.normal(0, 1, len(pos))
+        dr0 = rng.normal(0, 1, (n, 6)) * np.array([5e-5, 2e-5, 1e-5, 2e-6, 1e-4, 1e-3])
+        _, dl, na = O.loss_dir(lat, beam, c0, T, dk=dk, dr0=dr0, exact=False)
+        ref = dl / na
+        got = float(np.dot(g.dk, dk) + np.sum(g.dx0 * dr0))
+        assert abs(got - ref) <= 1e-9 * abs(ref), (got, ref)
+        # and the exact-Hamiltonian gradient differs by far more than the tolerance
+        ge = ringpass_grad(c0, lat, beam, T, fast=fast, ctx=ctx)
+        assert abs(float(np.dot(ge.dk, dk) + np.sum(ge.dx0 * dr0)) - ref) > 1e-7 * abs(ref)
 
 
 def test_ring_with_wiggler(ctx):
@@ -176,3 +238,6 @@ def test_ring_with_wiggler(ctx):
     alive = olf == 0
     rms = np.sqrt(np.mean(oc[alive] ** 2, axis=0))
     assert np.all(np.max(np.abs(gc - oc)[alive], axis=0) <= 1e-12 * rms)
+    # the reverse pass does not cover the wiggler
+    with pytest.raises(TrackPadHIPError, match="TP_EUNSUPPORTED"):
+        ringpass_grad(c0[8:16], lat, beam, 2, ctx=ctx)

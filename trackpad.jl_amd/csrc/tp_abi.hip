@@ -353,6 +353,7 @@ int tp_table_destroy(tp_table* t) {
   (void)hipStreamSynchronize(ctx->stream);
   if (t->dev) (void)hipFree(t->dev);
   if (t->d_slots) (void)hipFree(t->d_slots);
+  if (t->d_minv) (void)hipFree(t->d_minv);
   delete t;
   ctx_release(ctx);
   return TP_OK;
@@ -631,12 +632,76 @@ struct DevBuf {
 
 namespace {
 
-// Variant of the reverse kernels: the largest one (TP_FEAT_ADJ_ALL) only for Forest exact
-// bends; a table whose forward variant is ALL for its misalignments alone runs the C reverse
-// kernel, which undoes t1/t2 for every variant at run time.
-int adj_variant(unsigned features, int variant) {
-  if (features & FT_ESBEND) return TP_VARIANT_ALL;
+// Variant of the reverse kernels: the largest one (TP_FEAT_ADJ_ALL) for Forest exact bends,
+// the linearised drift, the families of tags 8-21 and r1/r2 rotations; a table whose forward
+// variant is ALL for its t1/t2 translations alone runs the C reverse kernel, which undoes
+// t1/t2 for every variant at run time.
+int adj_variant(unsigned features, int variant, bool rotations) {
+  if ((features & (FT_ESBEND | FT_LINH | FT_EXTRA)) || rotations) return TP_VARIANT_ALL;
   return variant > TP_VARIANT_C ? TP_VARIANT_C : variant;
+}
+
+bool has_rotations(const std::vector<tp_rec>& recs) {
+  for (const tp_rec& r : recs)
+    if (r.flags & (TP_F_R1 | TP_F_R2)) return true;
+  return false;
+}
+
+// Inverse of a 6 x 6 column-major matrix by Gauss-Jordan elimination with partial pivoting in
+// long double; false if a pivot vanishes.
+bool invert6(const double* M, double* out) {
+  long double a[6][12];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      a[i][j] = M[i + 6 * j];
+      a[i][6 + j] = (i == j) ? 1.0L : 0.0L;
+    }
+  for (int c = 0; c < 6; ++c) {
+    int piv = c;
+    for (int i = c + 1; i < 6; ++i)
+      if (fabsl(a[i][c]) > fabsl(a[piv][c])) piv = i;
+    if (!(fabsl(a[piv][c]) > 0.0L)) return false;
+    if (piv != c)
+      for (int j = 0; j < 12; ++j) std::swap(a[c][j], a[piv][j]);
+    const long double d = a[c][c];
+    for (int j = 0; j < 12; ++j) a[c][j] /= d;
+    for (int i = 0; i < 6; ++i) {
+      if (i == c) continue;
+      const long double f = a[i][c];
+      if (f != 0.0L)
+        for (int j = 0; j < 12; ++j) a[i][j] -= f * a[c][j];
+    }
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) out[i + 6 * j] = (double)a[i][6 + j];
+  return true;
+}
+
+// The reverse pass undoes r' = R r with R^-1: minv (shaped like the pool, zero elsewhere)
+// receives the inverses of the r1 / r2 matrices of recs, whose pool starts at pool[base],
+// at the same offsets.
+bool rotation_inverses(const std::vector<tp_rec>& recs, const std::vector<double>& pool, size_t base,
+                       std::vector<double>& minv) {
+  for (const tp_rec& r : recs) {
+    if (r.flags & TP_F_R1)
+      if (!invert6(pool.data() + base + r.moff + 12, minv.data() + base + r.moff + 12)) return false;
+    if (r.flags & TP_F_R2)
+      if (!invert6(pool.data() + base + r.moff + 48, minv.data() + base + r.moff + 48)) return false;
+  }
+  return true;
+}
+
+// The device copy of a table's rotation inverses, built on the first reverse-mode call.
+int ensure_minv(tp_ctx* ctx, const tp_table* t, const char* who) {
+  tp_table* tm = const_cast<tp_table*>(t);
+  if (tm->d_minv || !has_rotations(t->recs)) return TP_OK;
+  std::vector<double> minv(t->pool.size(), 0.0);
+  if (!rotation_inverses(t->recs, t->pool, 0, minv))
+    return fail(TP_EUNSUPPORTED, "%s: a singular r1/r2 matrix has no inverse", who);
+  TP_HIP(hipMalloc(&tm->d_minv, minv.size() * sizeof(double)));
+  TP_HIP(hipMemcpyAsync(tm->d_minv, minv.data(), minv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  TP_HIP(hipStreamSynchronize(ctx->stream));
+  return TP_OK;
 }
 
 // A time-varying table's strength slots are those of row 0 (per position); the reverse pass
@@ -660,11 +725,8 @@ int adj_check(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, in
   if (nturns < 1) return fail(TP_EINVAL, "%s: nturns = %d", who, nturns);
   if (ckpt_stride < 0) return fail(TP_EINVAL, "%s: ckpt_stride = %d", who, ckpt_stride);
   if (flags & ~(TP_TRACK_FAST | TP_TRACK_TIMED)) return fail(TP_EINVAL, "%s: unsupported flags 0x%x", who, flags);
-  if (t->features & (FT_LINH | FT_EXTRA))
-    return fail(TP_EUNSUPPORTED, "%s: the linearised drift and the element families beyond drifts, multipoles, "
-                "bends and RF cavities have no adjoint here", who);
   for (const tp_rec& r : t->recs) {
-    if (r.flags & (TP_F_R1 | TP_F_R2)) return fail(TP_EUNSUPPORTED, "%s: r1/r2 rotations have no adjoint here", who);
+    if (r.tag == TP_WIGGLER) return fail(TP_EUNSUPPORTED, "%s: the wiggler has no adjoint here", who);
     if (r.tag == TP_ESBEND && (r.flags & TP_F_DRIFTLIMIT))
       return fail(TP_EUNSUPPORTED, "%s: exact bends with |angle/L| < 1e-6 (drift limit) have no adjoint here", who);
   }
@@ -703,6 +765,8 @@ extern "C" {
 int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, int32_t ckpt_stride,
                uint32_t flags, double* loss, int64_t* n_alive, double* dL_dk, double* dL_dx0) {
   int rc = adj_check(ctx, t, p, turn0, nturns, ckpt_stride, flags, "tp_adjoint");
+  if (rc) return rc;
+  rc = ensure_minv(ctx, t, "tp_adjoint");
   if (rc) return rc;
   tp_table* tm = const_cast<tp_table*>(t);
   if (!tm->d_slots) {
@@ -775,7 +839,9 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
   a.ld = n;
   a.n_seq = (int32_t)t->hdr.n_seq;
   a.n_slots = n_slots;
-  a.variant = adj_variant(t->features, t->variant);
+  a.variant = adj_variant(t->features, t->variant, has_rotations(t->recs));
+  a.exact_h = (t->hdr.flags & TP_H_EXACT_HAMILTONIAN) ? 1 : 0;
+  a.minv = t->d_minv;
   a.waves = ctx->back_waves;
   a.n_rows = t->hdr.n_rows > 1 ? (int32_t)t->hdr.n_rows : 1;
   a.row_turn0 = t->hdr.row_turn0;
@@ -831,6 +897,8 @@ int tp_jacobian(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, 
                 uint32_t flags, double* jac) {
   int rc = adj_check(ctx, t, p, turn0, nturns, ckpt_stride, flags, "tp_jacobian");
   if (rc) return rc;
+  rc = ensure_minv(ctx, t, "tp_jacobian");
+  if (rc) return rc;
   const int64_t n = p->n;
   const int32_t K = ckpt_stride > 0 ? ckpt_stride : 1;
   const int32_t nseg = (nturns + K - 1) / K;
@@ -863,7 +931,9 @@ int tp_jacobian(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, 
   a.ld = n;
   a.n_seq = (int32_t)t->hdr.n_seq;
   a.n_slots = 0;
-  a.variant = adj_variant(t->features, t->variant);
+  a.variant = adj_variant(t->features, t->variant, has_rotations(t->recs));
+  a.exact_h = (t->hdr.flags & TP_H_EXACT_HAMILTONIAN) ? 1 : 0;
+  a.minv = t->d_minv;
   a.waves = ctx->back_waves;
   a.n_rows = t->hdr.n_rows > 1 ? (int32_t)t->hdr.n_rows : 1;
   a.row_turn0 = t->hdr.row_turn0;
@@ -1048,7 +1118,8 @@ int batch_plan(tp_batch* b, int mode, const int32_t* turn0, const int32_t* nturn
   a.item_partial = (double*)b->d_item_partial;
   a.max_slots = b->max_slots;
   a.variant = b->variant;
-  a.back_variant = adj_variant(b->features, b->variant);
+  a.back_variant = adj_variant(b->features, b->variant, b->d_minv != nullptr);
+  a.minv = b->d_minv;
   a.exact_h = b->exact_h;
   a.block = b->block;
   const int occ_f = fast ? tp_batch_occupancy_fast(b->variant, b->exact_h, adj, b->block, b->fwd_waves)
@@ -1187,11 +1258,8 @@ int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_part
     b->max_slots = std::max(b->max_slots, ns);
     if (b->no_adjoint.empty()) {
       if (!rows_share_slots(t)) b->no_adjoint = "a position changes its main multipole order between rows";
-      if (t->features & (FT_LINH | FT_EXTRA))
-        b->no_adjoint = "the linearised drift and the element families beyond drifts, multipoles, bends and RF "
-                        "cavities have no adjoint here";
       for (const tp_rec& r : t->recs) {
-        if (r.flags & (TP_F_R1 | TP_F_R2)) b->no_adjoint = "r1/r2 rotations have no adjoint here";
+        if (r.tag == TP_WIGGLER) b->no_adjoint = "the wiggler has no adjoint here";
         if (r.tag == TP_ESBEND && (r.flags & TP_F_DRIFTLIMIT))
           b->no_adjoint = "exact bends with |angle/L| < 1e-6 (drift limit) have no adjoint here";
       }
@@ -1203,11 +1271,19 @@ int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_part
   else if ((feat & ~TP_FEAT_C) == 0) b->variant = TP_VARIANT_C;
   else b->variant = TP_VARIANT_ALL;
   b->exact_h = (feat & FT_LINH) ? 0 : 1;
+  std::vector<double> minv;
+  for (int32_t v = 0; v < n_var; ++v) {
+    if (!has_rotations(tables[v]->recs)) continue;
+    if (minv.empty()) minv.assign(pool.size(), 0.0);
+    if (!rotation_inverses(tables[v]->recs, pool, (size_t)b->pool_off[v], minv) && b->no_adjoint.empty())
+      b->no_adjoint = "a singular r1/r2 matrix has no inverse";
+  }
   TP_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
   hipError_t e = dalloc((void**)&b->d_seq, seq.size() * sizeof(int32_t));
   if (e == hipSuccess) e = dalloc((void**)&b->d_recs, recs.size() * sizeof(tp_rec));
   if (e == hipSuccess) e = dalloc((void**)&b->d_pool, pool.size() * sizeof(double));
   if (e == hipSuccess) e = dalloc((void**)&b->d_slots, slots.size() * sizeof(int32_t));
+  if (e == hipSuccess && !minv.empty()) e = dalloc((void**)&b->d_minv, minv.size() * sizeof(double));
   hipStream_t st = ctx->stream;
   if (e == hipSuccess && !seq.empty()) e = hipMemcpyAsync(b->d_seq, seq.data(), seq.size() * 4, hipMemcpyHostToDevice, st);
   if (e == hipSuccess && !recs.empty())
@@ -1215,9 +1291,11 @@ int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_part
   if (e == hipSuccess && !pool.empty()) e = hipMemcpyAsync(b->d_pool, pool.data(), pool.size() * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess && !slots.empty())
     e = hipMemcpyAsync(b->d_slots, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !minv.empty())
+    e = hipMemcpyAsync(b->d_minv, minv.data(), minv.size() * 8, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) {
-    for (void* p : {(void*)b->d_seq, (void*)b->d_recs, (void*)b->d_pool, (void*)b->d_slots})
+    for (void* p : {(void*)b->d_seq, (void*)b->d_recs, (void*)b->d_pool, (void*)b->d_slots, (void*)b->d_minv})
       if (p) (void)hipFree(p);
     return fail(e == hipErrorOutOfMemory ? TP_ENOMEM : TP_EHIP, "tp_batch_create: %s", hipGetErrorName(e));
   }
@@ -1246,7 +1324,7 @@ int tp_batch_destroy(tp_batch* b) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   batch_free_plan(b);
-  for (void* p : {(void*)b->d_seq, (void*)b->d_recs, (void*)b->d_pool, (void*)b->d_slots})
+  for (void* p : {(void*)b->d_seq, (void*)b->d_recs, (void*)b->d_pool, (void*)b->d_slots, (void*)b->d_minv})
     if (p) (void)hipFree(p);
   for (tp_particles* p : b->parts) particles_release(p);
   delete b;

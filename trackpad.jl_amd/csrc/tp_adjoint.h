@@ -12,7 +12,7 @@
 #ifndef TP_ADJOINT_H
 #define TP_ADJOINT_H
 
-#include "tp_maps.h"
+#include "tp_extra.h"
 
 namespace tpk {
 
@@ -24,7 +24,28 @@ struct Ad {
 // drift6 (exact Hamiltonian): inverse + VJP.  s is the drift's output on entry and its
 // input on return.  NormL depends only on px, py, delta, which the drift leaves alone.
 // ---------------------------------------------------------------------------------------
-TP_DEV void drift_back(St& s, Ad& a, const Cx& c, double L, double Lb) {
+// drift6 with USE_EXACT_HAMILTONIAN = false (src/tracking.jl:100-107): NormL = L/(1+delta),
+// x' = x + NormL px, y' = y + NormL py, z' = z + NormL (px^2 + py^2) / (2 (1+delta)).
+TP_DEV void drift_lin_back(St& s, Ad& a, double L) {
+  const double u = 1.0 + s.d;
+  const double N = L / u;
+  s.x = s.x - N * s.px;
+  s.y = s.y - N * s.py;
+  const double q = N * (s.px * s.px + s.py * s.py) / (2.0 * u);
+  s.z = s.z - q;
+  // dN/d delta = -N/u, dq/d delta = -2q/u, dq/d px = N px / u
+  const double g = a.z * N / u;
+  const double ad = -(a.x * s.px + a.y * s.py) * N / u - a.z * 2.0 * q / u;
+  a.px = a.px + a.x * N + g * s.px;
+  a.py = a.py + a.y * N + g * s.py;
+  a.d = a.d + ad;
+}
+
+TP_DEV void drift_back(St& s, Ad& a, const Cx& c, double L, double Lb, bool lin = false) {
+  if (lin) {
+    drift_lin_back(s, a, L);
+    return;
+  }
   const double pz2 = c.A - s.px * s.px - s.py * s.py;
 #if TP_FAST
   const double rz = rsqrt_nr(pz2);
@@ -313,53 +334,6 @@ TP_DEV void rf_kick_back(St& s, Ad& a, const double* __restrict__ p) {
 // by forward-mode dual numbers (D1) through the same formulas; the maps run twice per bend
 // per turn at most, so four evaluations cost little and keep the register need small.
 // ---------------------------------------------------------------------------------------
-struct D1 {
-  double v, d;
-  D1() = default;
-  TP_DEV D1(double x) : v(x), d(0.0) {}
-  TP_DEV D1(double x, double dx) : v(x), d(dx) {}
-};
-TP_DEV D1 operator+(D1 a, D1 b) { return D1(a.v + b.v, a.d + b.d); }
-TP_DEV D1 operator-(D1 a, D1 b) { return D1(a.v - b.v, a.d - b.d); }
-TP_DEV D1 operator-(D1 a) { return D1(-a.v, -a.d); }
-TP_DEV D1 operator*(D1 a, D1 b) { return D1(a.v * b.v, a.d * b.v + a.v * b.d); }
-TP_DEV D1 operator/(D1 a, D1 b) {
-  const double q = a.v / b.v;
-  return D1(q, (a.d - q * b.d) / b.v);
-}
-TP_DEV D1 operator+(double a, D1 b) { return D1(a + b.v, b.d); }
-TP_DEV D1 operator+(D1 a, double b) { return D1(a.v + b, a.d); }
-TP_DEV D1 operator-(double a, D1 b) { return D1(a - b.v, -b.d); }
-TP_DEV D1 operator-(D1 a, double b) { return D1(a.v - b, a.d); }
-TP_DEV D1 operator*(double a, D1 b) { return D1(a * b.v, a * b.d); }
-TP_DEV D1 operator*(D1 a, double b) { return D1(a.v * b, a.d * b); }
-TP_DEV D1 operator/(D1 a, double b) { return D1(a.v / b, a.d / b); }
-TP_DEV D1 operator/(double a, D1 b) {
-  const double q = a / b.v;
-  return D1(q, -q * b.d / b.v);
-}
-TP_DEV double val(double a) { return a; }
-TP_DEV double val(D1 a) { return a.v; }
-TP_DEV double gsqrt(double a) { return sqrt(a); }
-TP_DEV D1 gsqrt(D1 a) {
-  const double r = sqrt(a.v);
-  return D1(r, 0.5 * a.d / r);
-}
-TP_DEV double gtan(double a) { return tan_lib(a); }
-TP_DEV D1 gtan(D1 a) {
-  const double t = tan_lib(a.v);
-  return D1(t, a.d * (1.0 + t * t));
-}
-TP_DEV double gatan(double a) { return atan(a); }
-TP_DEV D1 gatan(D1 a) { return D1(atan(a.v), a.d / (1.0 + a.v * a.v)); }
-TP_DEV double gcos(double a) { return cos_lib(a); }
-TP_DEV D1 gcos(D1 a) { return D1(cos_lib(a.v), -sin_lib(a.v) * a.d); }
-// asin of the argument clamped to [-1, 1] (src/tracking.jl:839-840); zero slope where clamped
-TP_DEV double gasin_c(double a) { return asin_s(clamp1(a)); }
-TP_DEV D1 gasin_c(D1 a) {
-  if (!(fabs(a.v) < 1.0)) return D1(asin_s(clamp1(a.v)), 0.0);
-  return D1(asin_s(a.v), a.d / sqrt(1.0 - a.v * a.v));
-}
 // pxyz (src/tracking.jl:717-720): sqrt(B^2 - px^2 - py^2), 0 where that is not positive
 template <class T>
 TP_DEV T pxyz_g(T B, T px, T py) {
@@ -524,6 +498,210 @@ __device__ __noinline__ SA fringe_back(St s, Ad a, Cx c, double b0, double gK) {
   a.py = a.py + g2;
   a.d = a.d + g3;
   return SA{s, a};
+}
+
+// ---------------------------------------------------------------------------------------
+// r1 / r2 rotations (the R * r products of every magnet pass!, e.g. src/tracking.jl:387-391):
+// r' = M r with M column-major (M[i + 6 j]); undone with the inverse Minv that the host
+// computes once per table, and the VJP is a = M^T a'.
+// ---------------------------------------------------------------------------------------
+TP_DEV void mat6_back(St& s, Ad& a, const double* __restrict__ M, const double* __restrict__ Minv) {
+  const double r[6] = {s.x, s.px, s.y, s.py, s.z, s.d};
+  const double l[6] = {a.x, a.px, a.y, a.py, a.z, a.d};
+  double o[6], g[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double acc = Minv[i] * r[0], gacc = M[6 * i] * l[0];
+#pragma unroll
+    for (int j = 1; j < 6; ++j) {
+      acc = acc + Minv[i + 6 * j] * r[j];
+      gacc = gacc + M[j + 6 * i] * l[j];
+    }
+    o[i] = acc;
+    g[i] = gacc;
+  }
+  s = St{o[0], o[1], o[2], o[3], o[4], o[5]};
+  a = Ad{g[0], g[1], g[2], g[3], g[4], g[5]};
+}
+
+// ---------------------------------------------------------------------------------------
+// The other element families (tp_extra.h; src/tracking.jl:529-605, 1076-1218, 1531-1663)
+// backwards.  Inverses:
+//   * Solenoid and the LBend body are flows of quadratic Hamiltonians (plus, for LBend, a
+//     constant dispersion term): the map of -L undoes them in x, px, y, py (checked on the
+//     oracle to the last bit / 1e-19), and z, which they change by a function of the other
+//     coordinates, is restored by subtracting that function at the recovered input;
+//   * Corrector, the Lorentz boosts and LongitudinalRFMap in closed form;
+//   * YRotation by the rotation of -angle (the same group as yrot);
+//   * Translation, the Gaussian beam-beam kicks, AccelCavity and the wakes are shears: they
+//     add to some coordinates a function of coordinates they leave alone, so the input is
+//     out - (M(out) - out);
+//   * CrabCavity: its kick phase is taken at the entrance z, before the first half drift;
+//     the entrance z is found by fixed-point steps through the undone kick and drift
+//     (a contraction with factor ~ L |k V x| / 2).
+// VJPs by dual numbers through the same map text (vjp6), except the crab's (by hand) and
+// LBend's edges (edge_back).  The wiggler and its radiation are not covered.
+// ---------------------------------------------------------------------------------------
+template <class F>
+TP_DEV Ad vjp6(const St& s, const Ad& a, F&& map) {
+  double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0, g4 = 0.0, g5 = 0.0;
+#pragma nounroll
+  for (int k = 0; k < 6; ++k) {
+    const SD q{D1(s.x, k == 0), D1(s.px, k == 1), D1(s.y, k == 2), D1(s.py, k == 3), D1(s.z, k == 4), D1(s.d, k == 5)};
+    const SD o = map(q);
+    const double gk = a.x * o.x.d + a.px * o.px.d + a.y * o.y.d + a.py * o.py.d + a.z * o.z.d + a.d * o.d.d;
+    g0 = k == 0 ? gk : g0;
+    g1 = k == 1 ? gk : g1;
+    g2 = k == 2 ? gk : g2;
+    g3 = k == 3 ? gk : g3;
+    g4 = k == 4 ? gk : g4;
+    g5 = k == 5 ? gk : g5;
+  }
+  return Ad{g0, g1, g2, g3, g4, g5};
+}
+
+// Input of a shear map M from its output: out - (M(out) - out)
+TP_DEV St unshear(const St& o, const St& m) {
+  return St{2.0 * o.x - m.x, 2.0 * o.px - m.px, 2.0 * o.y - m.y, 2.0 * o.py - m.py, 2.0 * o.z - m.z, 2.0 * o.d - m.d};
+}
+
+// drift6 inverse only (no VJP), exact or linearised
+TP_DEV St drift_inv(St s, double L, double Lb, double beti, bool lin) {
+  if (lin) {
+    const double N = L / (1.0 + s.d);
+    s.x = s.x - N * s.px;
+    s.y = s.y - N * s.py;
+    s.z = s.z - N * (s.px * s.px + s.py * s.py) / (2.0 * (1.0 + s.d));
+    return s;
+  }
+  Cx c;
+  mkctx(s, c, beti);
+  const double NormL = L / sqrt(c.A - s.px * s.px - s.py * s.py);
+  s.x = s.x - NormL * s.px;
+  s.y = s.y - NormL * s.py;
+  s.z = s.z - NormL * c.B + Lb;
+  return s;
+}
+
+// CrabCavity backwards (src/tracking.jl:1531-1547): [half drift] kick [half drift], the
+// kick px += V sin(a), delta -= V' x cos(a) with a = (k z_entry + phi) beti.
+TP_DEV SA crab_back(St s, Ad a, uint32_t fl, const double* __restrict__ p, double beti, bool lin) {
+  const bool dr = (fl & TP_F_RF_DRIFT) != 0;
+  Cx c;
+  if (dr) {
+    mkctx(s, c, beti);
+    drift_back(s, a, c, p[0], p[1], lin);  // s: the kicked state
+  }
+  const St k = s;
+  St s1 = k, s0 = k;
+  double z0 = k.z, ph = 0.0;
+#pragma nounroll
+  for (int it = 0; it < (dr ? 6 : 1); ++it) {
+    ph = (p[3] * z0 + p[4]) * beti;
+    s1.px = k.px - p[2] * sin(ph);
+    s1.d = k.d + p[5] * k.x * cos(ph);
+    s0 = dr ? drift_inv(s1, p[0], p[1], beti, lin) : s1;
+    z0 = s0.z;
+  }
+  // kick VJP at s1: d px'/d a = V cos(a) beti, d delta'/d a = V' x sin(a) beti, d delta'/dx
+  const double a_ph = a.px * p[2] * cos(ph) * beti + a.d * p[5] * k.x * sin(ph) * beti;
+  a.x = a.x - a.d * p[5] * cos(ph);
+  if (dr) {
+    mkctx(s1, c, beti);
+    St t = s1;
+    drift_back(t, a, c, p[0], p[1], lin);
+  }
+  a.z = a.z + a_ph * p[3];
+  return SA{s0, a};
+}
+
+__device__ __noinline__ SA extra_back(St s, Ad a, int tag, uint32_t fl, int n, const double* __restrict__ p,
+                                      double beti, bool lin) {
+  St in = s;
+  switch (tag) {
+    case TP_SOLENOID: {
+      const double q[2] = {-p[0], p[1]};
+      in = solenoid(s, q);
+      St z0 = in;
+      z0.z = 0.0;
+      in.z = s.z - solenoid(z0, p).z;
+      in.d = s.d;
+      a = vjp6(in, a, [&](const SD& r) { return solenoid(r, p); });
+      break;
+    }
+    case TP_CORRECTOR: {
+      const double p_norm = 1.0 / (1.0 + s.d);
+      const double NormL = p[0] * p_norm;
+      in.px = s.px - p[1];
+      in.py = s.py - p[2];
+      in.x = s.x - NormL * (in.px + p[4]);
+      in.y = s.y - NormL * (in.py + p[5]);
+      St z0 = in;
+      z0.z = 0.0;
+      in.z = s.z - corrector(z0, p).z;
+      a = vjp6(in, a, [&](const SD& r) { return corrector(r, p); });
+      break;
+    }
+    case TP_TRANSLATION:
+      in = unshear(s, translation(s, p, beti));
+      a = vjp6(in, a, [&](const SD& r) { return translation(r, p, beti); });
+      break;
+    case TP_YROTATION: {
+      const double q[3] = {p[0], -p[1], -p[2]};
+      in = y_rotation(s, q, beti);
+      a = vjp6(in, a, [&](const SD& r) { return y_rotation(r, p, beti); });
+      break;
+    }
+    case TP_CRAB:
+      return crab_back(s, a, fl, p, beti, lin);
+    case TP_ACCEL:
+      in = unshear(s, accel_kick(s, p));
+      a = vjp6(in, a, [&](const SD& r) { return accel_kick(r, p); });
+      break;
+    case TP_LONGRF:
+      in.z = s.z + p[0] * s.d;
+      a = vjp6(in, a, [&](const SD& r) { return longrf_slip(r, p); });
+      break;
+    case TP_BOOST:
+    case TP_INVBOOST: {
+      const bool inv = tag == TP_INVBOOST;
+      in.z = s.z / p[1];
+      in.px = s.px / p[1];
+      in.py = s.py / p[1];
+      in.x = inv ? s.x + p[0] * in.z : s.x - p[0] * in.z;
+      in.d = inv ? s.d - p[0] * in.px : s.d + p[0] * in.px;
+      a = vjp6(in, a, [&](const SD& r) { return boost(r, p, inv); });
+      break;
+    }
+    case TP_GAUSSBB:
+      in = unshear(s, gauss_bb(s, n, p));
+      a = vjp6(in, a, [&](const SD& r) { return gauss_bb(r, n, p); });
+      break;
+    case TP_RLCWAKE:
+      in = unshear(s, rlc_wake(s, p));
+      a = vjp6(in, a, [&](const SD& r) { return rlc_wake(r, p); });
+      break;
+    case TP_TABWAKE:
+      in = unshear(s, tab_wake(s, n, p));
+      a = vjp6(in, a, [&](const SD& r) { return tab_wake(r, n, p); });
+      break;
+    case TP_LBEND: {
+      const double irho = p[4];
+      edge_back(s, a, 1, p + 9, irho, true);
+      const double q[5] = {-p[0], p[1], p[2], p[3], p[4]};
+      in = lbend_body(s, q);
+      St z0 = in;
+      z0.z = 0.0;
+      in.z = s.z - lbend_body(z0, p).z;
+      in.d = s.d;
+      a = vjp6(in, a, [&](const SD& r) { return lbend_body(r, p); });
+      edge_back(in, a, 1, p + 5, irho, false);
+      break;
+    }
+    default:  // TP_NAN: a particle that passes it is lost, so none reaches it backwards
+      break;
+  }
+  return SA{in, a};
 }
 
 }  // namespace tpk

@@ -39,34 +39,35 @@ namespace TPK_BUILD {
   }
 
 template <int KK>
-TP_DEV void mpole_back(St& s, Ad& a, const Cx& c, const double* __restrict__ p, int ns, int m, int g, double& gacc) {
+TP_DEV void mpole_back(St& s, Ad& a, const Cx& c, const double* __restrict__ p, int ns, int m, int g, double& gacc,
+                       bool lin) {
   const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3], L1b = p[4], L2b = p[5];
   const double* poly = p + 6;
   // the Yoshida step D1 K1 D2 K2 D2 K1 D1 is a palindrome: undo it in the same order
   for (int st = 0; st < ns; ++st) {
-    drift_back(s, a, c, L1, L1b);
+    drift_back(s, a, c, L1, L1b, lin);
     kick_back<KK>(s, a, poly, m, K1, g, gacc);
-    drift_back(s, a, c, L2, L2b);
+    drift_back(s, a, c, L2, L2b, lin);
     kick_back<KK>(s, a, poly, m, K2, g, gacc);
-    drift_back(s, a, c, L2, L2b);
+    drift_back(s, a, c, L2, L2b, lin);
     kick_back<KK>(s, a, poly, m, K1, g, gacc);
-    drift_back(s, a, c, L1, L1b);
+    drift_back(s, a, c, L1, L1b, lin);
   }
 }
 
 template <int KK>
-TP_DEV void sbend_back(St& s, Ad& a, const Cx& c, const double* __restrict__ p, int ns, int m, double beti) {
+TP_DEV void sbend_back(St& s, Ad& a, const Cx& c, const double* __restrict__ p, int ns, int m, double beti, bool lin) {
   const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3], L1b = p[4], L2b = p[5];
   const double irho = p[6], K1i = p[7], K2i = p[8];
   const double* poly = p + 17;
   for (int st = 0; st < ns; ++st) {
-    drift_back(s, a, c, L1, L1b);
+    drift_back(s, a, c, L1, L1b, lin);
     bkick_back<KK>(s, a, poly, m, K1, K1i, irho, beti);
-    drift_back(s, a, c, L2, L2b);
+    drift_back(s, a, c, L2, L2b, lin);
     bkick_back<KK>(s, a, poly, m, K2, K2i, irho, beti);
-    drift_back(s, a, c, L2, L2b);
+    drift_back(s, a, c, L2, L2b, lin);
     bkick_back<KK>(s, a, poly, m, K1, K1i, irho, beti);
-    drift_back(s, a, c, L1, L1b);
+    drift_back(s, a, c, L1, L1b, lin);
   }
 }
 
@@ -121,23 +122,28 @@ TP_DEV void sub6(St& s, const double* __restrict__ t) {
   s.d = s.d - t[5];
 }
 
-// One element backwards: s is the element's output on entry, its input on return.
+// One element backwards: s is the element's output on entry, its input on return.  minv
+// holds the inverses of the r1 / r2 matrices at the pool offsets of the matrices (variant
+// with FT_MISALIGN); lin selects the linearised drift (variant with FT_LINH).
 template <unsigned FEAT>
-TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double* __restrict__ pool, double beti,
-                            double& gacc) {
+TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double* __restrict__ pool,
+                            const double* __restrict__ minv, double beti, bool lin, double& gacc) {
   const double* __restrict__ p = pool + rec.off;
   const uint32_t fl = rec.flags;
   const int m = rec.max_order, ns = rec.nsteps, g = rec.gorder;
-  if (fl & TP_F_T2) {
-    sub6(s, pool + rec.moff + 6);
+  if (fl & (TP_F_T2 | TP_F_R2)) {
+    if (fl & TP_F_T2) sub6(s, pool + rec.moff + 6);
+    if constexpr ((FEAT & FT_MISALIGN) != 0) {
+      if (fl & TP_F_R2) mat6_back(s, a, pool + rec.moff + 48, minv + rec.moff + 48);
+    }
     mkctx(s, c, beti);
   }
   switch (rec.tag) {
     case TP_DRIFT:
-      drift_back(s, a, c, p[0], p[1]);
+      drift_back(s, a, c, p[0], p[1], lin);
       break;
     case TP_MPOLE: {
-#define TP_CALL(KK) mpole_back<KK>(s, a, c, p, ns, m, g, gacc)
+#define TP_CALL(KK) mpole_back<KK>(s, a, c, p, ns, m, g, gacc, lin)
       TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
       break;
@@ -153,7 +159,7 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
         const int fin = (fl >> TP_F_FIN_SHIFT) & 0xF, fout = (fl >> TP_F_FOUT_SHIFT) & 0xF;
         const double irho = p[6];
         if (fout != 0) edge_back(s, a, fout, p + 13, irho, true);
-#define TP_CALL(KK) sbend_back<KK>(s, a, c, p, ns, m, beti)
+#define TP_CALL(KK) sbend_back<KK>(s, a, c, p, ns, m, beti, lin)
         TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
         if (fin != 0) edge_back(s, a, fin, p + 9, irho, false);
@@ -195,19 +201,30 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
     case TP_RFCAV:
       if constexpr ((FEAT & FT_RF) != 0) {
         const bool dr = (fl & TP_F_RF_DRIFT) != 0;
-        if (dr) drift_back(s, a, c, p[0], p[1]);
+        if (dr) drift_back(s, a, c, p[0], p[1], lin);
         if (fl & TP_F_RF_KICK) {
           rf_kick_back(s, a, p);
           mkctx(s, c, beti);
         }
-        if (dr) drift_back(s, a, c, p[0], p[1]);
+        if (dr) drift_back(s, a, c, p[0], p[1], lin);
       }
       break;
-    default:  // TP_MARKER
+    case TP_MARKER:
+      break;
+    default:  // the other families (tags TP_SOLENOID ..; the host refuses the wiggler)
+      if constexpr ((FEAT & FT_EXTRA) != 0) {
+        const SA r = extra_back(s, a, rec.tag, fl, ns, p, beti, lin);
+        s = r.s;
+        a = r.a;
+        mkctx(s, c, beti);
+      }
       break;
   }
-  if (fl & TP_F_T1) {
-    sub6(s, pool + rec.moff);
+  if (fl & (TP_F_T1 | TP_F_R1)) {
+    if constexpr ((FEAT & FT_MISALIGN) != 0) {
+      if (fl & TP_F_R1) mat6_back(s, a, pool + rec.moff + 12, minv + rec.moff + 12);
+    }
+    if (fl & TP_F_T1) sub6(s, pool + rec.moff);
     mkctx(s, c, beti);
   }
 }
@@ -220,9 +237,9 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
 // turn_last - 1, ..., each with its own row.
 template <unsigned FEAT>
 TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq_rows, const tp_rec* __restrict__ recs,
-                       const double* __restrict__ pool, const int32_t* __restrict__ slot_of_pos, int32_t n_seq,
-                       int32_t nturns, int32_t turn_last, int32_t n_rows, int32_t row_turn0, double beti, double* row,
-                       int lane) {
+                       const double* __restrict__ pool, const double* __restrict__ minv,
+                       const int32_t* __restrict__ slot_of_pos, int32_t n_seq, int32_t nturns, int32_t turn_last,
+                       int32_t n_rows, int32_t row_turn0, double beti, bool lin, double* row, int lane) {
   for (int32_t t = 0; t < nturns; ++t) {
     const int32_t* __restrict__ seq =
         seq_rows + (n_rows > 1 ? (int64_t)(turn_last - t - row_turn0) * n_seq : 0);
@@ -240,7 +257,7 @@ TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq_rows
         if (e >= 2) ri_next = seq[e - 2];
       }
       double gacc = 0.0;
-      apply_elem_back<FEAT>(s, l, c, rec, pool, beti, gacc);
+      apply_elem_back<FEAT>(s, l, c, rec, pool, minv, beti, lin, gacc);
       if (slot >= 0) {
         const double w = wave_sum(gacc);
         if (lane == 0) row[slot] += w;
@@ -290,8 +307,9 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void adjoint_kernel(const int32_
   }
   Cx c;
   mkctx(s, c, a.beti);
-  back_turns<FEAT>(s, l, c, seq, recs, pool, slot_of_pos, a.n_seq, a.nturns, a.turn_last, a.n_rows, a.row_turn0,
-                   a.beti, lds + wave * a.n_slots, lane);
+  const bool lin = (FEAT & FT_LINH) != 0 && a.exact_h == 0;
+  back_turns<FEAT>(s, l, c, seq, recs, pool, a.minv, slot_of_pos, a.n_seq, a.nturns, a.turn_last, a.n_rows,
+                   a.row_turn0, a.beti, lin, lds + wave * a.n_slots, lane);
   if (i < a.n) {
     double* r = lam + i;
     r[0] = l.x;
@@ -320,11 +338,14 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_back_kernel(const TpV
                                                          int32_t* __restrict__ counter, const int32_t* __restrict__ seq,
                                                          const tp_rec* __restrict__ recs,
                                                          const double* __restrict__ pool,
+                                                         const double* __restrict__ minv,
                                                          const int32_t* __restrict__ slots,
                                                          const double* __restrict__ ckpt,
-                                                         double* __restrict__ item_partial, int32_t max_slots) {
+                                                         double* __restrict__ item_partial, int32_t max_slots,
+                                                         int32_t exact_h) {
   extern __shared__ double lds[];  // [waves per block][max_slots]
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
+  const bool lin = (FEAT & FT_LINH) != 0 && exact_h == 0;
   double* row = lds + wave * max_slots;
   for (;;) {
     int32_t it = 0;
@@ -351,8 +372,9 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_back_kernel(const TpV
       }
       mkctx(s, c, d.beti);
       const int32_t nt = (j == d.nseg - 1) ? d.nturns - j * d.ckpt_stride : d.ckpt_stride;
-      back_turns<FEAT>(s, l, c, seq + d.seq_off, recs + d.rec_off, pool + d.pool_off, slots + d.pos_off, d.n_seq, nt,
-                       d.turn0 + j * d.ckpt_stride + nt - 1, d.n_rows, d.row_turn0, d.beti, row, lane);
+      back_turns<FEAT>(s, l, c, seq + d.seq_off, recs + d.rec_off, pool + d.pool_off,
+                       minv ? minv + d.pool_off : nullptr, slots + d.pos_off, d.n_seq, nt,
+                       d.turn0 + j * d.ckpt_stride + nt - 1, d.n_rows, d.row_turn0, d.beti, lin, row, lane);
     }
     if (li < d.n) {
       double* r = d.lam + li;
@@ -403,7 +425,8 @@ __global__ void reduce_kernel(const double* __restrict__ partial, int nblocks, i
 }
 
 // Register budgets compiled: the default, and at least 3 or 4 waves per SIMD for variants
-// B and C (the default leaves their reverse kernels at 2).
+// B and C (the default leaves their reverse kernels at 2); the all-families variant at 3
+// (its out-of-line maps would otherwise take up to 256 VGPRs and set the kernel's count).
 #define TP_WPE_SWITCH(waves, CALL)                            \
   if constexpr (FEAT == TP_FEAT_B || FEAT == TP_FEAT_C) {     \
     if ((waves) == 3) { CALL(3); }                            \
@@ -430,7 +453,8 @@ static void launch_back_v(const TpBatchArgs& b, hipStream_t stream) {
   const size_t lds = (size_t)(b.block / 64) * (size_t)(b.max_slots > 0 ? b.max_slots : 1) * sizeof(double);
 #define TP_CALL(W)                                                                                             \
   hipLaunchKernelGGL((batch_back_kernel<FEAT, W>), dim3(b.grid), dim3(b.block), lds, stream, b.desc, b.items, \
-                     b.n_items, b.counter, b.seq, b.recs, b.pool, b.slots, b.ckpt, b.item_partial, b.max_slots)
+                     b.n_items, b.counter, b.seq, b.recs, b.pool, b.minv, b.slots, b.ckpt, b.item_partial,       \
+                     b.max_slots, b.exact_h)
   TP_WPE_SWITCH(b.waves, TP_CALL)
 #undef TP_CALL
 }

@@ -35,9 +35,9 @@ enum : unsigned {
 #define TP_FEAT_B (FT_ESBEND_LE | FT_RF)                          // rings of exact bends + RF
 #define TP_FEAT_C (FT_ESBEND_LE | FT_RF | FT_SBEND | FT_GENKICK)  // + high-order multipoles
 #define TP_FEAT_ALL 255u
-// The reverse kernels' largest variant: every family the adjoint covers (no linearised
-// drift, no tags beyond TP_RFCAV's group); the host selects it only for Forest exact bends.
-#define TP_FEAT_ADJ_ALL (TP_FEAT_C | FT_ESBEND | FT_MISALIGN)
+// The reverse kernels' largest variant: every family (the host refuses the wiggler); it is
+// selected for Forest exact bends, r1/r2 rotations, the linearised drift and tags 8-21.
+#define TP_FEAT_ADJ_ALL TP_FEAT_ALL
 enum { TP_VARIANT_A = 0, TP_VARIANT_B = 1, TP_VARIANT_C = 2, TP_VARIANT_ALL = 3 };
 
 struct TpKArgs {
@@ -106,6 +106,7 @@ struct TpBatchArgs {
   const tp_rec* recs;
   const double* pool;
   const int32_t* slots;  // concatenated slot_of_pos
+  const double* minv;    // concatenated r1/r2 inverses at pool offsets (adjoint; null without rotations)
   double* ckpt;          // checkpoint slabs (adjoint)
   double* item_loss;     // per item: loss sum, survivors (adjoint)
   double* item_partial;  // per item: max_slots gradient sums (adjoint)
@@ -138,6 +139,8 @@ struct TpAdjArgs {
                                // loss sum, 2x / 2y; 2 + i: start from the unit vector e_i
   int32_t variant;
   int32_t waves;               // register budget: 0 (compiler's), 3 or 4 waves per SIMD
+  const double* minv;          // inverses of the r1/r2 matrices at their pool offsets (or null)
+  int32_t exact_h;             // USE_EXACT_HAMILTONIAN (0: linearised drift)
   int32_t turn_last;           // turn number of the segment's last turn
   int32_t n_rows;              // sequence rows of a time-varying table (1 = static)
   int32_t row_turn0;           // turn of row 0

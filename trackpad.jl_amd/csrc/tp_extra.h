@@ -7,11 +7,13 @@
 // 1076-1218, 1531-1663; Julia evaluates a*b*c as (a*b)*c and x^2 as x*x) so that the parity
 // build reproduces it.  The maps are compiled only into the kernel variant that holds every
 // feature, and they run in one out-of-line function, so their registers and transcendentals
-// do not weigh on the hot element loop.
+// do not weigh on the hot element loop.  The maps of the families the reverse pass
+// covers are written over the state type S (St, or SD of dual numbers: tp_dual.h) so that
+// the same text gives the forward map and its Jacobian columns.
 #ifndef TP_EXTRA_H
 #define TP_EXTRA_H
 
-#include "tp_maps.h"
+#include "tp_dual.h"
 
 namespace tpk {
 
@@ -19,38 +21,48 @@ TP_DEV St nan_state() {
   const double n = nan_d();
   return St{n, n, n, n, n, n};
 }
+TP_DEV SD nan_state(SD) {
+  const double n = nan_d();
+  return SD{n, n, n, n, n, n};
+}
+TP_DEV St nan_state(St) { return nan_state(); }
 
 // pz of Translation / YRotation: sqrt(1 + 2 delta beti + delta^2 - px^2 - py^2), NaN map
 // when not positive (src/tracking.jl:1183-1186, 1203-1206)
-TP_DEV bool pz_of(const St& s, double beti, double& pz) {
-  const double pz2 = 1.0 + 2.0 * s.d * beti + s.d * s.d - s.px * s.px - s.py * s.py;
-  if (!(pz2 > 0.0)) return false;
-  pz = sqrt(pz2);
+template <class S, class T>
+TP_DEV bool pz_of(const S& s, double beti, T& pz) {
+  const T pz2 = 1.0 + 2.0 * s.d * beti + s.d * s.d - s.px * s.px - s.py * s.py;
+  if (!(val(pz2) > 0.0)) return false;
+  pz = gsqrt(pz2);
   return true;
 }
 
 // Solenoid with ks != 0, src/tracking.jl:542-560
-TP_DEV St solenoid(St s, const double* __restrict__ p) {
+template <class S>
+TP_DEV S solenoid(S s, const double* __restrict__ p) {
+  using T = decltype(s.x);
   const double L = p[0], ks = p[1];
-  const double p_norm = 1.0 / (1.0 + s.d);
-  const double x = s.x, xpr = s.px * p_norm, y = s.y, ypr = s.py * p_norm;
-  const double H = ks * p_norm / 2.0;
-  const double S = sin(L * H), C = cos(L * H);
-  St o;
-  o.x = x * C * C + xpr * C * S / H + y * C * S + ypr * S * S / H;
-  o.px = (-x * H * C * S + xpr * C * C - y * H * S * S + ypr * C * S) / p_norm;
-  o.y = -x * C * S - xpr * S * S / H + y * C * C + ypr * C * S / H;
-  o.py = (x * H * S * S - xpr * C * S - y * C * S * H + ypr * C * C) / p_norm;
+  const T p_norm = 1.0 / (1.0 + s.d);
+  const T x = s.x, xpr = s.px * p_norm, y = s.y, ypr = s.py * p_norm;
+  const T H = ks * p_norm / 2.0;
+  const T S_ = gsin(L * H), C = gcos(L * H);
+  S o;
+  o.x = x * C * C + xpr * C * S_ / H + y * C * S_ + ypr * S_ * S_ / H;
+  o.px = (-x * H * C * S_ + xpr * C * C - y * H * S_ * S_ + ypr * C * S_) / p_norm;
+  o.y = -x * C * S_ - xpr * S_ * S_ / H + y * C * C + ypr * C * S_ / H;
+  o.py = (x * H * S_ * S_ - xpr * C * S_ - y * C * S_ * H + ypr * C * C) / p_norm;
   o.z = s.z + L * (H * H * (x * x + y * y) + 2.0 * H * (xpr * y - ypr * x) + xpr * xpr + ypr * ypr) / 2.0;
   o.d = s.d;
   return o;
 }
 
 // Corrector, src/tracking.jl:586-596 (p[3] = xkick^2/3 + ykick^2/3, p[4..5] = kicks/2)
-TP_DEV St corrector(St s, const double* __restrict__ p) {
-  const double p_norm = 1.0 / (1.0 + s.d);
-  const double NormL = p[0] * p_norm;
-  St o = s;
+template <class S>
+TP_DEV S corrector(S s, const double* __restrict__ p) {
+  using T = decltype(s.x);
+  const T p_norm = 1.0 / (1.0 + s.d);
+  const T NormL = p[0] * p_norm;
+  S o = s;
   o.z = s.z + NormL * p_norm * (p[3] + s.px * s.px + s.py * s.py + s.px * p[1] + s.py * p[2]) / 2.0;
   o.x = s.x + NormL * (s.px + p[4]);
   o.px = s.px + p[1];
@@ -60,10 +72,11 @@ TP_DEV St corrector(St s, const double* __restrict__ p) {
 }
 
 // Translation, src/tracking.jl:1182-1192
-TP_DEV St translation(St s, const double* __restrict__ p, double beti) {
-  double pz;
-  if (!pz_of(s, beti, pz)) return nan_state();
-  St o = s;
+template <class S>
+TP_DEV S translation(S s, const double* __restrict__ p, double beti) {
+  decltype(s.x) pz;
+  if (!pz_of(s, beti, pz)) return nan_state(s);
+  S o = s;
   o.x = s.x - (p[0] + p[2] * s.px / pz);
   o.y = s.y - (p[1] + p[2] * s.py / pz);
   o.z = s.z + p[2] * (beti + s.d) / pz;
@@ -71,13 +84,15 @@ TP_DEV St translation(St s, const double* __restrict__ p, double beti) {
 }
 
 // YRotation, src/tracking.jl:1194-1218 (p = cos, sin, tan of -angle; angle 0 is a marker)
-TP_DEV St y_rotation(St s, const double* __restrict__ p, double beti) {
+template <class S>
+TP_DEV S y_rotation(S s, const double* __restrict__ p, double beti) {
+  using T = decltype(s.x);
   const double ca = p[0], sa = p[1], ta = p[2];
-  double pz;
-  if (!pz_of(s, beti, pz)) return nan_state();
-  const double ptt = 1.0 - ta * s.px / pz;
-  if (fabs(ptt) < 1.4901161193847656e-08) return nan_state();  // sqrt(eps(Float64))
-  St o = s;
+  T pz;
+  if (!pz_of(s, beti, pz)) return nan_state(s);
+  const T ptt = 1.0 - ta * s.px / pz;
+  if (fabs(val(ptt)) < 1.4901161193847656e-08) return nan_state(s);  // sqrt(eps(Float64))
+  S o = s;
   o.x = s.x / (ca * ptt);
   o.px = ca * s.px + sa * pz;
   o.y = s.y + ta * s.x * s.py / (pz * ptt);
@@ -106,14 +121,16 @@ TP_DEV St crab(St s, uint32_t fl, const double* __restrict__ p, double beti) {
 }
 
 // StrongGaussianBeam / StrongThinGaussianBeam, src/tracking.jl:1610-1641
-TP_DEV St gauss_bb(St s, int nslice, const double* __restrict__ p) {
+template <class S>
+TP_DEV S gauss_bb(S s, int nslice, const double* __restrict__ p) {
+  using T = decltype(s.x);
   const double sx2 = p[0], sy2 = p[1], factor = p[2];
-  double px = s.px, py = s.py;
+  T px = s.px, py = s.py;
   for (int i = 0; i < nslice; ++i) {
     const double* q = p + 3 + 3 * i;
-    const double dx = s.x - q[1];
-    const double dy = s.y - q[2];
-    const double g = q[0] * exp(-0.5 * (dx * dx / sx2 + dy * dy / sy2));
+    const T dx = s.x - q[1];
+    const T dy = s.y - q[2];
+    const T g = q[0] * gexp(-0.5 * (dx * dx / sx2 + dy * dy / sy2));
     px = px - factor * g * dx / sx2;
     py = py - factor * g * dy / sy2;
   }
@@ -124,87 +141,134 @@ TP_DEV St gauss_bb(St s, int nslice, const double* __restrict__ p) {
 
 // LongitudinalWake: _linear_interpolate of src/elements/wake.jl:67-84 (first interval
 // with x < times[i], as the reference scans)
-TP_DEV double lin_interp(double x, const double* __restrict__ xs, const double* __restrict__ ys, int n) {
-  if (x <= xs[0]) {
+template <class T>
+TP_DEV T lin_interp(T x, const double* __restrict__ xs, const double* __restrict__ ys, int n) {
+  if (val(x) <= xs[0]) {
     const double slope = (ys[1] - ys[0]) / (xs[1] - xs[0]);
     return ys[0] + slope * (x - xs[0]);
-  } else if (x >= xs[n - 1]) {
+  } else if (val(x) >= xs[n - 1]) {
     const double slope = (ys[n - 1] - ys[n - 2]) / (xs[n - 1] - xs[n - 2]);
     return ys[n - 1] + slope * (x - xs[n - 1]);
   }
   for (int i = 1; i < n; ++i) {
-    if (x < xs[i]) {
+    if (val(x) < xs[i]) {
       const double slope = (ys[i] - ys[i - 1]) / (xs[i] - xs[i - 1]);
       return ys[i - 1] + slope * (x - xs[i - 1]);
     }
   }
-  return ys[n - 1];
+  return T(ys[n - 1]);
 }
 
 // LBend body, src/tracking.jl:1076-1150 (p = L, K, angle, by_error, angle/L)
-TP_DEV St lbend_body(St s, const double* __restrict__ p) {
+template <class S>
+TP_DEV S lbend_body(S s, const double* __restrict__ p) {
+  using T = decltype(s.x);
   const double L = p[0], grd = p[1], by_error = p[3], Kx = p[4];
-  const double p_norm = 1.0 / (1.0 + s.d);
-  const double G1 = (Kx * Kx + grd) * p_norm;
-  const double G2 = -grd * p_norm;
+  const T p_norm = 1.0 / (1.0 + s.d);
+  const T G1 = (Kx * Kx + grd) * p_norm;
+  const T G2 = -grd * p_norm;
   const double tol = 1.4901161193847656e-08;  // sqrt(eps(Float64))
-  double MHD = 1.0, M12 = L, M21 = 0.0, MVD = 1.0, M34 = L, M43 = 0.0;
-  double arg1 = 0.0, sqrtG1 = 0.0;
-  if (fabs(G1) >= tol) {
-    if (G1 > 0.0) {
-      sqrtG1 = sqrt(G1);
+  T MHD = 1.0, M12 = L, M21 = 0.0, MVD = 1.0, M34 = L, M43 = 0.0;
+  T arg1 = 0.0, sqrtG1 = 0.0;
+  if (fabs(val(G1)) >= tol) {
+    if (val(G1) > 0.0) {
+      sqrtG1 = gsqrt(G1);
       arg1 = L * sqrtG1;
-      MHD = cos(arg1);
-      M12 = sin(arg1) / sqrtG1;
-      M21 = -sin(arg1) * sqrtG1;
+      MHD = gcos(arg1);
+      M12 = gsin(arg1) / sqrtG1;
+      M21 = -gsin(arg1) * sqrtG1;
     } else {
-      sqrtG1 = sqrt(-G1);
+      sqrtG1 = gsqrt(-G1);
       arg1 = L * sqrtG1;
-      MHD = cosh(arg1);
-      M12 = sinh(arg1) / sqrtG1;
-      M21 = sinh(arg1) * sqrtG1;
+      MHD = gcosh(arg1);
+      M12 = gsinh(arg1) / sqrtG1;
+      M21 = gsinh(arg1) * sqrtG1;
     }
   }
-  if (fabs(G2) >= tol) {
-    if (G2 > 0.0) {
-      const double sqrtG2 = sqrt(G2);
-      const double arg2 = L * sqrtG2;
-      MVD = cos(arg2);
-      M34 = sin(arg2) / sqrtG2;
-      M43 = -sin(arg2) * sqrtG2;
+  if (fabs(val(G2)) >= tol) {
+    if (val(G2) > 0.0) {
+      const T sqrtG2 = gsqrt(G2);
+      const T arg2 = L * sqrtG2;
+      MVD = gcos(arg2);
+      M34 = gsin(arg2) / sqrtG2;
+      M43 = -gsin(arg2) * sqrtG2;
     } else {
-      const double sqrtG2 = sqrt(-G2);
-      const double arg2 = L * sqrtG2;
-      MVD = cosh(arg2);
-      M34 = sinh(arg2) / sqrtG2;
-      M43 = sinh(arg2) * sqrtG2;
+      const T sqrtG2 = gsqrt(-G2);
+      const T arg2 = L * sqrtG2;
+      MVD = gcosh(arg2);
+      M34 = gsinh(arg2) / sqrtG2;
+      M43 = gsinh(arg2) * sqrtG2;
     }
   }
-  const double x = s.x, xpr = s.px * p_norm, y = s.y, ypr = s.py * p_norm;
-  const double dterm = s.d * p_norm - by_error;
-  double x_new = MHD * x + M12 * xpr;
-  double px_new = (M21 * x + MHD * xpr) / p_norm;
-  if (fabs(G1) < tol) {
+  const T x = s.x, xpr = s.px * p_norm, y = s.y, ypr = s.py * p_norm;
+  const T dterm = s.d * p_norm - by_error;
+  T x_new = MHD * x + M12 * xpr;
+  T px_new = (M21 * x + MHD * xpr) / p_norm;
+  if (fabs(val(G1)) < tol) {
     x_new = x_new + dterm * L * L * Kx / 2.0;
     px_new = px_new + dterm * L * Kx / p_norm;
-  } else if (G1 > 0.0) {
-    x_new = x_new + dterm * (1.0 - cos(arg1)) * Kx / G1;
-    px_new = px_new + dterm * sin(arg1) * Kx / (sqrtG1 * p_norm);
+  } else if (val(G1) > 0.0) {
+    x_new = x_new + dterm * (1.0 - gcos(arg1)) * Kx / G1;
+    px_new = px_new + dterm * gsin(arg1) * Kx / (sqrtG1 * p_norm);
   } else {
-    x_new = x_new + dterm * (1.0 - cosh(arg1)) * Kx / G1;
-    px_new = px_new + dterm * sinh(arg1) * Kx / (sqrtG1 * p_norm);
+    x_new = x_new + dterm * (1.0 - gcosh(arg1)) * Kx / G1;
+    px_new = px_new + dterm * gsinh(arg1) * Kx / (sqrtG1 * p_norm);
   }
-  const double y_new = MVD * y + M34 * ypr;
-  const double py_new = (M43 * y + MVD * ypr) / p_norm;
-  double z_new = s.z + xpr * xpr * (L + MHD * M12) / 4.0;
-  if (fabs(G1) >= tol) {
+  const T y_new = MVD * y + M34 * ypr;
+  const T py_new = (M43 * y + MVD * ypr) / p_norm;
+  T z_new = s.z + xpr * xpr * (L + MHD * M12) / 4.0;
+  if (fabs(val(G1)) >= tol) {
     z_new = z_new + (L - MHD * M12) * (x * x * G1 + dterm * dterm * Kx * Kx / G1 - 2.0 * x * Kx * dterm) / 4.0;
     z_new = z_new + M12 * M21 * (x * xpr - xpr * dterm * Kx / G1) / 2.0;
     z_new = z_new + (Kx * x * M12 + xpr * (1.0 - MHD) * Kx / G1 + dterm * (L - M12) * Kx * Kx / G1);
   }
   z_new = z_new + ((L - MVD * M34) * y * y * G2 + ypr * ypr * (L + MVD * M34)) / 4.0;
   z_new = z_new + M34 * M43 * y * ypr / 2.0;
-  return St{x_new, px_new, y_new, py_new, z_new, s.d};
+  return S{x_new, px_new, y_new, py_new, z_new, s.d};
+}
+
+// The z- and delta-kicks of the families without drifts: AccelCavity (src/tracking.jl:
+// 1549-1559), LongitudinalRFMap (:1575-1585), LongitudinalRLCWake (:1647-1654,
+// src/elements/wake.jl:25-36), LongitudinalWake (:1656-1663, src/elements/wake.jl:86-91),
+// and the Lorentz boosts (:1587-1604)
+template <class S>
+TP_DEV S accel_kick(S s, const double* __restrict__ p) {
+  s.d = s.d + p[0] * (gsin(p[1] * s.z + p[2]) - p[3]);
+  return s;
+}
+template <class S>
+TP_DEV S longrf_slip(S s, const double* __restrict__ p) {
+  s.z = s.z - p[0] * s.d;
+  return s;
+}
+template <class S>
+TP_DEV S rlc_wake(S s, const double* __restrict__ p) {
+  using T = decltype(s.x);
+  const T zc = s.z / 2.99792458e8;
+  const T t = val(zc) < 0.0 ? zc : T(0.0);  // min(z / c, 0)
+  const T a = p[2] * t;
+  const T w = p[1] * (gcos(a) + gsin(a) / p[3]) * gexp(p[4] * t / p[5]);
+  s.d = s.d - p[0] * w;
+  return s;
+}
+template <class S>
+TP_DEV S tab_wake(S s, int n, const double* __restrict__ p) {
+  using T = decltype(s.x);
+  const T t = s.z / 2.99792458e8;
+  const T w = (val(t) > p[2]) ? T(0.0) : lin_interp(t * p[1], p + 3, p + 3 + n, n);
+  s.d = s.d - p[0] * w;
+  return s;
+}
+template <class S>
+TP_DEV S boost(S s, const double* __restrict__ p, bool inverse) {
+  if (inverse) {
+    const auto x_new = s.x - p[0] * s.z;
+    const auto d_new = s.d + p[0] * s.px;
+    return S{x_new, s.px * p[1], s.y, s.py * p[1], s.z * p[1], d_new};
+  }
+  const auto x_new = s.x + p[0] * s.z;
+  const auto d_new = s.d - p[0] * s.px;
+  return S{x_new, s.px * p[1], s.y, s.py * p[1], s.z * p[1], d_new};
 }
 
 // ---------------------------------------------------------------------------------------
@@ -368,38 +432,20 @@ __device__ __noinline__ St extra_map(St s, int tag, uint32_t fl, int n, const do
       return y_rotation(s, p, beti);
     case TP_CRAB:
       return crab<EXH>(s, fl, p, beti);
-    case TP_ACCEL:  // src/tracking.jl:1549-1559
-      s.d = s.d + p[0] * (sin(p[1] * s.z + p[2]) - p[3]);
-      return s;
-    case TP_LONGRF:  // src/tracking.jl:1575-1585
-      s.z = s.z - p[0] * s.d;
-      return s;
-    case TP_BOOST: {  // src/tracking.jl:1587-1595
-      const double x_new = s.x + p[0] * s.z;
-      const double d_new = s.d - p[0] * s.px;
-      return St{x_new, s.px * p[1], s.y, s.py * p[1], s.z * p[1], d_new};
-    }
-    case TP_INVBOOST: {  // src/tracking.jl:1597-1604
-      const double x_new = s.x - p[0] * s.z;
-      const double d_new = s.d + p[0] * s.px;
-      return St{x_new, s.px * p[1], s.y, s.py * p[1], s.z * p[1], d_new};
-    }
+    case TP_ACCEL:
+      return accel_kick(s, p);
+    case TP_LONGRF:
+      return longrf_slip(s, p);
+    case TP_BOOST:
+      return boost(s, p, false);
+    case TP_INVBOOST:
+      return boost(s, p, true);
     case TP_GAUSSBB:
       return gauss_bb(s, n, p);
-    case TP_RLCWAKE: {  // src/tracking.jl:1647-1654, src/elements/wake.jl:25-36
-      const double zc = s.z / 2.99792458e8;
-      const double t = zc < 0.0 ? zc : 0.0;  // min(z / c, 0)
-      const double a = p[2] * t;
-      const double w = p[1] * (cos(a) + sin(a) / p[3]) * exp(p[4] * t / p[5]);
-      s.d = s.d - p[0] * w;
-      return s;
-    }
-    case TP_TABWAKE: {  // src/tracking.jl:1656-1663, src/elements/wake.jl:86-91
-      const double t = s.z / 2.99792458e8;
-      const double w = (t > p[2]) ? 0.0 : lin_interp(t * p[1], p + 3, p + 3 + n, n);
-      s.d = s.d - p[0] * w;
-      return s;
-    }
+    case TP_RLCWAKE:
+      return rlc_wake(s, p);
+    case TP_TABWAKE:
+      return tab_wake(s, n, p);
     case TP_LBEND: {  // src/tracking.jl:1152-1170: entrance edge, body, exit edge (method 1)
       const double irho = p[4];
       edge_lin(s, 1, p + 5, irho, false);

@@ -46,6 +46,7 @@ struct tp_table {
   std::vector<double> pool;      // host copy
   int32_t* d_slots = nullptr;    // gradient slot of each position (adjoint), built on first use
   int32_t n_slots = 0;
+  double* d_minv = nullptr;      // r1/r2 inverses at the matrices' pool offsets (adjoint), on first use
 };
 
 // A particle set is reference counted too: a batch that binds it holds a reference.
@@ -79,6 +80,7 @@ struct tp_batch {
   tp_rec* d_recs = nullptr;
   double* d_pool = nullptr;
   int32_t* d_slots = nullptr;
+  double* d_minv = nullptr;  // r1/r2 inverses of every variant at the concatenated pool offsets (or null)
   int block = 256, fwd_waves = 0, back_waves = 0, cus = 0;
   // the planned launch (rebuilt when its key changes)
   int mode = 0;  // 0 none, 1 track, 2 adjoint

@@ -69,10 +69,11 @@ def set_strengths(lat: Lattice, positions, values) -> Lattice:
 
 def ringpass_grad(coords0, lat: Lattice, beam: Beam, nturns: int, *, turn: int = 0, ckpt_stride: int = 1,
                   fast: bool = False, timed: bool = False, ctx: Optional[HIPContext] = None,
-                  aperture: Optional[float] = None, time: float = 0.0, dt_turn: float = 0.0) -> Gradients:
+                  aperture: Optional[float] = None, time: float = 0.0, dt_turn: float = 0.0,
+                  exact_hamiltonian: bool = True) -> Gradients:
     ctx = ctx or default_context()
     tab = ctx.table(compile_table(lat, beam, aperture=aperture, turn=turn, nturns=nturns, time=time,
-                                  dt_turn=dt_turn))
+                                  dt_turn=dt_turn, exact_hamiltonian=exact_hamiltonian))
     p = HIPParticles.from_coords(coords0, ctx=ctx)
     n = p.n
     loss = C.c_double()
