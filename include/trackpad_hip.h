@@ -167,8 +167,8 @@ int tp_track_host(tp_ctx* ctx, const tp_table* t, double* coords, int64_t n, int
  * Octupole; 0 for other positions), dL_dx0 (N x 6 column-major host array) = dL/d initial
  * coordinates.  Particles lost during tracking do not enter L and get zero gradient.
  * Supported: every element the tables hold (both Hamiltonians, t1/t2 and r1/r2
- * misalignments) except the Wiggler and an ExactSBend in its drift limit (|angle/L| <
- * 1e-6), which return TP_EUNSUPPORTED, as does a singular r1/r2 matrix.  A time-varying table runs each turn on its row (turns outside the
+ * misalignments) except an ExactSBend in its drift limit (|angle/L| < 1e-6), which returns
+ * TP_EUNSUPPORTED, as does a singular r1/r2 matrix.  A time-varying table runs each turn on its row (turns outside the
  * rows: TP_EINVAL); dL_dk of a position is then the derivative with respect to an offset
  * added to its strength at every turn.  flags: TP_TRACK_FAST / TP_TRACK_TIMED as for
  * tp_track. */

@@ -281,8 +281,7 @@ def test_batch_adjoint_vs_quad_precision_fd(ctx):
 def test_batch_refusals_and_lifetime(ctx):
     beam = _beam()
     ring = _variants(1, seed=12)[0]
-    from trackpad_hip import Wiggler
-    other = Lattice([Drift(1.0), Wiggler(0.4, lw=0.2, Bmax=0.8, Nsteps=4), Quadrupole(0.4, 1.0)])
+    other = Lattice([Drift(1.0), ExactSBend(1.0, 1e-8), Quadrupole(0.4, 1.0)])  # drift-limit bend: no adjoint
     c = _coords(20, seed=90)
     tabs = [ctx.table(compile_table(ring, beam)), ctx.table(compile_table(other, beam))]
     parts = [HIPParticles.from_coords(c, None, ctx) for _ in range(2)]

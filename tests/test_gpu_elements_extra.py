@@ -218,7 +218,7 @@ def test_ring_with_wiggler(ctx):
     loss records identical with the oracle and the coordinates within 1e-12 of the oracle
     relative to each plane's spread (per-evaluation last-place differences of sin / cosh
     between the device and host libraries, accumulated over ~4,000 potential evaluations per
-    particle-turn)."""
+    particle-turn); and gradients through it and through a radiating wiggler."""
     beam = Beam(3.0e9)
     q = lambda k: Quadrupole(0.4, k, num_int_steps=8)
     lat = Lattice([q(1.0), Drift(1.0), Wiggler(0.8, lw=0.2, Bmax=0.9, Nsteps=8, energy=3e9), Drift(1.0), q(-1.0),
@@ -238,6 +238,22 @@ def test_ring_with_wiggler(ctx):
     alive = olf == 0
     rms = np.sqrt(np.mean(oc[alive] ** 2, axis=0))
     assert np.all(np.max(np.abs(gc - oc)[alive], axis=0) <= 1e-12 * rms)
-    # the reverse pass does not cover the wiggler
-    with pytest.raises(TrackPadHIPError, match="TP_EUNSUPPORTED"):
-        ringpass_grad(c0[8:16], lat, beam, 2, ctx=ctx)
+    # reverse mode through the wigglers, and through a radiating one (2nd-order maps and
+    # radiation kicks undone one by one): gradients against the oracle's dual-number
+    # derivatives along a random direction, 1e-9
+    from trackpad_hip import strength_positions
+    rad = Lattice([q(1.0), Drift(1.0), Wiggler(1.0, lw=0.1, Bmax=1.2, Nsteps=8, energy=3e9, rad=1), Drift(1.0),
+                   q(-1.0), Drift(0.5)])
+    c = c0[8:40]
+    rng = np.random.default_rng(64)
+    for lat_ in (lat, rad):
+        g = ringpass_grad(c, lat_, beam, 3, ctx=ctx)
+        assert g.n_alive == len(c)
+        pos = strength_positions(lat_)
+        dk = np.zeros(len(lat_))
+        dk[pos] = rng.normal(0, 1, len(pos))
+        dr0 = rng.normal(0, 1, c.shape) * np.array([5e-5, 2e-5, 1e-5, 2e-6, 1e-4, 1e-3])
+        _, dl, na = O.loss_dir(lat_, beam, c, 3, dk=dk, dr0=dr0)
+        ref = dl / na
+        got = float(np.dot(g.dk, dk) + np.sum(g.dx0 * dr0))
+        assert abs(got - ref) <= 1e-9 * abs(ref), (got, ref)

@@ -633,7 +633,7 @@ struct DevBuf {
 namespace {
 
 // Variant of the reverse kernels: the largest one (TP_FEAT_ADJ_ALL) for Forest exact bends,
-// the linearised drift, the families of tags 8-21 and r1/r2 rotations; a table whose forward
+// the linearised drift, the families of tags 8-22 and r1/r2 rotations; a table whose forward
 // variant is ALL for its t1/t2 translations alone runs the C reverse kernel, which undoes
 // t1/t2 for every variant at run time.
 int adj_variant(unsigned features, int variant, bool rotations) {
@@ -726,7 +726,6 @@ int adj_check(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, in
   if (ckpt_stride < 0) return fail(TP_EINVAL, "%s: ckpt_stride = %d", who, ckpt_stride);
   if (flags & ~(TP_TRACK_FAST | TP_TRACK_TIMED)) return fail(TP_EINVAL, "%s: unsupported flags 0x%x", who, flags);
   for (const tp_rec& r : t->recs) {
-    if (r.tag == TP_WIGGLER) return fail(TP_EUNSUPPORTED, "%s: the wiggler has no adjoint here", who);
     if (r.tag == TP_ESBEND && (r.flags & TP_F_DRIFTLIMIT))
       return fail(TP_EUNSUPPORTED, "%s: exact bends with |angle/L| < 1e-6 (drift limit) have no adjoint here", who);
   }
@@ -1259,7 +1258,6 @@ int tp_batch_create(tp_ctx* ctx, int32_t n_var, tp_table* const* tables, tp_part
     if (b->no_adjoint.empty()) {
       if (!rows_share_slots(t)) b->no_adjoint = "a position changes its main multipole order between rows";
       for (const tp_rec& r : t->recs) {
-        if (r.tag == TP_WIGGLER) b->no_adjoint = "the wiggler has no adjoint here";
         if (r.tag == TP_ESBEND && (r.flags & TP_F_DRIFTLIMIT))
           b->no_adjoint = "exact bends with |angle/L| < 1e-6 (drift limit) have no adjoint here";
       }

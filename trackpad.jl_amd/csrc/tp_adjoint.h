@@ -540,7 +540,8 @@ TP_DEV void mat6_back(St& s, Ad& a, const double* __restrict__ M, const double* 
 //     the entrance z is found by fixed-point steps through the undone kick and drift
 //     (a contraction with factor ~ L |k V x| / 2).
 // VJPs by dual numbers through the same map text (vjp6), except the crab's (by hand) and
-// LBend's edges (edge_back).  The wiggler and its radiation are not covered.
+// LBend's edges (edge_back); the wiggler's 2nd-order maps and radiation kicks one by one
+// (wiggler_back).
 // ---------------------------------------------------------------------------------------
 template <class F>
 TP_DEV Ad vjp6(const St& s, const Ad& a, F&& map) {
@@ -613,6 +614,121 @@ TP_DEV SA crab_back(St s, Ad a, uint32_t fl, const double* __restrict__ p, doubl
   }
   a.z = a.z + a_ph * p[3];
   return SA{s0, a};
+}
+
+// Inverse of wig_map_2nd(dl) started at Zw_in: its shears (functions of x, y only) and drifts
+// (functions of px or py and delta only) undone in reverse order.
+TP_DEV St wig_map_2nd_inv(St s, const double* __restrict__ h, int nh, int nv, double dl, double Zw_in) {
+  const double delta = s.d;
+  const double inv1pd = 1.0 / (1.0 + delta);
+  const double dld = dl * inv1pd;
+  const double dl2 = dl / 2.0;
+  const double dl2d = dl2 * inv1pd;
+  const double Zw = Zw_in + dl2;
+  double x = s.x, px = s.px, y = s.y, py = s.py, z = s.z, a, b;
+  wig_ay(h, nh, nv, x, y, Zw, a, b);
+  px = px - b;
+  py = py - a;
+  z = z - (dl2d / 2.0) * (py * py) * inv1pd;
+  y = y - dl2d * py;
+  wig_ay(h, nh, nv, x, y, Zw, a, b);
+  px = px + b;
+  py = py + a;
+  wig_ax(h, nh, nv, x, y, Zw, a, b);
+  px = px - a;
+  py = py - b;
+  z = z - (dld / 2.0) * (px * px) * inv1pd;
+  x = x - dld * px;
+  wig_ax(h, nh, nv, x, y, Zw, a, b);
+  px = px + a;
+  py = py + b;
+  wig_ay(h, nh, nv, x, y, Zw, a, b);
+  px = px - b;
+  py = py - a;
+  z = z - (dl2d / 2.0) * (py * py) * inv1pd;
+  y = y - dl2d * py;
+  wig_ay(h, nh, nv, x, y, Zw, a, b);
+  px = px + b;
+  py = py + a;
+  return St{x, px, y, py, z, delta};
+}
+
+// Inverse of wig_rad_kick: 1 + delta' = u - K u^2 (u = 1 + delta, K = Crad B^2/H^2 dl) is
+// solved for the root near u', u = 2u' / (1 + sqrt(1 - 4 K u')); px, py are rescaled by
+// 1 - K u^2 between the ax / ay shears.
+TP_DEV St wig_rad_kick_inv(St s, const double* __restrict__ p, const double* __restrict__ h, int nh, int nv,
+                           double Zw) {
+  double ax, axpy, ay, aypx;
+  wig_ax(h, nh, nv, s.x, s.y, Zw, ax, axpy);
+  wig_ay(h, nh, nv, s.x, s.y, Zw, ay, aypx);
+  s.px = s.px - ax;
+  s.py = s.py - ay;
+  const double B2 = wig_b2(h, nh, nv, s.x, s.y, Zw);
+  if (B2 != 0.0) {
+    const double K = p[7] * (B2 / (p[6] * p[6])) * p[8];
+    const double u1 = 1.0 + s.d;
+    const double u = 2.0 * u1 / (1.0 + sqrt(1.0 - 4.0 * K * u1));
+    const double scale = 1.0 - K * u * u;
+    s.px = s.px / scale;
+    s.py = s.py / scale;
+    s.d = u - 1.0;
+  }
+  s.px = s.px + ax;
+  s.py = s.py + ay;
+  return s;
+}
+
+// Zw at the start of 2nd-order map j (0, 1, 2) of step k, accumulated as the forward pass does
+TP_DEV double wig_zw(int k, int j, double dl1, double dl0) {
+  double Zw = 0.0;
+  for (int kk = 0; kk <= k; ++kk)
+    for (int jj = 0; jj < 3; ++jj) {
+      if (kk == k && jj == j) return Zw;
+      const double dl2 = (jj == 1 ? dl0 : dl1) / 2.0;
+      Zw = Zw + dl2;
+      Zw = Zw + dl2;
+    }
+  return Zw;
+}
+
+// Wiggler (src/tracking.jl:1380-1525) backwards: every 2nd-order map and radiation kick undone
+// in reverse order, each with the VJP of the same map text at its reconstructed input.
+TP_DEV SA wiggler_back(St s, Ad a, int nstep, const double* __restrict__ p) {
+  const double dl1 = p[1], dl0 = p[2];
+  const int nh = (int)p[3], nv = (int)p[4];
+  const bool rad = p[5] != 0.0;
+  const double* h = p + 9;
+  for (int k = nstep - 1; k >= 0; --k) {
+    if (rad) {
+      const double Zr = wig_zw(k + 1, 0, dl1, dl0);
+      const St in = wig_rad_kick_inv(s, p, h, nh, nv, Zr);
+      a = vjp6(in, a, [&](SD r) {
+        wig_rad_kick(r, p, h, nh, nv, Zr);
+        return r;
+      });
+      s = in;
+    }
+    for (int j = 2; j >= 0; --j) {
+      const double dl = j == 1 ? dl0 : dl1;
+      const double Zin = wig_zw(k, j, dl1, dl0);
+      const St in = wig_map_2nd_inv(s, h, nh, nv, dl, Zin);
+      a = vjp6(in, a, [&](SD r) {
+        double z = Zin;
+        wig_map_2nd(r, h, nh, nv, dl, z);
+        return r;
+      });
+      s = in;
+    }
+  }
+  if (rad) {
+    const St in = wig_rad_kick_inv(s, p, h, nh, nv, 0.0);
+    a = vjp6(in, a, [&](SD r) {
+      wig_rad_kick(r, p, h, nh, nv, 0.0);
+      return r;
+    });
+    s = in;
+  }
+  return SA{s, a};
 }
 
 __device__ __noinline__ SA extra_back(St s, Ad a, int tag, uint32_t fl, int n, const double* __restrict__ p,
@@ -698,6 +814,8 @@ __device__ __noinline__ SA extra_back(St s, Ad a, int tag, uint32_t fl, int n, c
       edge_back(in, a, 1, p + 5, irho, false);
       break;
     }
+    case TP_WIGGLER:
+      return wiggler_back(s, a, n, p);
     default:  // TP_NAN: a particle that passes it is lost, so none reaches it backwards
       break;
   }

@@ -275,64 +275,71 @@ TP_DEV S boost(S s, const double* __restrict__ p, bool inverse) {
 // Wiggler, src/tracking.jl:1224-1525.  Harmonic h (11 doubles): kx ky kz tz, then the
 // leading factors of the ax, axpy, ay, aypx terms, of Bx and By, and the sinc flag.
 // ---------------------------------------------------------------------------------------
-TP_DEV double sinc_taylor(double x) {  // _sinc_taylor, :1224-1227
-  const double x2 = x * x;
+template <class T>
+TP_DEV T sinc_taylor(T x) {  // _sinc_taylor, :1224-1227
+  const T x2 = x * x;
   return 1.0 - x2 / 6.0 * (1.0 - x2 / 20.0 * (1.0 - x2 / 42.0 * (1.0 - x2 / 72.0)));
 }
 
 // _wig_ax_axpy (:1229-1279) and _wig_ay_aypx (:1281-1331) at (x, y, Zw)
-TP_DEV void wig_ax(const double* __restrict__ h, int nh, int nv, double x, double y, double Zw, double& ax,
-                   double& axpy) {
+template <class T>
+TP_DEV void wig_ax(const double* __restrict__ h, int nh, int nv, T x, T y, double Zw, T& ax, T& axpy) {
   ax = 0.0;
   axpy = 0.0;
   for (int i = 0; i < nh; ++i, h += 11) {
     const double kx = h[0], ky = h[1], kz = h[2], tz = h[3];
-    const double cx = cos(kx * x), chy = cosh(ky * y), sz = sin(kz * Zw + tz);
+    const T cx = gcos(kx * x), chy = gcosh(ky * y);
+    const double sz = sin(kz * Zw + tz);
     ax = ax + h[4] * cx * chy * sz;
-    const double shy = sinh(ky * y);
-    const double sxkx = h[10] != 0.0 ? sin(kx * x) / kx : x * sinc_taylor(kx * x);
+    const T shy = gsinh(ky * y);
+    const T sxkx = h[10] != 0.0 ? gsin(kx * x) / kx : x * sinc_taylor(kx * x);
     axpy = axpy + h[5] * sxkx * shy * sz;
   }
   for (int i = 0; i < nv; ++i, h += 11) {
     const double kx = h[0], ky = h[1], kz = h[2], tz = h[3];
-    const double shx = sinh(kx * x), sy = sin(ky * y), sz = sin(kz * Zw + tz);
+    const T shx = gsinh(kx * x), sy = gsin(ky * y);
+    const double sz = sin(kz * Zw + tz);
     ax = ax + h[4] * shx * sy * sz;
-    const double chx = cosh(kx * x), cy = cos(ky * y);
+    const T chx = gcosh(kx * x), cy = gcos(ky * y);
     axpy = axpy + h[5] * chx * cy * sz;
   }
 }
 
-TP_DEV void wig_ay(const double* __restrict__ h, int nh, int nv, double x, double y, double Zw, double& ay,
-                   double& aypx) {
+template <class T>
+TP_DEV void wig_ay(const double* __restrict__ h, int nh, int nv, T x, T y, double Zw, T& ay, T& aypx) {
   ay = 0.0;
   aypx = 0.0;
   for (int i = 0; i < nh; ++i, h += 11) {
     const double kx = h[0], ky = h[1], kz = h[2], tz = h[3];
-    const double sx = sin(kx * x), shy = sinh(ky * y), sz = sin(kz * Zw + tz);
+    const T sx = gsin(kx * x), shy = gsinh(ky * y);
+    const double sz = sin(kz * Zw + tz);
     ay = ay + h[6] * sx * shy * sz;
-    const double cx = cos(kx * x), chy = cosh(ky * y);
+    const T cx = gcos(kx * x), chy = gcosh(ky * y);
     aypx = aypx + h[7] * cx * chy * sz;
   }
   for (int i = 0; i < nv; ++i, h += 11) {
     const double kx = h[0], ky = h[1], kz = h[2], tz = h[3];
-    const double chx = cosh(kx * x), cy = cos(ky * y), sz = sin(kz * Zw + tz);
+    const T chx = gcosh(kx * x), cy = gcos(ky * y);
+    const double sz = sin(kz * Zw + tz);
     ay = ay + h[6] * chx * cy * sz;
-    const double shx = sinh(kx * x);
-    const double syky = h[10] != 0.0 ? sin(ky * y) / ky : y * sinc_taylor(ky * y);
+    const T shx = gsinh(kx * x);
+    const T syky = h[10] != 0.0 ? gsin(ky * y) / ky : y * sinc_taylor(ky * y);
     aypx = aypx + h[7] * shx * syky * sz;
   }
 }
 
 // _wig_map_2nd, :1333-1379 (Zw is advanced by the caller's copy)
-TP_DEV void wig_map_2nd(St& s, const double* __restrict__ h, int nh, int nv, double dl, double& Zw) {
-  const double delta = s.d;
-  const double inv1pd = 1.0 / (1.0 + delta);
-  const double dld = dl * inv1pd;
+template <class S>
+TP_DEV void wig_map_2nd(S& s, const double* __restrict__ h, int nh, int nv, double dl, double& Zw) {
+  using T = decltype(s.x);
+  const T delta = s.d;
+  const T inv1pd = 1.0 / (1.0 + delta);
+  const T dld = dl * inv1pd;
   const double dl2 = dl / 2.0;
-  const double dl2d = dl2 * inv1pd;
+  const T dl2d = dl2 * inv1pd;
   Zw = Zw + dl2;
-  double x = s.x, px = s.px, y = s.y, py = s.py, z = s.z;
-  double a, b;
+  T x = s.x, px = s.px, y = s.y, py = s.py, z = s.z;
+  T a, b;
   wig_ay(h, nh, nv, x, y, Zw, a, b);
   px = px - b;
   py = py - a;
@@ -358,41 +365,48 @@ TP_DEV void wig_map_2nd(St& s, const double* __restrict__ h, int nh, int nv, dou
   px = px + b;
   py = py + a;
   Zw = Zw + dl2;
-  s = St{x, px, y, py, z, delta};
+  s = S{x, px, y, py, z, delta};
 }
 
-// _wig_B (Bx, By; :1401-1457) and _wig_radiation_kicks (:1459-1470), with the ax / ay
-// kicks around them as _wig_pass_4th_rad applies them (:1472-1507)
-TP_DEV void wig_rad_kick(St& s, const double* __restrict__ p, const double* __restrict__ h, int nh, int nv,
-                         double Zw) {
-  double ax, axpy, ay, aypx;
-  wig_ax(h, nh, nv, s.x, s.y, Zw, ax, axpy);
-  wig_ay(h, nh, nv, s.x, s.y, Zw, ay, aypx);
-  s.px = s.px - ax;
-  s.py = s.py - ay;
-  double Bx = 0.0, By = 0.0;
+// The field's bending term B^2 of _wig_B (Bx, By; :1401-1457) at (x, y, Zw)
+template <class T>
+TP_DEV T wig_b2(const double* __restrict__ h, int nh, int nv, T x, T y, double Zw) {
+  T Bx = 0.0, By = 0.0;
   const double* q = h;
   for (int i = 0; i < nh; ++i, q += 11) {
     const double kx = q[0], ky = q[1], kz = q[2], tz = q[3];
-    const double sx = sin(kx * s.x), cx = cos(kx * s.x), chy = cosh(ky * s.y), shy = sinh(ky * s.y);
+    const T sx = gsin(kx * x), cx = gcos(kx * x), chy = gcosh(ky * y), shy = gsinh(ky * y);
     const double cz = cos(kz * Zw + tz);
     Bx = Bx + q[8] * sx * shy * cz;
     By = By - q[9] * cx * chy * cz;
   }
   for (int i = 0; i < nv; ++i, q += 11) {
     const double kx = q[0], ky = q[1], kz = q[2], tz = q[3];
-    const double shx = sinh(kx * s.x), chx = cosh(kx * s.x), sy = sin(ky * s.y), cy = cos(ky * s.y);
+    const T shx = gsinh(kx * x), chx = gcosh(kx * x), sy = gsin(ky * y), cy = gcos(ky * y);
     const double cz = cos(kz * Zw + tz);
     Bx = Bx + q[8] * chx * cy * cz;
     By = By - q[9] * shx * sy * cz;
   }
-  const double B2 = Bx * Bx + By * By;
-  if (B2 != 0.0) {
+  return Bx * Bx + By * By;
+}
+
+// _wig_radiation_kicks (:1459-1470), with the ax / ay kicks around them as
+// _wig_pass_4th_rad applies them (:1472-1507)
+template <class S>
+TP_DEV void wig_rad_kick(S& s, const double* __restrict__ p, const double* __restrict__ h, int nh, int nv, double Zw) {
+  using T = decltype(s.x);
+  T ax, axpy, ay, aypx;
+  wig_ax(h, nh, nv, s.x, s.y, Zw, ax, axpy);
+  wig_ay(h, nh, nv, s.x, s.y, Zw, ay, aypx);
+  s.px = s.px - ax;
+  s.py = s.py - ay;
+  const T B2 = wig_b2(h, nh, nv, s.x, s.y, Zw);
+  if (val(B2) != 0.0) {
     const double H = p[6];
-    const double irho2 = B2 / (H * H);
-    const double dFactor = (1.0 + s.d) * (1.0 + s.d);
-    const double dDelta = -p[7] * dFactor * irho2 * p[8];
-    const double scale = 1.0 + dDelta;
+    const T irho2 = B2 / (H * H);
+    const T dFactor = (1.0 + s.d) * (1.0 + s.d);
+    const T dDelta = -p[7] * dFactor * irho2 * p[8];
+    const T scale = 1.0 + dDelta;
     s.px = s.px * scale;
     s.py = s.py * scale;
     s.d = s.d + dDelta;
