@@ -234,6 +234,32 @@ def test_survivor_compaction_between_launches(ctx):
     one = gpu_track(ctx, lat, beam, wide, 6)
     many = gpu_track(ctx, lat, beam, wide, 6, tpl=1)
     assert one[1].all() and all(np.array_equal(a, b) for a, b in zip(one, many))
+    # a scan continued in chunks: each call compacts the particles lost in earlier calls out
+    # before its first launch; the records and coordinates equal the one-call run's
+    from trackpad_hip import HIPParticles, compile_table, track_
+    tab = ctx.table(compile_table(lat, beam))
+    for fast in (False, True):
+        one = gpu_track(ctx, lat, beam, grid, 12, fast=fast)
+        ctx.set_launch(0, 0, 2)
+        try:
+            p = HIPParticles.from_coords(grid, None, ctx)
+            for t0 in (0, 4, 8):
+                track_(p, tab, nturns=4, turn=t0, fast=fast)
+            c, lf, lt, le = p.download()
+        finally:
+            ctx.set_launch(0, 0, 0)
+        for a, b in zip(one, (np.ascontiguousarray(c), lf, lt, le)):
+            assert np.array_equal(a, b), fast
+    # every particle already lost: a call launches nothing and changes nothing
+    p = HIPParticles.from_coords(wide, None, ctx)
+    track_(p, tab, nturns=6)
+    before = p.download()
+    ctx.set_launch(0, 0, 1)
+    try:
+        track_(p, tab, nturns=3, turn=6)
+    finally:
+        ctx.set_launch(0, 0, 0)
+    assert all(np.array_equal(a, b) for a, b in zip(before, p.download()))
 
 
 def test_da_grid_orders_to_10(ctx):

@@ -511,9 +511,10 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
                 (long long)turn0 + nturns, a.row_turn0, (long long)a.row_turn0 + a.n_rows);
   const bool timed = (flags & TP_TRACK_TIMED) != 0;
   const int chunk = ctx->tpl > 0 ? ctx->tpl : (nturns > 0 ? nturns : 1);
-  // Between launches the survivors are compacted into a particle list (compact_alive), so
-  // lanes of lost particles stop costing issue slots in the following launches.  It needs a
-  // count read back per launch, so asynchronous calls and single launches skip it.
+  // Before the first launch and between launches the survivors are compacted into a
+  // particle list (compact_alive), so lanes of lost particles stop costing issue slots in the
+  // following launches.  It needs a count read back per launch, so asynchronous calls and
+  // single launches skip it.
   const bool compact = !(flags & TP_TRACK_ASYNC) && nturns > chunk && p->n <= 0x7fffffffLL;
   if (compact && !p->d_order) TP_HIP(hipMalloc(&p->d_order, (size_t)(2 * p->n + 1) * sizeof(int32_t)));
   int32_t* lists[2] = {p->d_order, p->d_order ? p->d_order + p->n : nullptr};
@@ -533,7 +534,24 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
   // particles sends the rest straight to `chunk`, so a beam that loses nothing pays one
   // extra launch.
   int32_t len = compact ? 1 : chunk;
-  if (p->n > 0 && a.n_seq > 0) {
+  bool any_alive = true;
+  if (compact && p->n > 0 && a.n_seq > 0) {
+    // particles lost in earlier calls (a scan continued in chunks) leave the list before the
+    // first launch
+    hipError_t e = tp_launch_fill_i32(d_count, 0, 1, ctx->stream);
+    if (e == hipSuccess) e = tp_launch_compact_alive(nullptr, (int32_t)a.n, p->d_lt, lists[0], d_count, ctx->stream);
+    if (e != hipSuccess) return fail(TP_EHIP, "tp_track: compaction launch failed: %s", hipGetErrorName(e));
+    int32_t alive = 0;
+    TP_HIP(hipMemcpyAsync(&alive, d_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    any_alive = alive > 0;
+    if (alive > 0 && alive < a.n) {
+      cur = 0;
+      a.order = lists[cur];
+      a.n = alive;
+    }
+  }
+  if (p->n > 0 && a.n_seq > 0 && any_alive) {
     for (int32_t done = 0; done < nturns; done += a.nturns) {
       a.turn0 = turn0 + done;
       a.nturns = (nturns - done < len) ? (nturns - done) : len;
