@@ -528,11 +528,11 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
   const int max_grid = occ > 0 ? occ * ctx->cus : 0;
   double ms_total = 0.0;
   int launches = 0;
-  // With compaction the launches start short and double up to `chunk` turns (1, 2, 4, ...)
-  // while they lose particles: a lossy beam (a dynamic-aperture grid loses most of its
-  // particles in the first turns) is compacted early; a launch that loses under 1 % of its
-  // particles sends the rest straight to `chunk`, so a beam that loses nothing pays one
-  // extra launch.
+  // With compaction the launches start at one turn and follow the losses: one turn while
+  // they lose 2 % of their particles per turn or more and the survivors fill the device (a
+  // dynamic-aperture grid loses most of its particles in the first turns), doubling below
+  // that, straight to `chunk` turns once a launch loses under 1 %, so a beam that loses
+  // nothing pays one extra launch.
   int32_t len = compact ? 1 : chunk;
   bool any_alive = true;
   if (compact && p->n > 0 && a.n_seq > 0) {
@@ -555,7 +555,6 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
     for (int32_t done = 0; done < nturns; done += a.nturns) {
       a.turn0 = turn0 + done;
       a.nturns = (nturns - done < len) ? (nturns - done) : len;
-      if (len < chunk) len = (len > chunk / 2) ? chunk : 2 * len;
       const int64_t blocks = (a.n + (int64_t)ctx->block * ctx->ppl - 1) / ((int64_t)ctx->block * ctx->ppl);
       a.counter = (max_grid > 0 && blocks > max_grid) ? ctx->d_counter : nullptr;
       hipError_t e = hipSuccess;
@@ -584,7 +583,15 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
         TP_HIP(hipMemcpyAsync(&alive, d_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         TP_HIP(hipStreamSynchronize(ctx->stream));
         if (alive == 0) break;  // nothing left to track
-        if (alive >= a.n - a.n / 100) len = chunk;
+        // next launch: the full chunk once a launch loses under 1 %, one turn while a launch
+        // loses 2 % or more per turn and the survivors still fill the device (compacting after
+        // every turn; a smaller beam leaves SIMDs idle anyway), else twice as long
+        const double lost = (double)(a.n - alive) / (double)a.n;
+        const int64_t alive_blocks = ((int64_t)alive + (int64_t)ctx->block * ctx->ppl - 1) /
+                                     ((int64_t)ctx->block * ctx->ppl);
+        if (lost < 0.01) len = chunk;
+        else if (lost >= 0.02 * a.nturns && max_grid > 0 && alive_blocks > max_grid) len = 1;
+        else len = (int32_t)std::min<int64_t>(2 * (int64_t)a.nturns, chunk);
         if (alive < a.n) {      // switch to the shorter list (else keep the current one)
           cur = nxt;
           a.order = lists[cur];
