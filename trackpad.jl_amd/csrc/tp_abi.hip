@@ -723,9 +723,12 @@ int ensure_minv(tp_ctx* ctx, const tp_table* t, const char* who) {
   std::vector<double> minv(t->pool.size(), 0.0);
   if (!rotation_inverses(t->recs, t->pool, 0, minv))
     return fail(TP_EUNSUPPORTED, "%s: a singular r1/r2 matrix has no inverse", who);
-  TP_HIP(hipMalloc(&tm->d_minv, minv.size() * sizeof(double)));
-  TP_HIP(hipMemcpyAsync(tm->d_minv, minv.data(), minv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  DevBuf buf;
+  TP_HIP(hipMalloc(&buf.p, minv.size() * sizeof(double)));
+  TP_HIP(hipMemcpyAsync(buf.p, minv.data(), minv.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
   TP_HIP(hipStreamSynchronize(ctx->stream));
+  tm->d_minv = (double*)buf.p;  // kept by the table from here on
+  buf.p = nullptr;
   return TP_OK;
 }
 
@@ -798,9 +801,12 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
     int32_t ns = 0;
     for (uint32_t e = 0; e < t->hdr.n_seq; ++e)
       if (t->recs[t->seq[e]].gorder >= 0) slots[e] = ns++;
-    TP_HIP(hipMalloc(&tm->d_slots, slots.size() * sizeof(int32_t)));
-    TP_HIP(hipMemcpyAsync(tm->d_slots, slots.data(), slots.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    DevBuf buf;
+    TP_HIP(hipMalloc(&buf.p, slots.size() * sizeof(int32_t)));
+    TP_HIP(hipMemcpyAsync(buf.p, slots.data(), slots.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     TP_HIP(hipStreamSynchronize(ctx->stream));
+    tm->d_slots = (int32_t*)buf.p;  // kept by the table from here on
+    buf.p = nullptr;
     tm->n_slots = ns;
   }
   const int32_t n_slots = tm->n_slots;
