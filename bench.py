@@ -31,13 +31,17 @@ magnet with order-10 normal and skew errors) on the 1024 x 1024 x 3 grid (3,145,
 particles); one step is --turns turns (default 20: the full 10,000-turn scan is ~45 minutes
 of GPU time, and the rate is independent of the turn count).
 
+Workload ``da_lossless``: the same ring on 1e6 particles of the DBA beam's distribution at
+half its size (none lost), 5 turns per step: the kernel's rate on the order-10 lattice
+without the scan's losses.
+
 Workload ``diff``: the differentiation configuration -- dL/d(120 k1, 80 k2, x0) of
 L = mean(x^2 + y^2) after 100 turns of the DBA ring over 4,096 particles; one step is one
 tp_adjoint call (forward with checkpoints + reverse pass + reduction).  ``value`` is
 gradient particle-turns/s over the device time of the two passes; the line adds their cost
 relative to plain tracking of the same beam.
 
-Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--workload dba|fodo|da|diff|batched]
+Usage: python bench.py [--gpus 1] [--steps 2] [--warmup 1] [--workload dba|fodo|da|da_lossless|diff|batched]
                        [--kernel fast|exact] [--turns-per-launch 100]
 """
 
@@ -97,6 +101,13 @@ def workload(name, n_override, t_override):
             # a sample of the grid spread over all three momenta and the whole (x, y) square
             return g if n_ >= len(g) else g[np.linspace(0, len(g) - 1, n_).astype(np.int64)]
 
+        aperture = None
+    elif name == "da_lossless":
+        # the dynamic-aperture ring on a beam it keeps (the DBA distribution at half size):
+        # the kernel's own rate on the order-10 lattice, without the scan's losses
+        lat = W.da_ring(beam=beam)
+        n, T = 1_000_000, 5
+        mk = lambda n_: W.dba_particles(n_, seed=3) * 0.5
         aperture = None
     else:
         raise SystemExit(f"unknown workload {name!r}")
@@ -402,7 +413,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 2; batched 1)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (default 1; batched 0)")
-    ap.add_argument("--workload", default="dba", choices=["dba", "fodo", "da", "diff", "batched"])
+    ap.add_argument("--workload", default="dba", choices=["dba", "fodo", "da", "da_lossless", "diff", "batched"])
     ap.add_argument("--variants", type=int, default=256, help="batched: number of lattice variants")
     ap.add_argument("--turn-scale", type=float, default=1.0, help="batched: scale every T_v (1 = the workload)")
     ap.add_argument("--compare-turn-scale", type=float, default=0.05,

@@ -47,5 +47,7 @@ def test_workload_sizes_and_flops():
     # a spread sample of the grid: every momentum, the full square
     assert s.shape == (2048, 6) and set(np.unique(s[:, 5])) == {-0.02, 0.0, 0.02}
     assert s[:, 0].min() == pytest.approx(-0.015) and s[:, 2].max() == pytest.approx(0.015, abs=1e-4)
+    beam, lat, n, T, mk, ap = bench.workload("da_lossless", 0, 0)
+    assert (n, T, len(lat)) == (1_000_000, 5, 2001) and mk(8).shape == (8, 6)
     beam, lat, n, T, mk, ap = bench.workload("dba", 4096, 100)
     assert (n, T) == (4096, 100) and mk(16).shape == (16, 6)
