@@ -229,6 +229,18 @@ def test_survivor_compaction_between_launches(ctx):
         lost = one[1].astype(bool)
         assert 0.3 < lost.mean() < 0.95, lost.mean()
         assert (one[2][lost] >= 0).all() and (one[2][lost] < 12).all()
+    # the grid loses particles fast enough that launches cover quarter turns (more launches
+    # than turns); the records above are the same bits all the same
+    from trackpad_hip import HIPParticles, compile_table, track_
+    ctx.set_launch(0, 0, 3)
+    try:
+        p = HIPParticles.from_coords(grid, None, ctx)
+        track_(p, ctx.table(compile_table(lat, beam)), nturns=12, timed=True)
+        assert ctx.timing()["launches"] > 12
+        for a, b in zip(gpu_track(ctx, lat, beam, grid, 12), p.download()):
+            assert np.array_equal(a, np.ascontiguousarray(b))
+    finally:
+        ctx.set_launch(0, 0, 0)
     # all particles lost early: the remaining launches are skipped, records unchanged
     wide = W.da_grid(4, 4, half=0.5)
     one = gpu_track(ctx, lat, beam, wide, 6)
@@ -236,7 +248,6 @@ def test_survivor_compaction_between_launches(ctx):
     assert one[1].all() and all(np.array_equal(a, b) for a, b in zip(one, many))
     # a scan continued in chunks: each call compacts the particles lost in earlier calls out
     # before its first launch; the records and coordinates equal the one-call run's
-    from trackpad_hip import HIPParticles, compile_table, track_
     tab = ctx.table(compile_table(lat, beam))
     for fast in (False, True):
         one = gpu_track(ctx, lat, beam, grid, 12, fast=fast)

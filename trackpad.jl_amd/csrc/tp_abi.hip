@@ -528,12 +528,14 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
   const int max_grid = occ > 0 ? occ * ctx->cus : 0;
   double ms_total = 0.0;
   int launches = 0;
-  // With compaction the launches start at one turn and follow the losses: one turn while
-  // they lose 2 % of their particles per turn or more and the survivors fill the device (a
-  // dynamic-aperture grid loses most of its particles in the first turns), doubling below
-  // that, straight to `chunk` turns once a launch loses under 1 %, so a beam that loses
-  // nothing pays one extra launch.
+  // With compaction the launches follow the losses.  The first covers a quarter of a turn;
+  // while launches lose 8 % of their particles per turn or more (a dynamic-aperture grid
+  // loses most of its particles in its first turns) they stay a quarter turn long, so lost
+  // particles leave the lanes within the turn; at 2 % per turn or more they are one turn
+  // long, below that they double, and once a launch loses under 1 % the rest runs in
+  // launches of `chunk` turns (a beam that loses nothing pays one short extra launch).
   int32_t len = compact ? 1 : chunk;
+  int32_t parts = compact ? 4 : 1;  // > 1: the next launch covers 1/parts of a turn
   bool any_alive = true;
   if (compact && p->n > 0 && a.n_seq > 0) {
     // particles lost in earlier calls (a scan continued in chunks) leave the list before the
@@ -552,9 +554,19 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
     }
   }
   if (p->n > 0 && a.n_seq > 0 && any_alive) {
-    for (int32_t done = 0; done < nturns; done += a.nturns) {
+    int32_t done = 0, epos = 0;  // whole turns tracked, and elements of turn `done`
+    while (done < nturns) {
       a.turn0 = turn0 + done;
-      a.nturns = (nturns - done < len) ? (nturns - done) : len;
+      a.e_first = epos;
+      int32_t e_end = a.n_seq;  // the launch ends before this element of its last turn
+      if (parts > 1) {
+        a.nturns = 1;
+        const int32_t step = (a.n_seq + parts - 1) / parts;
+        e_end = (a.n_seq - epos > step) ? epos + step : a.n_seq;
+      } else {
+        a.nturns = (nturns - done < len) ? (nturns - done) : len;
+      }
+      a.e_last = e_end < a.n_seq ? e_end : 0;
       const int64_t blocks = (a.n + (int64_t)ctx->block * ctx->ppl - 1) / ((int64_t)ctx->block * ctx->ppl);
       a.counter = (max_grid > 0 && blocks > max_grid) ? ctx->d_counter : nullptr;
       hipError_t e = hipSuccess;
@@ -572,7 +584,16 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
         TP_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ms_total += ms;
       }
-      if (compact && done + a.nturns < nturns) {
+      // turns the launch covered, then the position after it
+      const double covered = (double)(a.nturns - 1) + (double)(e_end - epos) / (double)a.n_seq;
+      if (e_end == a.n_seq) {
+        done += a.nturns;
+        epos = 0;
+      } else {
+        done += a.nturns - 1;
+        epos = e_end;
+      }
+      if (compact && done < nturns) {
         const int nxt = cur == 1 ? 0 : 1;
         e = tp_launch_fill_i32(d_count, 0, 1, ctx->stream);
         if (e == hipSuccess)
@@ -583,14 +604,12 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
         TP_HIP(hipMemcpyAsync(&alive, d_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         TP_HIP(hipStreamSynchronize(ctx->stream));
         if (alive == 0) break;  // nothing left to track
-        // next launch: the full chunk once a launch loses under 1 %, one turn while a launch
-        // loses 2 % or more per turn and the survivors still fill the device (compacting after
-        // every turn; a smaller beam leaves SIMDs idle anyway), else twice as long
         const double lost = (double)(a.n - alive) / (double)a.n;
-        const int64_t alive_blocks = ((int64_t)alive + (int64_t)ctx->block * ctx->ppl - 1) /
-                                     ((int64_t)ctx->block * ctx->ppl);
-        if (lost < 0.01) len = chunk;
-        else if (lost >= 0.02 * a.nturns && max_grid > 0 && alive_blocks > max_grid) len = 1;
+        const double per_turn = lost / (covered > 0.0 ? covered : 1.0);
+        parts = 1;
+        if (per_turn >= 0.08) parts = 4;
+        else if (per_turn >= 0.02) len = 1;
+        else if (lost < 0.01) len = chunk;
         else len = (int32_t)std::min<int64_t>(2 * (int64_t)a.nturns, chunk);
         if (alive < a.n) {      // switch to the shorter list (else keep the current one)
           cur = nxt;

@@ -57,6 +57,10 @@ struct TpKArgs {
   int32_t* lost_elem;
   int32_t turn0;
   int32_t nturns;
+  // Element range of a launch: the first turn starts at element e_first, the last ends before
+  // element e_last (0 = at the end of the turn); 0 / 0 = whole turns.
+  int32_t e_first;
+  int32_t e_last;
   int32_t variant;  // TP_VARIANT_*
   int32_t waves;    // register budget of the kernel: 0 (compiler's), 5 or 6 waves per SIMD
   int32_t n_rows;   // sequence rows (1: static lattice; > 1: one row per turn from row_turn0)

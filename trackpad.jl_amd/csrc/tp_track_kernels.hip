@@ -531,16 +531,19 @@ TP_DEV void run_turns(St* s, Cx* c, bool* alive, const int32_t* __restrict__ seq
     if (!__any(anyl)) break;
     // the lattice of this turn: row (turn - row_turn0) of a time-varying table
     const int32_t* __restrict__ seq_t = seq + (a.n_rows > 1 ? (int64_t)(a.turn0 + t - a.row_turn0) * a.n_seq : 0);
+    // elements of this turn: a launch may start inside its first turn and end inside its last
+    const int32_t eb = t == 0 ? a.e_first : 0;
+    const int32_t ee = (t == a.nturns - 1 && a.e_last > 0) ? a.e_last : a.n_seq;
     // Software-pipelined scalar loads: the record of element e+1 and the sequence entry of
     // element e+2 are requested while element e runs, so each element waits for one
     // round trip of scalar loads (its parameters) instead of three.
-    int32_t ri_next = a.n_seq > 1 ? seq_t[1] : 0;
-    tp_rec rec_next = a.n_seq > 0 ? recs[seq_t[0]] : tp_rec{};
-    for (int32_t e = 0; e < a.n_seq; ++e) {
+    int32_t ri_next = eb + 1 < ee ? seq_t[eb + 1] : 0;
+    tp_rec rec_next = eb < ee ? recs[seq_t[eb]] : tp_rec{};
+    for (int32_t e = eb; e < ee; ++e) {
       const tp_rec rec = rec_next;
-      if (e + 1 < a.n_seq) {
+      if (e + 1 < ee) {
         rec_next = recs[ri_next];
-        if (e + 2 < a.n_seq) ri_next = seq_t[e + 2];
+        if (e + 2 < ee) ri_next = seq_t[e + 2];
       }
       St s0[PPL];
 #pragma unroll
