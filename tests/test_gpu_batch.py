@@ -43,6 +43,25 @@ def _same(a, b, fast, what):
         assert np.all(np.abs(a - b) <= 1e-11 * np.where(scale > 0, scale, 1.0)), what
 
 
+def _cross_variant_close(g, r, v, fast):
+    """Variant v of a batched adjoint against its own tp_adjoint call, when the batch runs a
+    kernel variant with more features than the call does.  Parity build: loss within 1e-13,
+    dx0 bit for bit, dk within 1e-12 of the largest.  Fast build: the kernel variants round
+    differently (FMA contraction is the compiler's choice per kernel, and the drifts of the
+    variants without general kicks use the series form of 1/pz), and the differences grow
+    through the turns, so loss, dx0 and dk agree within the adjoint's own accuracy, 1e-9."""
+    tol = 1e-9 if fast else 1e-13
+    assert abs(g.loss[v] - r.loss) <= tol * abs(r.loss), (v, abs(g.loss[v] - r.loss) / abs(r.loss))
+    if fast:
+        scale = np.max(np.abs(r.dx0), axis=0, keepdims=True)
+        assert np.all(np.abs(g.dx0[v] - r.dx0) <= 1e-9 * scale), v
+    else:
+        assert np.array_equal(g.dx0[v], r.dx0), v
+    scale = np.max(np.abs(r.dk))
+    tol = 1e-9 if fast else 1e-12
+    assert np.all(np.abs(g.dk[v] - r.dk) <= tol * scale), (v, np.max(np.abs(g.dk[v] - r.dk)) / scale)
+
+
 def _per_variant_track(ctx, tabs, coords, flags, ts, t0s, fast):
     out = []
     for tab, c, lf, t, t0 in zip(tabs, coords, flags, ts, t0s):
@@ -199,10 +218,7 @@ def test_batch_adjoint_forest_variant(ctx, fast):
     for v, (lat, c, t) in enumerate(zip(lats, coords, ts)):
         r = ringpass_grad(c, lat, beam, int(t), ckpt_stride=2, fast=fast, ctx=ctx)
         assert g.n_alive[v] == r.n_alive == ns[v]
-        assert abs(g.loss[v] - r.loss) <= 1e-13 * abs(r.loss), v
-        _same(g.dx0[v], r.dx0, fast, v)
-        scale = np.max(np.abs(r.dk))
-        assert np.all(np.abs(g.dk[v] - r.dk) <= 1e-12 * scale), (v, np.max(np.abs(g.dk[v] - r.dk)) / scale)
+        _cross_variant_close(g, r, v, fast)
 
 
 @pytest.mark.parametrize("fast", [False, True])
@@ -226,17 +242,7 @@ def test_batch_adjoint_every_family(ctx, fast):
     for v, (lat, c, t) in enumerate(zip(lats, coords, ts)):
         r = ringpass_grad(c, lat, beam, int(t), ckpt_stride=2, fast=fast, ctx=ctx)
         assert g.n_alive[v] == r.n_alive == ns[v]
-        assert abs(g.loss[v] - r.loss) <= 1e-13 * abs(r.loss), v
-        if fast:
-            # the DBA variant runs the all-families reverse kernel in the batch and the B kernel
-            # alone: FMA contraction differs between the two, within the adjoint's 1e-9
-            scale = np.max(np.abs(r.dx0), axis=0, keepdims=True)
-            assert np.all(np.abs(g.dx0[v] - r.dx0) <= 1e-9 * scale), v
-        else:
-            assert np.array_equal(g.dx0[v], r.dx0), v
-        scale = np.max(np.abs(r.dk))
-        tol = 1e-9 if fast else 1e-12
-        assert np.all(np.abs(g.dk[v] - r.dk) <= tol * scale), (v, np.max(np.abs(g.dk[v] - r.dk)) / scale)
+        _cross_variant_close(g, r, v, fast)
 
 
 def test_batch_adjoint_vs_quad_precision_fd(ctx):

@@ -425,7 +425,7 @@ struct SA {
 
 // yrot backwards: s enters as the output and leaves as the input (yrot with -phi).  x and px
 // are replaced, y and z incremented, py and delta pass through.
-__device__ __noinline__ SA yrot_back(St s, Ad a, Cx c, double co, double si) {
+__device__ __noinline__ SA yrot_back(St s, Ad a, CxE c, double co, double si) {
   s = yrot(s, c, co, -si);
   double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
 #pragma nounroll
@@ -446,7 +446,7 @@ __device__ __noinline__ SA yrot_back(St s, Ad a, Cx c, double co, double si) {
 }
 
 // bend_edge backwards (bend_edge with -theta), same structure as yrot
-__device__ __noinline__ SA wedge_back(St s, Ad a, Cx c, double rhoinv, double co, double si, double s2t, double theta) {
+__device__ __noinline__ SA wedge_back(St s, Ad a, CxE c, double rhoinv, double co, double si, double s2t, double theta) {
   s = bend_edge(s, c, rhoinv, co, -si, -s2t, -theta);
   double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
 #pragma nounroll
@@ -469,7 +469,7 @@ __device__ __noinline__ SA wedge_back(St s, Ad a, Cx c, double rhoinv, double co
 
 // bend_fringe backwards: y is replaced, x incremented, py and z decremented; px and delta
 // pass through.  Five fixed-point steps for the input py (each gains ~6 digits or more).
-__device__ __noinline__ SA fringe_back(St s, Ad a, Cx c, double b0, double gK) {
+__device__ __noinline__ SA fringe_back(St s, Ad a, CxE c, double b0, double gK) {
   const double yf = s.y;
   double py = s.py, phi, dpx, dpy, dd;
 #pragma nounroll

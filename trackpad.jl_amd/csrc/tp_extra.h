@@ -108,14 +108,14 @@ TP_DEV St crab(St s, uint32_t fl, const double* __restrict__ p, double beti) {
   Cx c;
   if (fl & TP_F_RF_DRIFT) {
     mkctx(s, c, beti);
-    drift<EXH>(s, c, p[0], p[1]);
+    drift<EXH, false>(s, c, p[0], p[1]);
   }
   const double a = ang * beti;
   s.px = s.px + p[2] * sin(a);
   s.d = s.d - p[5] * s.x * cos(a);
   if (fl & TP_F_RF_DRIFT) {
     mkctx(s, c, beti);
-    drift<EXH>(s, c, p[0], p[1]);
+    drift<EXH, false>(s, c, p[0], p[1]);
   }
   return s;
 }

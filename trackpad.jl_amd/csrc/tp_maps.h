@@ -7,7 +7,8 @@
 //              transcendentals (asin, sin, tan, atan, cos) come from the device math
 //              library and agree to a few ulp.
 //   TP_FAST=1  with -ffp-contract=fast: FMA contraction, reciprocal square roots by
-//              Newton-Raphson, multiplications by hoisted reciprocals and a series
+//              a refined hardware estimate or (drifts at small angles) a series about
+//              a hoisted 1/sqrt(A), multiplications by hoisted reciprocals and a series
 //              asin for small arguments.  Validated against the parity build and the
 //              oracle to tolerance (1e-12 relative after one turn).
 //
@@ -36,8 +37,16 @@ struct St {
 // pz2 = ((1 + (2 delta) beti) + delta^2) - px^2 - py^2 and of (beti + delta)
 // (src/tracking.jl:90, 98); BB = dp1^2 of pxyz (src/tracking.jl:718) with dp1 = beti+delta;
 // C = 1 + delta of the linearised drift (src/tracking.jl:102).
-struct Cx {
+// The fast build adds R = 1/sqrt(A) and IA = k/A, k = sqrt(3)/2, for the drift's series
+// form (inv_pz); the out-of-line maps take only the base (CxE), so the additions are not
+// passed to them.
+struct CxE {
   double A, B, BB, C;
+};
+struct Cx : CxE {
+#if TP_FAST
+  double R, IA;
+#endif
 };
 
 TP_DEV void mkctx(const St& s, Cx& c, double beti) {
@@ -45,6 +54,11 @@ TP_DEV void mkctx(const St& s, Cx& c, double beti) {
   c.B = beti + s.d;
   c.BB = c.B * c.B;
   c.C = 1.0 + s.d;
+#if TP_FAST
+  // correctly rounded: these are formed once per change of delta, not per drift
+  c.R = 1.0 / sqrt(c.A);
+  c.IA = 0.8660254037844386 / c.A;
+#endif
 }
 
 #if TP_FAST
@@ -58,6 +72,28 @@ TP_DEV double rsqrt_nr(double v) {
   const double t = v * y;
   const double e = __builtin_fma(-t, y, 1.0);
   return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
+}
+// L / pz of the exact-Hamiltonian drift, pz = sqrt(A - px^2 - py^2), A = (1 + delta)^2 form
+// hoisted in Cx.  With w = (px^2 + py^2) / A,
+//   L / pz = (L / sqrt(A)) (1 - w)^(-1/2) = (L R) (1 + w/2 + 3w^2/8 + 5w^3/16 + ...),
+// and for w < 2^-14 the terms after w^3 sum to below 35/128 w^4 / (1 - w) < 2^-57 relative,
+// so the cubic gives L / pz within about two ulp from full-rate FMAs only (the hardware
+// reciprocal square root of rsqrt_nr issues at a quarter of the FMA rate).  The cubic runs
+// on v = k w, k = sqrt(3)/2 (IA = k/A), which makes the w^2 coefficient the inline constant
+// 1/2:
+//   L / pz = L R + v (Q1 L R + L R v (1/2 + Q3 v)),  Q1 = 1/(2k),  Q3 = 5/(16 k^3),
+// so the one remaining constant sits in a scalar register and L R, Q1 L R are per element
+// and particle (hoisted out of a magnet's steps); the coefficients' roundings weigh in with
+// a factor v < 2^-14.  Larger angles (and NaN, pz2 <= 0, which has w >= 1) take rsqrt_nr
+// lane by lane, so each lane's result depends only on its own coordinates.
+TP_DEV double inv_pz(const St& s, const Cx& c, double L) {
+  const double v = (s.px * s.px + s.py * s.py) * c.IA;
+  if (__builtin_expect(v < 0.8660254037844386 * 0x1p-14, 1)) {
+    const double Lr = L * c.R;
+    const double t = __builtin_fma(v, 0.4811252243246881, 0.5);                // Q3, 1/2
+    return __builtin_fma(v, __builtin_fma(Lr * v, t, Lr * 0.5773502691896257), Lr);  // Q1
+  }
+  return L * rsqrt_nr(c.A - s.px * s.px - s.py * s.py);
 }
 // sqrt(v) = v / sqrt(v) = t (1 - e)^(-1/2) with t = v y, from the same estimate
 TP_DEV double sqrt_nr(double v) {
@@ -132,20 +168,20 @@ TP_DEV double nan_d() { return __builtin_nan(""); }
 // ---------------------------------------------------------------------------------------
 // drift6, src/tracking.jl:87-108
 // ---------------------------------------------------------------------------------------
-template <bool EXH>
+template <bool EXH, bool SER = true>
 TP_DEV void drift(St& s, const Cx& c, double L, double Lb) {
   if (EXH) {
-    double pz2 = c.A - s.px * s.px - s.py * s.py;
 #if TP_FAST
-    // NaN when pz2 <= 0 (rsqrt_nr), so x, y, z become NaN and the particle is lost at
+    // NaN when pz2 <= 0 (inv_pz), so x, y, z become NaN and the particle is lost at
     // this element's exit with its entry coordinates, as after the reference's NaN map.
     // (Yoshida steps use drift_xy and sum the path length per step.)
     // The path-length change NormL B - L beti is formed with one rounding.
-    const double NormL = L * rsqrt_nr(pz2);
+    const double NormL = SER ? inv_pz(s, c, L) : L * rsqrt_nr(c.A - s.px * s.px - s.py * s.py);
     s.x = s.x + NormL * s.px;
     s.y = s.y + NormL * s.py;
     s.z = s.z + __builtin_fma(NormL, c.B, -Lb);
 #else
+    double pz2 = c.A - s.px * s.px - s.py * s.py;
     double NormL = L / sqrt(pz2);
     NormL = (pz2 <= 0.0) ? nan_d() : NormL;  // reference returns NaN x 6 (:91-94)
     s.x = s.x + NormL * s.px;
@@ -166,9 +202,9 @@ TP_DEV void drift(St& s, const Cx& c, double L, double Lb) {
 // z += (beti + delta) sum(NormL) - sum(L beti), one FMA per step instead of two operations
 // per drift; the one rounding of the step's sum is below the reference's own rounding of
 // z + NormL (beti + delta) at every drift.
+template <bool SER>
 TP_DEV double drift_xy(St& s, const Cx& c, double L) {
-  const double pz2 = c.A - s.px * s.px - s.py * s.py;
-  const double NormL = L * rsqrt_nr(pz2);
+  const double NormL = SER ? inv_pz(s, c, L) : L * rsqrt_nr(c.A - s.px * s.px - s.py * s.py);
   s.x = s.x + NormL * s.px;
   s.y = s.y + NormL * s.py;
   return NormL;
@@ -296,7 +332,7 @@ TP_DEV void edge_lin(St& s, int method, const double* __restrict__ blk, double i
 // Exact-bend helpers (Forest), src/tracking.jl:717-894
 // ---------------------------------------------------------------------------------------
 // pxyz(dp1, px, py) with dp1^2 = c.BB, src/tracking.jl:717-720
-TP_DEV double pxyz(const Cx& c, double px, double py) {
+TP_DEV double pxyz(const CxE& c, double px, double py) {
   double v = c.BB - px * px - py * py;
 #if TP_FAST
   // sqrt_nr is NaN for v <= 0 (and for NaN v), and fmax returns its other operand for a
@@ -309,7 +345,7 @@ TP_DEV double pxyz(const Cx& c, double px, double py) {
 
 
 // yrot, src/tracking.jl:727-745 (caller skips phi == 0)
-__device__ __noinline__ St yrot(St s, Cx c, double co, double si) {
+__device__ __noinline__ St yrot(St s, CxE c, double co, double si) {
   const double pz = pxyz(c, s.px, s.py);
   const double p = co * pz - si * s.px;
   const double px_new = si * pz + co * s.px;
@@ -322,7 +358,7 @@ __device__ __noinline__ St yrot(St s, Cx c, double co, double si) {
 }
 
 // bend_fringe (Forest 13.13), src/tracking.jl:752-815
-__device__ __noinline__ St bend_fringe(St s, Cx c, double irho, double gK) {
+__device__ __noinline__ St bend_fringe(St s, CxE c, double irho, double gK) {
   const double b0 = irho;
   const double dp1 = c.B;
   const double pz = pxyz(c, s.px, s.py);
@@ -375,7 +411,7 @@ __device__ __noinline__ St bend_fringe(St s, Cx c, double irho, double gK) {
 }
 
 // bend_edge (Forest 12.41), src/tracking.jl:822-851 (caller skips |rhoinv| < 1e-6).
-__device__ __noinline__ St bend_edge(St s, Cx c, double rhoinv, double co, double si, double s2t, double theta) {
+__device__ __noinline__ St bend_edge(St s, CxE c, double rhoinv, double co, double si, double s2t, double theta) {
   const double pz = pxyz(c, s.px, s.py);
   const double d2 = pxyz(c, 0.0, s.py);
   const double px_new = s.px * co + (pz - rhoinv * s.x) * si;

@@ -35,7 +35,7 @@ namespace TPK_BUILD {  // kernel names carry the build flavour (profiles tell th
 
 // Thick multipole: Yoshida-4 with strthinkick, src/tracking.jl:317-341 / :386-491.
 // MM > 0: general kick of order MM with its coefficients held in registers (kick_cf).
-template <bool EXH, int KK, int PPL, int MM = 0>
+template <bool EXH, int KK, int PPL, bool SER, int MM = 0>
 TP_DEV void mpole_steps(St* s, const Cx* c, const double* __restrict__ p, int nsteps, int m) {
   const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3], L1b = p[4], L2b = p[5];
   const double* poly = p + 6;
@@ -45,13 +45,13 @@ TP_DEV void mpole_steps(St* s, const Cx* c, const double* __restrict__ p, int ns
     const double Lsb = 2.0 * L1b + 2.0 * L2b;
 #define TP_YSTEP(k, KICK1, KICK2)                                 \
   {                                                               \
-    double S_ = drift_xy(s[k], c[k], L1);                         \
+    double S_ = drift_xy<SER>(s[k], c[k], L1);                  \
     KICK1;                                                        \
-    S_ = S_ + drift_xy(s[k], c[k], L2);                           \
+    S_ = S_ + drift_xy<SER>(s[k], c[k], L2);                    \
     KICK2;                                                        \
-    S_ = S_ + drift_xy(s[k], c[k], L2);                           \
+    S_ = S_ + drift_xy<SER>(s[k], c[k], L2);                    \
     KICK1;                                                        \
-    S_ = S_ + drift_xy(s[k], c[k], L1);                           \
+    S_ = S_ + drift_xy<SER>(s[k], c[k], L1);                    \
     s[k].z = s[k].z + __builtin_fma(S_, c[k].B, -Lsb);            \
   }
     if constexpr (MM > 0) {
@@ -85,13 +85,13 @@ TP_DEV void mpole_steps(St* s, const Cx* c, const double* __restrict__ p, int ns
     for (int st = 0; st < nsteps; ++st) {
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
-        drift<EXH>(s[k], c[k], L1, L1b);
+        drift<EXH, SER>(s[k], c[k], L1, L1b);
         kick_cf<MM>(s[k], cf, K1);
-        drift<EXH>(s[k], c[k], L2, L2b);
+        drift<EXH, SER>(s[k], c[k], L2, L2b);
         kick_cf<MM>(s[k], cf, K2);
-        drift<EXH>(s[k], c[k], L2, L2b);
+        drift<EXH, SER>(s[k], c[k], L2, L2b);
         kick_cf<MM>(s[k], cf, K1);
-        drift<EXH>(s[k], c[k], L1, L1b);
+        drift<EXH, SER>(s[k], c[k], L1, L1b);
       }
     }
     return;
@@ -99,19 +99,19 @@ TP_DEV void mpole_steps(St* s, const Cx* c, const double* __restrict__ p, int ns
   for (int st = 0; st < nsteps; ++st) {
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-      drift<EXH>(s[k], c[k], L1, L1b);
+      drift<EXH, SER>(s[k], c[k], L1, L1b);
       kick<KK>(s[k], poly, m, K1);
-      drift<EXH>(s[k], c[k], L2, L2b);
+      drift<EXH, SER>(s[k], c[k], L2, L2b);
       kick<KK>(s[k], poly, m, K2);
-      drift<EXH>(s[k], c[k], L2, L2b);
+      drift<EXH, SER>(s[k], c[k], L2, L2b);
       kick<KK>(s[k], poly, m, K1);
-      drift<EXH>(s[k], c[k], L1, L1b);
+      drift<EXH, SER>(s[k], c[k], L1, L1b);
     }
   }
 }
 
 // Expanded-Hamiltonian bend body: Yoshida-4 with bndthinkick, src/tracking.jl:350-375
-template <bool EXH, int KK, int PPL>
+template <bool EXH, int KK, int PPL, bool SER>
 TP_DEV void sbend_steps(St* s, const Cx* c, const double* __restrict__ p, int nsteps, int m, double beti) {
   const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3], L1b = p[4], L2b = p[5];
   const double irho = p[6], K1i = p[7], K2i = p[8];
@@ -119,13 +119,13 @@ TP_DEV void sbend_steps(St* s, const Cx* c, const double* __restrict__ p, int ns
   for (int st = 0; st < nsteps; ++st) {
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
-      drift<EXH>(s[k], c[k], L1, L1b);
+      drift<EXH, SER>(s[k], c[k], L1, L1b);
       bkick<KK>(s[k], poly, m, K1, K1i, irho, beti);
-      drift<EXH>(s[k], c[k], L2, L2b);
+      drift<EXH, SER>(s[k], c[k], L2, L2b);
       bkick<KK>(s[k], poly, m, K2, K2i, irho, beti);
-      drift<EXH>(s[k], c[k], L2, L2b);
+      drift<EXH, SER>(s[k], c[k], L2, L2b);
       bkick<KK>(s[k], poly, m, K1, K1i, irho, beti);
-      drift<EXH>(s[k], c[k], L1, L1b);
+      drift<EXH, SER>(s[k], c[k], L1, L1b);
     }
   }
 }
@@ -301,14 +301,14 @@ TP_DEV void mat6(St& s, const double* __restrict__ M) {
     default: break;                                                                      \
   }
 
-template <bool EXH, int KK, int PPL>
+template <bool EXH, int KK, int PPL, bool SER>
 TP_DEV void mpole_dispatch(St* s, const Cx* c, const double* __restrict__ p, int nsteps, int m) {
   if constexpr (KK == TP_KICK_GEN) {
-#define TP_CALLM(MM) mpole_steps<EXH, KK, PPL, MM>(s, c, p, nsteps, m)
+#define TP_CALLM(MM) mpole_steps<EXH, KK, PPL, SER, MM>(s, c, p, nsteps, m)
     TP_GEN_ORDERS(TP_CALLM)
 #undef TP_CALLM
   }
-  mpole_steps<EXH, KK, PPL>(s, c, p, nsteps, m);
+  mpole_steps<EXH, KK, PPL, SER>(s, c, p, nsteps, m);
 }
 
 template <int KK, int PPL>
@@ -335,6 +335,10 @@ TP_DEV void esbend_dispatch(St* s, const Cx* c, const double* __restrict__ p, in
 
 template <bool EXH, unsigned FEAT, int PPL>
 TP_DEV void apply_elem(St* s, Cx* c, const tp_rec& rec, const double* __restrict__ pool, double beti) {
+  // The series drift (inv_pz) holds two more values per particle and two per element; the
+  // variants with general kicks keep up to 22 coefficients in registers and use the
+  // reciprocal square root instead.
+  constexpr bool SER = (FEAT & FT_GENKICK) == 0;
   const double* __restrict__ p = pool + rec.off;
   const uint32_t fl = rec.flags;
   if constexpr ((FEAT & FT_MISALIGN) != 0) {
@@ -353,10 +357,10 @@ TP_DEV void apply_elem(St* s, Cx* c, const tp_rec& rec, const double* __restrict
   switch (rec.tag) {
     case TP_DRIFT:
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) drift<EXH>(s[k], c[k], p[0], p[1]);
+      for (int k = 0; k < PPL; ++k) drift<EXH, SER>(s[k], c[k], p[0], p[1]);
       break;
     case TP_MPOLE: {
-#define TP_CALL(KK) mpole_dispatch<EXH, KK, PPL>(s, c, p, ns, m)
+#define TP_CALL(KK) mpole_dispatch<EXH, KK, PPL, SER>(s, c, p, ns, m)
       TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
       break;
@@ -376,7 +380,7 @@ TP_DEV void apply_elem(St* s, Cx* c, const tp_rec& rec, const double* __restrict
 #pragma unroll
           for (int k = 0; k < PPL; ++k) edge_lin(s[k], fin, p + 9, irho, false);
         }
-#define TP_CALL(KK) sbend_steps<EXH, KK, PPL>(s, c, p, ns, m, beti)
+#define TP_CALL(KK) sbend_steps<EXH, KK, PPL, SER>(s, c, p, ns, m, beti)
         TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
         if (fout != 0) {
@@ -442,13 +446,13 @@ TP_DEV void apply_elem(St* s, Cx* c, const tp_rec& rec, const double* __restrict
         const bool dr = (fl & TP_F_RF_DRIFT) != 0;
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
-          if (dr) drift<EXH>(s[k], c[k], p[0], p[1]);
+          if (dr) drift<EXH, SER>(s[k], c[k], p[0], p[1]);
           if (fl & TP_F_RF_KICK) {
             const double phase = p[3] * ((s[k].z - p[4]) / 2.99792458e8) - p[5];
             s[k].d = s[k].d - p[2] * sin_lib(phase) / p[6];
             mkctx(s[k], c[k], beti);
           }
-          if (dr) drift<EXH>(s[k], c[k], p[0], p[1]);
+          if (dr) drift<EXH, SER>(s[k], c[k], p[0], p[1]);
         }
       }
       break;
