@@ -88,12 +88,12 @@ TP_DEV double rsqrt_nr(double v) {
 // lane by lane, so each lane's result depends only on its own coordinates.
 TP_DEV double inv_pz(const St& s, const Cx& c, double L) {
   const double v = (s.px * s.px + s.py * s.py) * c.IA;
-  if (__builtin_expect(v < 0.8660254037844386 * 0x1p-14, 1)) {
-    const double Lr = L * c.R;
-    const double t = __builtin_fma(v, 0.4811252243246881, 0.5);                // Q3, 1/2
-    return __builtin_fma(v, __builtin_fma(Lr * v, t, Lr * 0.5773502691896257), Lr);  // Q1
-  }
-  return L * rsqrt_nr(c.A - s.px * s.px - s.py * s.py);
+  const double Lr = L * c.R;
+  const double t = __builtin_fma(v, 0.4811252243246881, 0.5);                       // Q3, 1/2
+  double NormL = __builtin_fma(v, __builtin_fma(Lr * v, t, Lr * 0.5773502691896257), Lr);  // Q1
+  // bound k 2^-14 rounded down to 20 fraction bits (a 32-bit literal of the compare)
+  if (__builtin_expect(!(v < 0x1.bb67ap-15), 0)) NormL = L * rsqrt_nr(c.A - s.px * s.px - s.py * s.py);
+  return NormL;
 }
 // sqrt(v) = v / sqrt(v) = t (1 - e)^(-1/2) with t = v y, from the same estimate
 TP_DEV double sqrt_nr(double v) {
