@@ -62,11 +62,16 @@ FP64_PEAK = 78.6e12
 HBM_PEAK = 8.0e12
 
 # HBM bytes per particle of one tracking-kernel launch, measured with rocprofv3 --pmc on the
-# 1e6-particle DBA launch: FETCH_SIZE 25,562.4 KiB x 1.998 (the counter reads 23,457.1 KiB for a
-# plain 48 MB device copy) + WRITE_SIZE 47,491.3 KiB x 1.00 (exact on the same copy; 616 KiB
-# above the 46,875 KiB of coordinates are the tile queue's atomics), per 1e6 particles.
-# Independent of the turns per launch (the state stays in registers in between).
-MEASURED_BYTES_PER_PARTICLE_LAUNCH = {"dba": (25562.4 * 1.998 + 47491.3) * 1024 / 1e6}
+# 1e6-particle DBA launch of 100 turns, per register budget of the kernel (waves/SIMD; 0 = the
+# compiler's, 4 waves): FETCH_SIZE x 1.998 (the counter reads 23,457 KiB for a plain 48 MB
+# device copy) + WRITE_SIZE x 1.00 (exact on the same copy), per 1e6 particles.  Budget 0:
+# 25,578.8 + 47,491.3 KiB (616 KiB above the 46,875 KiB of coordinates are the tile queue's
+# atomics), independent of the turns per launch since the state stays in registers in
+# between; budgets 5 and 6 add the part of their register spills (32 and 96 B per lane) that
+# leaves the L2 over those 100 turns.
+MEASURED_BYTES_PER_PARTICLE_LAUNCH = {("dba", 0): (25578.8 * 1.998 + 47491.3) * 1024 / 1e6,
+                                      ("dba", 5): (27054.6 * 1.998 + 57852.3) * 1024 / 1e6,
+                                      ("dba", 6): (33381.9 * 1.998 + 104208.7) * 1024 / 1e6}
 
 
 def host_cores() -> int:
@@ -397,7 +402,7 @@ def streamed(ctx, tab, init, work, args, fast, fpt, n, n_seq):
     avg_s = tm["kernel_ms"] / 1e3 / launches
     alive_per_launch = live / launches
     algo = 104.0 * alive_per_launch
-    bpp = MEASURED_BYTES_PER_PARTICLE_LAUNCH.get(args.workload)
+    bpp = MEASURED_BYTES_PER_PARTICLE_LAUNCH.get((args.workload, args.fwd_waves))
     flops = fpt * alive_per_launch / avg_s
     return {"turns_per_launch": 1, "turns": T, "launches": tm["launches"], "avg_launch_ms": avg_s * 1e3,
             "bytes_per_launch": algo, "achieved_GBps": algo / avg_s / 1e9, "peak_GBps": HBM_PEAK / 1e9,
@@ -513,7 +518,7 @@ def main():
     avg_launch_s = kernel_ms / 1e3 / max(launches, 1)
     live_per_launch = float(live_turns) * args.steps / max(launches, 1)
     achieved = fpt * live_per_launch / avg_launch_s
-    bpp = MEASURED_BYTES_PER_PARTICLE_LAUNCH.get(args.workload)
+    bpp = MEASURED_BYTES_PER_PARTICLE_LAUNCH.get((args.workload, args.fwd_waves))
     traffic = None if bpp is None else bpp * n  # HBM bytes per launch (rocprof PMC, calibrated)
 
     out = {
