@@ -34,22 +34,28 @@ def _coords(n, seed, nlost=0):
 
 
 def _same(a, b, fast, what):
-    """Parity build: bit for bit.  Fast build: the same source, but FMA contraction is the
-    compiler's choice per kernel, so within 1e-11 of each plane's largest entry."""
+    """Parity build: bit for bit.  Fast build: FMA contraction is the compiler's choice per
+    kernel, and the drifts round differently in the batch kernel (series form of 1/pz) and in
+    a single-beam launch that leaves its waves alone on their SIMDs (refined rsqrt), so
+    within 1e-9 of each plane's largest entry (the differences grow through the turns), plus
+    1e-12 absolute: z adds NormL (beti + delta) - L beti at every drift, a difference of
+    terms of the drift's length, so its rounding is absolute (~1e-16 m per drift) however
+    small z itself is."""
     if not fast or a.size == 0:
         assert np.array_equal(a, b), (what, np.max(np.abs(a - b)) if a.size else 0)
     else:
         scale = np.max(np.abs(b), axis=0, keepdims=True)
-        assert np.all(np.abs(a - b) <= 1e-11 * np.where(scale > 0, scale, 1.0)), what
+        assert np.all(np.abs(a - b) <= 1e-9 * np.where(scale > 0, scale, 1.0) + 1e-12), what
 
 
 def _cross_variant_close(g, r, v, fast):
-    """Variant v of a batched adjoint against its own tp_adjoint call, when the batch runs a
-    kernel variant with more features than the call does.  Parity build: loss within 1e-13,
-    dx0 bit for bit, dk within 1e-12 of the largest.  Fast build: the kernel variants round
-    differently (FMA contraction is the compiler's choice per kernel, and the drifts of the
-    variants without general kicks use the series form of 1/pz), and the differences grow
-    through the turns, so loss, dx0 and dk agree within the adjoint's own accuracy, 1e-9."""
+    """Variant v of a batched adjoint against its own tp_adjoint call.  Parity build: loss
+    within 1e-13, dx0 bit for bit, dk within 1e-12 of the largest.  Fast build: the kernels
+    round differently (FMA contraction is the compiler's choice per kernel; the batch kernel's
+    drifts take the series form of 1/pz, a small single-beam launch the refined rsqrt, and a
+    batch holding a variant with more features runs that variant's kernel for all), and the
+    differences grow through the turns, so loss, dx0 and dk agree within the adjoint's own
+    accuracy, 1e-9."""
     tol = 1e-9 if fast else 1e-13
     assert abs(g.loss[v] - r.loss) <= tol * abs(r.loss), (v, abs(g.loss[v] - r.loss) / abs(r.loss))
     if fast:
@@ -189,13 +195,10 @@ def test_batch_adjoint_matches_per_variant(ctx, fast):
     for v, (lat, c, t) in enumerate(zip(lats, coords, ts)):
         r = ringpass_grad(c, lat, beam, int(t), ckpt_stride=2, fast=fast, ctx=ctx)
         assert g.n_alive[v] == r.n_alive
-        assert abs(g.loss[v] - r.loss) <= 1e-13 * abs(r.loss), v
         fin = parts[v].download()
         assert np.array_equal(fin[1], r.lost_flags)
         _same(np.ascontiguousarray(fin[0]), r.final, fast, v)
-        _same(g.dx0[v], r.dx0, fast, v)
-        scale = np.max(np.abs(r.dk))
-        assert np.all(np.abs(g.dk[v] - r.dk) <= 1e-12 * scale), (v, np.max(np.abs(g.dk[v] - r.dk)) / scale)
+        _cross_variant_close(g, r, v, fast)
         assert np.all(g.dk[v][r.dk == 0.0] == 0.0)
     assert g.n_alive[0] == 98
 

@@ -71,6 +71,7 @@ struct TpKArgs {
   // Persistent launch: waves take tiles of 64 x PPL list entries from this queue counter
   // (cleared before the launch) until it runs dry; null = one tile per wave of the grid.
   int32_t* counter;
+  int32_t ser;  // set by the launcher: series drift (the launch fills the SIMDs)
 };
 
 // One lattice variant of a batch: where its element table sits in the concatenated
