@@ -41,17 +41,20 @@ TP_DEV void drift_lin_back(St& s, Ad& a, double L) {
   a.d = a.d + ad;
 }
 
+// SER: 1/pz by the series form (inv_pz) -- the batched reverse kernel, whose launches fill
+// the SIMDs; otherwise the refined rsqrt.
+template <bool SER = false>
 TP_DEV void drift_back(St& s, Ad& a, const Cx& c, double L, double Lb, bool lin = false) {
   if (lin) {
     drift_lin_back(s, a, L);
     return;
   }
-  const double pz2 = c.A - s.px * s.px - s.py * s.py;
 #if TP_FAST
-  const double rz = rsqrt_nr(pz2);
+  const double rz = SER ? inv_pz(s, c, 1.0) : rsqrt_nr(c.A - s.px * s.px - s.py * s.py);
   const double NormL = L * rz;
   const double ipz2 = rz * rz;
 #else
+  const double pz2 = c.A - s.px * s.px - s.py * s.py;
   const double NormL = L / sqrt(pz2);
   const double ipz2 = 1.0 / pz2;
 #endif

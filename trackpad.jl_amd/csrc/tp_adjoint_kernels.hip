@@ -38,36 +38,36 @@ namespace TPK_BUILD {
       break;                                                               \
   }
 
-template <int KK>
+template <int KK, bool SER>
 TP_DEV void mpole_back(St& s, Ad& a, const Cx& c, const double* __restrict__ p, int ns, int m, int g, double& gacc,
                        bool lin) {
   const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3], L1b = p[4], L2b = p[5];
   const double* poly = p + 6;
   // the Yoshida step D1 K1 D2 K2 D2 K1 D1 is a palindrome: undo it in the same order
   for (int st = 0; st < ns; ++st) {
-    drift_back(s, a, c, L1, L1b, lin);
+    drift_back<SER>(s, a, c, L1, L1b, lin);
     kick_back<KK>(s, a, poly, m, K1, g, gacc);
-    drift_back(s, a, c, L2, L2b, lin);
+    drift_back<SER>(s, a, c, L2, L2b, lin);
     kick_back<KK>(s, a, poly, m, K2, g, gacc);
-    drift_back(s, a, c, L2, L2b, lin);
+    drift_back<SER>(s, a, c, L2, L2b, lin);
     kick_back<KK>(s, a, poly, m, K1, g, gacc);
-    drift_back(s, a, c, L1, L1b, lin);
+    drift_back<SER>(s, a, c, L1, L1b, lin);
   }
 }
 
-template <int KK>
+template <int KK, bool SER>
 TP_DEV void sbend_back(St& s, Ad& a, const Cx& c, const double* __restrict__ p, int ns, int m, double beti, bool lin) {
   const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3], L1b = p[4], L2b = p[5];
   const double irho = p[6], K1i = p[7], K2i = p[8];
   const double* poly = p + 17;
   for (int st = 0; st < ns; ++st) {
-    drift_back(s, a, c, L1, L1b, lin);
+    drift_back<SER>(s, a, c, L1, L1b, lin);
     bkick_back<KK>(s, a, poly, m, K1, K1i, irho, beti);
-    drift_back(s, a, c, L2, L2b, lin);
+    drift_back<SER>(s, a, c, L2, L2b, lin);
     bkick_back<KK>(s, a, poly, m, K2, K2i, irho, beti);
-    drift_back(s, a, c, L2, L2b, lin);
+    drift_back<SER>(s, a, c, L2, L2b, lin);
     bkick_back<KK>(s, a, poly, m, K1, K1i, irho, beti);
-    drift_back(s, a, c, L1, L1b, lin);
+    drift_back<SER>(s, a, c, L1, L1b, lin);
   }
 }
 
@@ -125,9 +125,11 @@ TP_DEV void sub6(St& s, const double* __restrict__ t) {
 // One element backwards: s is the element's output on entry, its input on return.  minv
 // holds the inverses of the r1 / r2 matrices at the pool offsets of the matrices (variant
 // with FT_MISALIGN); lin selects the linearised drift (variant with FT_LINH).
-template <unsigned FEAT>
+template <unsigned FEAT, bool SER_OK>
 TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double* __restrict__ pool,
                             const double* __restrict__ minv, double beti, bool lin, double& gacc) {
+  // the series drift needs two more hoisted values; kernels with general kicks never take it
+  constexpr bool SER = SER_OK && (FEAT & FT_GENKICK) == 0;
   const double* __restrict__ p = pool + rec.off;
   const uint32_t fl = rec.flags;
   const int m = rec.max_order, ns = rec.nsteps, g = rec.gorder;
@@ -140,10 +142,10 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
   }
   switch (rec.tag) {
     case TP_DRIFT:
-      drift_back(s, a, c, p[0], p[1], lin);
+      drift_back<SER>(s, a, c, p[0], p[1], lin);
       break;
     case TP_MPOLE: {
-#define TP_CALL(KK) mpole_back<KK>(s, a, c, p, ns, m, g, gacc, lin)
+#define TP_CALL(KK) mpole_back<KK, SER>(s, a, c, p, ns, m, g, gacc, lin)
       TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
       break;
@@ -159,7 +161,7 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
         const int fin = (fl >> TP_F_FIN_SHIFT) & 0xF, fout = (fl >> TP_F_FOUT_SHIFT) & 0xF;
         const double irho = p[6];
         if (fout != 0) edge_back(s, a, fout, p + 13, irho, true);
-#define TP_CALL(KK) sbend_back<KK>(s, a, c, p, ns, m, beti, lin)
+#define TP_CALL(KK) sbend_back<KK, SER>(s, a, c, p, ns, m, beti, lin)
         TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
         if (fin != 0) edge_back(s, a, fin, p + 9, irho, false);
@@ -201,12 +203,12 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
     case TP_RFCAV:
       if constexpr ((FEAT & FT_RF) != 0) {
         const bool dr = (fl & TP_F_RF_DRIFT) != 0;
-        if (dr) drift_back(s, a, c, p[0], p[1], lin);
+        if (dr) drift_back<SER>(s, a, c, p[0], p[1], lin);
         if (fl & TP_F_RF_KICK) {
           rf_kick_back(s, a, p);
           mkctx(s, c, beti);
         }
-        if (dr) drift_back(s, a, c, p[0], p[1], lin);
+        if (dr) drift_back<SER>(s, a, c, p[0], p[1], lin);
       }
       break;
     case TP_MARKER:
@@ -235,7 +237,7 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
 // gradients when slot_of_pos is null).  A time-varying table (n_rows > 1) holds the
 // lattice of turn r + row_turn0 in sequence row r; the turns undone here are turn_last,
 // turn_last - 1, ..., each with its own row.
-template <unsigned FEAT>
+template <unsigned FEAT, bool SER_OK>
 TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq_rows, const tp_rec* __restrict__ recs,
                        const double* __restrict__ pool, const double* __restrict__ minv,
                        const int32_t* __restrict__ slot_of_pos, int32_t n_seq, int32_t nturns, int32_t turn_last,
@@ -257,7 +259,7 @@ TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq_rows
         if (e >= 2) ri_next = seq[e - 2];
       }
       double gacc = 0.0;
-      apply_elem_back<FEAT>(s, l, c, rec, pool, minv, beti, lin, gacc);
+      apply_elem_back<FEAT, SER_OK>(s, l, c, rec, pool, minv, beti, lin, gacc);
       if (slot >= 0) {
         const double w = wave_sum(gacc);
         if (lane == 0) row[slot] += w;
@@ -308,7 +310,7 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void adjoint_kernel(const int32_
   Cx c;
   mkctx(s, c, a.beti);
   const bool lin = (FEAT & FT_LINH) != 0 && a.exact_h == 0;
-  back_turns<FEAT>(s, l, c, seq, recs, pool, a.minv, slot_of_pos, a.n_seq, a.nturns, a.turn_last, a.n_rows,
+  back_turns<FEAT, false>(s, l, c, seq, recs, pool, a.minv, slot_of_pos, a.n_seq, a.nturns, a.turn_last, a.n_rows,
                    a.row_turn0, a.beti, lin, lds + wave * a.n_slots, lane);
   if (i < a.n) {
     double* r = lam + i;
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_back_kernel(const TpV
       }
       mkctx(s, c, d.beti);
       const int32_t nt = (j == d.nseg - 1) ? d.nturns - j * d.ckpt_stride : d.ckpt_stride;
-      back_turns<FEAT>(s, l, c, seq + d.seq_off, recs + d.rec_off, pool + d.pool_off,
+      back_turns<FEAT, true>(s, l, c, seq + d.seq_off, recs + d.rec_off, pool + d.pool_off,
                        minv ? minv + d.pool_off : nullptr, slots + d.pos_off, d.n_seq, nt,
                        d.turn0 + j * d.ckpt_stride + nt - 1, d.n_rows, d.row_turn0, d.beti, lin, row, lane);
     }
