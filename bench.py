@@ -18,13 +18,21 @@ The JSON line carries:
   streamed       after the timed region: the streamed-state variant (one turn per launch,
                  the beam read from and written to HBM every turn) -- its HBM GB/s against
                  the 8 TB/s peak and its FP64 rate (--streamed-turns, 0 skips it)
+  parity_kernel  after the timed region: the parity build (no FMA contraction, IEEE
+                 division and square root, the reference's operation order) on the same
+                 beam, autotuned on its own, one launch of --parity-turns turns
+  batched, batched_adjoint, vs_per_variant_launches
+                 the metric's second half, measured after the DBA legs: the batched
+                 workload below, one forward graph launch and one adjoint graph launch
 
 Workload ``batched``: 256 variants of the DBA ring (strengths x (1 + N(0, 1%)), seeded),
 N_v log-uniform in [1k, 64k], T_v uniform in [100, 2000]; one step is one graph launch of
 the persistent batched kernel over all variants.  The line adds the batched adjoint (one
 graph launch: checkpointed forward, reverse pass, per-variant reduction) and its cost
-relative to the forward, and both against 256 plain per-variant launches on the same GPU
-(measured on every variant with the turn counts scaled by --compare-turn-scale).
+relative to the forward, and both against plain per-variant launches on the same GPU at
+the variants' full turn counts: --compare-variants variants spread over the beam sizes,
+timed both as one batch and as plain launches, and the 256-variant per-variant time
+extrapolated from them (time per turn interpolated in log N).
 
 Workload ``da``: the dynamic-aperture scan -- the 2,000-element ring (80 DBA cells, every
 magnet with order-10 normal and skew errors) on the 1024 x 1024 x 3 grid (3,145,728
@@ -74,12 +82,35 @@ MEASURED_BYTES_PER_PARTICLE_LAUNCH = {("dba", 0): (25578.8 * 1.998 + 47491.3) * 
                                       ("dba", 6): (33381.9 * 1.998 + 104208.7) * 1024 / 1e6}
 
 
-def host_cores() -> int:
+def visible_cores() -> int:
     try:
-        n = len(os.sched_getaffinity(0))
+        return len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        return os.cpu_count() or 1
+
+
+def host_cores() -> int:
+    """Threads of the CPU baseline: every core this process may use, or the share that
+    OMP_NUM_THREADS grants it (set on shared hosts, where the visible cores belong to
+    other jobs too)."""
+    n = visible_cores()
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cores_note(cores: int) -> str:
+    vis = visible_cores()
+    if vis == cores:
+        return f"{cores} threads = all visible host cores"
+    return f"{cores} threads = this process's CPU share (OMP_NUM_THREADS); the host shows {vis} cores"
+
+
+# The FP64 work of every kernel here runs on the vector ALUs (VALU): the maps are chains of
+# small nonlinear updates per particle, not contractions, so there is no MFMA work.
+BOUND = "vector_fp64"
+PIPE = "vector FP64 VALU (no MFMA: not a contraction); FP64 peak 78.6 TFLOP/s"
 
 
 def workload(name, n_override, t_override):
@@ -199,7 +230,7 @@ def bench_diff(args, fast):
         "adjoint": {"forward_ms": fwd / K, "reverse_ms": bwd / K, "plain_tracking_ms": plain / K,
                     "gradient_over_forward": (fwd + bwd) / plain, "wall_ms_per_step": wall * 1e3 / K,
                     "loss": g.loss, "survivors": g.n_alive},
-        "roofline": {"bound": "mfma", "pipe": "vector FP64 VALU; 4,096 particles are 64 waves on 1,024 SIMDs: "
+        "roofline": {"bound": BOUND, "pipe": "vector FP64 VALU; 4,096 particles are 64 waves on 1,024 SIMDs: "
                      "occupancy-bound, not at the roofline",
                      "achieved": fpt * n * T / (plain / K / 1e3) / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": fpt * n * T / (plain / K / 1e3) / FP64_PEAK, "traffic": None,
@@ -208,147 +239,195 @@ def bench_diff(args, fast):
     print(json.dumps(out), flush=True)
 
 
-def bench_batched(args, fast):
-    from trackpad_hip import Beam, HIPBatch, HIPContext, HIPParticles, compile_table
-    from trackpad_hip import workloads as W
-    from trackpad_hip.flops import flops_per_turn
-    import trackpad_hip._lib as L
+class BatchedSweep:
+    """The batched workload: 256 DBA variants (strengths x (1 + N(0, 1 %)), seeded), N_v
+    log-uniform in [1k, 64k], T_v uniform in [100, 2000], bound to one tp_batch."""
 
-    beam = Beam(W.ENERGY)
-    nvar = args.variants
-    rings, ns, ts = W.variant_rings(nvar=nvar, seed=11)
-    if args.turn_scale != 1.0:
-        ts = np.maximum(1, np.round(ts * args.turn_scale)).astype(np.int64)
-    ctx = HIPContext(0)
-    ctx.set_waves(args.fwd_waves, args.back_waves)
-    tabs = [ctx.table(compile_table(r, beam)) for r in rings]
-    c0 = [W.dba_particles(int(n), seed=1000 + v) for v, n in enumerate(ns)]
-    parts = [HIPParticles.from_coords(c, ctx=ctx) for c in c0]
-    init = [HIPParticles.from_coords(c, ctx=ctx) for c in c0]
-    b = HIPBatch(tabs, parts, ctx=ctx)
-    b.set_launch(args.block, args.fwd_waves, args.back_waves)
-    K = args.ckpt_stride
-    tune = None
-    if not args.no_autotune:
-        probe = np.maximum(1, ts // 100).astype(np.int32)
-        tune = b.autotune(probe, ckpt_stride=K, adjoint=True, fast=fast, cache=args.tune_cache)
-        args.block, args.fwd_waves, args.back_waves = tune["block"], tune["fwd_waves"], tune["back_waves"]
-        print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
-    n_seq = len(rings[0])
-    fpt = flops_per_turn(rings[0])
-    sched = float(np.sum(ns * ts))
-    print(f"[bench] batched: {nvar} variants, sum N = {int(ns.sum())}, T in [{ts.min()}, {ts.max()}], "
-          f"sum N*T = {sched:.3e}, kernel={args.kernel}", file=sys.stderr, flush=True)
+    def __init__(self, args, fast):
+        from trackpad_hip import Beam, HIPBatch, HIPContext, HIPParticles, compile_table
+        from trackpad_hip import workloads as W
+        from trackpad_hip.flops import flops_per_turn
 
-    def reset():
-        for p, q in zip(parts, init):
+        self.args, self.fast = args, fast
+        self.beam = Beam(W.ENERGY)
+        self.nvar = args.variants
+        self.rings, self.ns, ts = W.variant_rings(nvar=self.nvar, seed=11)
+        if args.turn_scale != 1.0:
+            ts = np.maximum(1, np.round(ts * args.turn_scale)).astype(np.int64)
+        self.ts = ts
+        self.ctx = ctx = HIPContext(0)
+        ctx.set_waves(args.fwd_waves, args.back_waves)
+        self.tabs = [ctx.table(compile_table(r, self.beam)) for r in self.rings]
+        c0 = [W.dba_particles(int(n), seed=1000 + v) for v, n in enumerate(self.ns)]
+        self.parts = [HIPParticles.from_coords(c, ctx=ctx) for c in c0]
+        self.init = [HIPParticles.from_coords(c, ctx=ctx) for c in c0]
+        self.batch = HIPBatch(self.tabs, self.parts, ctx=ctx)
+        self.block, self.fwd_waves, self.back_waves = args.block, args.fwd_waves, args.back_waves
+        self.batch.set_launch(self.block, self.fwd_waves, self.back_waves)
+        self.K = args.batched_ckpt_stride
+        self.tune = None
+        if not args.no_autotune:
+            probe = np.maximum(1, self.ts // 100).astype(np.int32)
+            self.tune = self.batch.autotune(probe, ckpt_stride=self.K, adjoint=True, fast=fast, cache=args.tune_cache)
+            self.block, self.fwd_waves, self.back_waves = (self.tune["block"], self.tune["fwd_waves"],
+                                                           self.tune["back_waves"])
+            print(f"[bench] batched autotune: {self.tune}", file=sys.stderr, flush=True)
+        self.n_seq = len(self.rings[0])
+        self.fpt = flops_per_turn(self.rings[0])
+        self.sched = float(np.sum(self.ns * self.ts))
+        print(f"[bench] batched: {self.nvar} variants, sum N = {int(self.ns.sum())}, T in [{self.ts.min()}, "
+              f"{self.ts.max()}], sum N*T = {self.sched:.3e}, kernel={args.kernel}", file=sys.stderr, flush=True)
+        # load the kernels (a 1-turn pass of every variant), then restore the beams
+        self.batch.track(np.ones(self.nvar, np.int32), fast=fast)
+        self.reset()
+
+    def reset(self):
+        for p, q in zip(self.parts, self.init):
             p.copy_from(q)
-        ctx.sync()
+        self.ctx.sync()
 
-    # preparation: load the kernels (a 1-turn pass of every variant), then restore the beams
-    b.track(np.ones(nvar, np.int32), fast=fast)
-    reset()
+    def forward(self, steps, warmup):
+        """Timed graph launches of the batched forward; returns (wall s, kernel ms) summed
+        over the timed steps."""
+        b, ts = self.batch, self.ts.astype(np.int32)
+        for i in range(warmup):
+            self.reset()
+            b.track(ts, fast=self.fast)
+            print(f"[bench] batched warmup step {i + 1}/{warmup} done", file=sys.stderr, flush=True)
+        wall = kernel_ms = 0.0
+        for i in range(steps):
+            self.reset()
+            t0 = time.perf_counter()
+            b.track(ts, fast=self.fast, timed=True)
+            wall += time.perf_counter() - t0
+            kernel_ms += self.ctx.timing()["kernel_ms"]
+            print(f"[bench] batched forward step {i + 1}/{steps}: {wall * 1e3 / (i + 1):.0f} ms", file=sys.stderr,
+                  flush=True)
+        live = 0.0
+        for p, t in zip(self.parts, self.ts):
+            _, _, lt, le = p.download()
+            live += live_turns_(lt, le, self.n_seq, int(t))
+        self.live = live
+        return wall, kernel_ms
 
-    def fwd_step():
-        b.track(ts.astype(np.int32), fast=fast, timed=True)
-        return ctx.timing()["kernel_ms"]
-
-    for i in range(args.warmup):
-        fwd_step()
-        reset()
-        print(f"[bench] warmup step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-    kernel_ms = 0.0
-    wall = 0.0
-    for i in range(args.steps):
-        reset()
+    def adjoint(self):
+        """One graph launch of the batched adjoint; returns (gradients, graph ms, wall s)."""
+        self.reset()
         t0 = time.perf_counter()
-        kernel_ms += fwd_step()
-        wall += time.perf_counter() - t0
-        print(f"[bench] batched forward step {i + 1}/{args.steps}: {wall * 1e3 / (i + 1):.0f} ms", file=sys.stderr,
-              flush=True)
-    info_f = b.info()
-    live = 0.0
-    for p, t in zip(parts, ts):
-        _, _, lt, le = p.download()
-        live += live_turns_(lt, le, n_seq, int(t))
-    avg_s = kernel_ms / 1e3 / args.steps
-    achieved = fpt * live / avg_s
+        g = self.batch.adjoint(self.ts.astype(np.int32), ckpt_stride=self.K, fast=self.fast, timed=True)
+        wall = time.perf_counter() - t0
+        ms = self.ctx.timing()["kernel_ms"]
+        self.info_a = self.batch.info()
+        print(f"[bench] batched adjoint: {ms:.0f} ms (graph), {wall * 1e3:.0f} ms wall, checkpoints "
+              f"{self.info_a['ckpt_bytes'] / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+        return g, ms, wall
 
-    # batched adjoint, one graph launch
-    reset()
-    t0 = time.perf_counter()
-    g = b.adjoint(ts.astype(np.int32), ckpt_stride=K, fast=fast, timed=True)
-    adj_wall = time.perf_counter() - t0
-    adj_ms = ctx.timing()["kernel_ms"]
-    info_a = b.info()
-    print(f"[bench] batched adjoint: {adj_ms:.0f} ms (graph), {adj_wall * 1e3:.0f} ms wall, "
-          f"checkpoints {info_a['ckpt_bytes'] / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+    def compare(self, nsub, fwd_graph_ms, adj_graph_ms):
+        """Plain per-variant launches (tp_track / tp_adjoint one variant after the other on the
+        library's stream) against the batch, at the variants' own turn counts, on nsub variants
+        spread evenly over the sorted beam sizes: the same nsub variants as one batch and as
+        nsub plain launches, both measured; and the 256-variant per-variant time extrapolated
+        from the subset's time per turn as a function of N (interpolated in log N), against the
+        measured 256-variant graphs."""
+        import ctypes as C
+        import trackpad_hip._lib as L
+        from trackpad_hip import HIPBatch
 
-    if args.compare_turn_scale <= 0:
-        bf = pf = ba = pa = float("nan")
-    else:
-        bf, pf, ba, pa = compare_per_variant(args, fast, b, ctx, parts, tabs, ts, ns, n_seq, reset, K, L)
-    out = batched_line(args, nvar, ns, ts, sched, wall, n_seq, info_f, fpt, live, avg_s, achieved, adj_ms,
-                       adj_wall, K, info_a, g, bf, pf, ba, pa)
-    if not args.no_cpu_baseline:
-        ns_, ts_ = (int(v) for v in args.cpu_sample.lower().split("x"))
-        cores = host_cores()
-        mk = lambda n_: W.dba_particles(n_, seed=1000)
-        v_all, dt_all = cpu_baseline(rings[0], beam, mk, None, ns_, ts_, cores)
-        out["cpu_baseline"] = {"value": v_all, "unit": "particle-turns/s", "cores": cores, "kind": "port",
-                               "sample": f"variant 0, {ns_} particles x {ts_} turns ({dt_all:.1f} s), oracle C++ "
-                                         "restatement, OpenMP over particles"}
-    print(json.dumps(out), flush=True)
+        ctx, fast, K = self.ctx, self.fast, self.K
+        order = np.argsort(self.ns, kind="stable")
+        sub = order[np.unique(np.round(np.linspace(0, self.nvar - 1, min(nsub, self.nvar))).astype(np.int64))]
+        sb = HIPBatch([self.tabs[v] for v in sub], [self.parts[v] for v in sub], ctx=ctx)
+        sb.set_launch(self.block, self.fwd_waves, self.back_waves)
+        tsub = self.ts[sub].astype(np.int32)
+        flags = L.TP_TRACK_FAST if fast else 0
+        self.reset()
+        sb.track(tsub, fast=fast)  # plan + capture outside the timed region
+        self.reset()
+        sb.track(tsub, fast=fast, timed=True)
+        bf = ctx.timing()["kernel_ms"]
+        self.reset()
+        per_f = []
+        for v in sub:
+            t0 = time.perf_counter()
+            L.check(ctx._lib.tp_track(ctx.handle, self.tabs[v].handle, self.parts[v].handle, 0, int(self.ts[v]), flags))
+            per_f.append((time.perf_counter() - t0) * 1e3)
+        print(f"[bench] compare forward ({len(sub)} variants, full turns): batched {bf:.0f} ms, per-variant "
+              f"{sum(per_f):.0f} ms", file=sys.stderr, flush=True)
+        self.reset()
+        sb.adjoint(tsub, ckpt_stride=K, fast=fast)  # plan + capture
+        self.reset()
+        sb.adjoint(tsub, ckpt_stride=K, fast=fast, timed=True)
+        ba = ctx.timing()["kernel_ms"]
+        self.reset()
+        loss, na = C.c_double(), C.c_int64()
+        dk = np.zeros(self.n_seq)
+        dx = np.zeros(6 * int(self.ns.max()))
+        pd = C.POINTER(C.c_double)
+        per_a = []
+        for v in sub:
+            t0 = time.perf_counter()
+            L.check(ctx._lib.tp_adjoint(ctx.handle, self.tabs[v].handle, self.parts[v].handle, 0, int(self.ts[v]), K,
+                                        flags, C.byref(loss), C.byref(na), dk.ctypes.data_as(pd),
+                                        dx.ctypes.data_as(pd)))
+            per_a.append((time.perf_counter() - t0) * 1e3)
+        print(f"[bench] compare adjoint ({len(sub)} variants, full turns): batched {ba:.0f} ms, per-variant "
+              f"{sum(per_a):.0f} ms", file=sys.stderr, flush=True)
+        sb.close()
+        self.reset()
+        # extrapolation to every variant: time per turn of a plain launch as a function of N
+        lx = np.log(self.ns[sub].astype(float))
+        est_f = float(np.sum(self.ts * np.interp(np.log(self.ns.astype(float)), lx,
+                                                 np.array(per_f) / self.ts[sub])))
+        est_a = float(np.sum(self.ts * np.interp(np.log(self.ns.astype(float)), lx,
+                                                 np.array(per_a) / self.ts[sub])))
+        return {"variants_measured": int(len(sub)), "turns": "each variant's own (full) turn count",
+                "particles_measured": int(self.ns[sub].sum()),
+                "forward_batched_ms": bf, "forward_per_variant_ms": float(sum(per_f)),
+                "forward_speedup": float(sum(per_f)) / bf,
+                "adjoint_batched_ms": ba, "adjoint_per_variant_ms": float(sum(per_a)),
+                "adjoint_speedup": float(sum(per_a)) / ba,
+                "all_variants_per_variant_forward_ms_est": est_f,
+                "all_variants_forward_speedup_est": est_f / fwd_graph_ms,
+                "all_variants_per_variant_adjoint_ms_est": est_a,
+                "all_variants_adjoint_speedup_est": est_a / adj_graph_ms,
+                "note": "subset: the same variants as one batch and as plain launches, both timed; all_variants: "
+                        "per-variant time per turn interpolated in log N from the subset, times each variant's "
+                        "turns, against the measured 256-variant graphs"}
 
 
-def compare_per_variant(args, fast, b, ctx, parts, tabs, ts, ns, n_seq, reset, K, L):
-    """The same variants as 256 plain launches (tp_track / tp_adjoint one variant after the
-    other on the library's stream), with every turn count scaled by --compare-turn-scale."""
-    import ctypes as C
-
-    sc = args.compare_turn_scale
-    tc = np.maximum(1, np.round(ts * sc)).astype(np.int32)
-    reset()
-    b.track(tc, fast=fast)  # plan + capture outside the timed region
-    reset()
-    t0 = time.perf_counter()
-    b.track(tc, fast=fast)
-    bf = time.perf_counter() - t0
-    reset()
-    t0 = time.perf_counter()
-    for p, tab, t in zip(parts, tabs, tc):
-        L.check(ctx._lib.tp_track(ctx.handle, tab.handle, p.handle, 0, int(t),
-                                  (L.TP_TRACK_FAST if fast else 0) | L.TP_TRACK_ASYNC))
-    ctx.sync()
-    pf = time.perf_counter() - t0
-    print(f"[bench] compare forward (turns x {sc}): batched {bf * 1e3:.0f} ms, per-variant {pf * 1e3:.0f} ms",
-          file=sys.stderr, flush=True)
-    reset()
-    b.adjoint(tc, ckpt_stride=K, fast=fast)  # plan + capture
-    reset()
-    t0 = time.perf_counter()
-    b.adjoint(tc, ckpt_stride=K, fast=fast)
-    ba = time.perf_counter() - t0
-    reset()
-    loss, na = C.c_double(), C.c_int64()
-    dk = np.zeros(n_seq)
-    dx = np.zeros(6 * int(ns.max()))
-    pd = C.POINTER(C.c_double)
-    t0 = time.perf_counter()
-    for p, tab, t in zip(parts, tabs, tc):
-        L.check(ctx._lib.tp_adjoint(ctx.handle, tab.handle, p.handle, 0, int(t), K, L.TP_TRACK_FAST if fast else 0,
-                                    C.byref(loss), C.byref(na), dk.ctypes.data_as(pd), dx.ctypes.data_as(pd)))
-    pa = time.perf_counter() - t0
-    print(f"[bench] compare adjoint (turns x {sc}): batched {ba * 1e3:.0f} ms, per-variant {pa * 1e3:.0f} ms",
-          file=sys.stderr, flush=True)
-    return bf, pf, ba, pa
+def batched_results(args, fast, steps, warmup):
+    """Batched forward (steps graph launches), batched adjoint (one graph launch) and the
+    comparison with per-variant launches; returns (sweep, dict of results, forward wall s)."""
+    sw = BatchedSweep(args, fast)
+    wall, kernel_ms = sw.forward(steps, warmup)
+    avg_s = kernel_ms / 1e3 / steps
+    achieved = sw.fpt * sw.live / avg_s
+    g, adj_ms, adj_wall = sw.adjoint()
+    res = {
+        "forward": {"particle_turns_per_s": sw.sched * steps / wall, "live_particle_turns_per_s": sw.live / avg_s,
+                    "graph_ms": avg_s * 1e3, "fp64_TFLOPs": achieved / 1e12, "fp64_frac": achieved / FP64_PEAK,
+                    "work_items": sw.batch.info()["n_items"], "grid_blocks": sw.batch.info()["grid_fwd"]},
+        "batched_adjoint": {"graph_ms": adj_ms, "wall_ms": adj_wall * 1e3, "forward_graph_ms": avg_s * 1e3,
+                            "adjoint_over_forward": adj_ms / (avg_s * 1e3), "ckpt_stride": sw.K,
+                            "ckpt_bytes": sw.info_a["ckpt_bytes"], "grid_back_blocks": sw.info_a["grid_back"],
+                            "mean_loss": float(np.mean(g.loss)), "survivors": int(g.n_alive.sum())},
+        "config": {"variants": sw.nvar, "particles_total": int(sw.ns.sum()), "turns_min": int(sw.ts.min()),
+                   "turns_max": int(sw.ts.max()), "particle_turns": sw.sched, "elements_per_turn": sw.n_seq,
+                   "block": sw.block, "fwd_waves": sw.fwd_waves, "back_waves": sw.back_waves,
+                   "autotuned": sw.tune is not None, "turn_scale": args.turn_scale},
+    }
+    if args.compare_variants > 0:
+        res["vs_per_variant_launches"] = sw.compare(args.compare_variants, avg_s * 1e3, adj_ms)
+    return sw, res, wall, achieved, avg_s
 
 
-def batched_line(args, nvar, ns, ts, sched, wall, n_seq, info_f, fpt, live, avg_s, achieved, adj_ms, adj_wall, K,
-                 info_a, g, bf, pf, ba, pa):
-    return {
+def bench_batched(args, fast):
+    from trackpad_hip import workloads as W
+
+    sw, res, wall, achieved, avg_s = batched_results(args, fast, args.steps, args.warmup)
+    out = {
         "metric": "particle-turns/s FP64 (1 GPU, % FP64 peak); batched adjoint/forward cost ratio",
-        "value": sched * args.steps / wall,
+        "value": sw.sched * args.steps / wall,
         "unit": "particle-turns/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -359,25 +438,51 @@ def batched_line(args, nvar, ns, ts, sched, wall, n_seq, info_f, fpt, live, avg_
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded Gaussian beams, seeded strength perturbations)",
-        "config": {"workload": f"batched_dba_{nvar}_variants", "variants": nvar, "particles_total": int(ns.sum()),
-                   "turns_min": int(ts.min()), "turns_max": int(ts.max()), "particle_turns": sched,
-                   "elements_per_turn": n_seq, "kernel": args.kernel, "block": args.block,
-                   "fwd_waves": args.fwd_waves, "back_waves": args.back_waves, "autotuned": not args.no_autotune,
-                   "work_items": info_f["n_items"], "grid_blocks": info_f["grid_fwd"],
-                   "flops_per_particle_turn": fpt, "turn_scale": args.turn_scale},
-        "live_particle_turns_per_s": live / avg_s,
-        "roofline": {"bound": "mfma", "pipe": "vector FP64 VALU (no MFMA: not a contraction); FP64 peak 78.6 TFLOP/s",
-                     "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK, "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": args.steps},
-        "batched_adjoint": {"graph_ms": adj_ms, "wall_ms": adj_wall * 1e3, "forward_graph_ms": avg_s * 1e3,
-                            "adjoint_over_forward": adj_ms / (avg_s * 1e3), "ckpt_stride": K,
-                            "ckpt_bytes": info_a["ckpt_bytes"], "grid_back_blocks": info_a["grid_back"],
-                            "mean_loss": float(np.mean(g.loss)), "survivors": int(g.n_alive.sum())},
-        "vs_per_variant_launches": {"turn_scale": args.compare_turn_scale, "forward_batched_ms": bf * 1e3,
-                                    "forward_per_variant_ms": pf * 1e3, "forward_speedup": pf / bf,
-                                    "adjoint_batched_ms": ba * 1e3, "adjoint_per_variant_ms": pa * 1e3,
-                                    "adjoint_speedup": pa / ba},
+        "config": dict({"workload": f"batched_dba_{sw.nvar}_variants", "kernel": args.kernel,
+                        "flops_per_particle_turn": sw.fpt}, **res["config"]),
+        "live_particle_turns_per_s": res["forward"]["live_particle_turns_per_s"],
+        "roofline": {"bound": BOUND, "pipe": PIPE, "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK, "traffic": None, "avg_launch_ms": avg_s * 1e3,
+                     "launches": args.steps},
+        "batched_adjoint": res["batched_adjoint"],
     }
+    if "vs_per_variant_launches" in res:
+        out["vs_per_variant_launches"] = res["vs_per_variant_launches"]
+    if not args.no_cpu_baseline:
+        ns_, ts_ = (int(v) for v in args.cpu_sample.lower().split("x"))
+        cores = host_cores()
+        mk = lambda n_: W.dba_particles(n_, seed=1000)
+        v_all, dt_all = cpu_baseline(sw.rings[0], sw.beam, mk, None, ns_, ts_, cores)
+        out["cpu_baseline"] = {"value": v_all, "unit": "particle-turns/s", "cores": cores, "kind": "port",
+                               "sample": f"variant 0, {ns_} particles x {ts_} turns ({dt_all:.1f} s), oracle C++ "
+                                         f"restatement, OpenMP over particles, {cores_note(cores)}"}
+    print(json.dumps(out), flush=True)
+
+
+def parity_leg(ctx, tab, init, work, args, fpt, n):
+    """The parity build (no FMA contraction, IEEE division and square root: the reference's
+    operations in its order) on the same beam after the timed region: autotuned on its own,
+    one launch of --parity-turns turns, timed with HIP events."""
+    from trackpad_hip import track_
+
+    T = args.parity_turns
+    tune = ctx.autotune(tab, init, probe_turns=2, fast=False, cache=args.tune_cache)
+    ctx.set_launch(tune["block"], tune["ppl"], 0)
+    work.copy_from(init)
+    track_(work, tab, nturns=1, fast=False)  # untimed: first launch at this shape
+    work.copy_from(init)
+    ctx.sync()
+    track_(work, tab, nturns=T, fast=False, timed=True)
+    tm = ctx.timing()
+    s = tm["kernel_ms"] / 1e3
+    rate = float(n) * T / s
+    print(f"[bench] parity build: {rate:.3e} particle-turns/s ({T} turns, {tm['kernel_ms']:.0f} ms)", file=sys.stderr,
+          flush=True)
+    return {"particle_turns_per_s": rate, "fp64_TFLOPs": fpt * rate / 1e12, "frac": fpt * rate / FP64_PEAK,
+            "avg_launch_ms": tm["kernel_ms"] / max(tm["launches"], 1), "launches": tm["launches"], "turns": T,
+            "block": tune["block"], "particles_per_lane": tune["ppl"], "fwd_waves": tune["fwd_waves"],
+            "note": "parity build: -ffp-contract=off, IEEE division and square root, the reference's operation "
+                    "order (bit-exact with the oracle where the maps use only + - * / sqrt)"}
 
 
 def streamed(ctx, tab, init, work, args, fast, fpt, n, n_seq):
@@ -421,10 +526,15 @@ def main():
     ap.add_argument("--workload", default="dba", choices=["dba", "fodo", "da", "da_lossless", "diff", "batched"])
     ap.add_argument("--variants", type=int, default=256, help="batched: number of lattice variants")
     ap.add_argument("--turn-scale", type=float, default=1.0, help="batched: scale every T_v (1 = the workload)")
-    ap.add_argument("--compare-turn-scale", type=float, default=0.05,
-                    help="batched: turn scale of the comparison with per-variant launches")
-    ap.add_argument("--ckpt-stride", type=int, default=None,
-                    help="adjoint: turns between checkpoints (default: batched 10, diff 1)")
+    ap.add_argument("--compare-variants", type=int, default=16,
+                    help="batched: variants (spread over the beam sizes) timed as plain per-variant launches at "
+                         "their full turn counts (0 = skip the comparison)")
+    ap.add_argument("--ckpt-stride", type=int, default=1, help="diff: turns between checkpoints")
+    ap.add_argument("--batched-ckpt-stride", type=int, default=10, help="batched adjoint: turns between checkpoints")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="dba: skip the batched legs (256-variant forward, adjoint, per-variant comparison)")
+    ap.add_argument("--parity-turns", type=int, default=50,
+                    help="dba: turns of the parity-build measurement after the timed region (0 = skip)")
     ap.add_argument("--kernel", default="fast", choices=["fast", "exact"])
     ap.add_argument("--particles", type=int, default=0, help="override N (default: the workload's)")
     ap.add_argument("--turns", type=int, default=0, help="override turns per step (default: the workload's)")
@@ -452,8 +562,6 @@ def main():
         args.steps = 1 if batched else 2
     if args.warmup is None:
         args.warmup = 0 if batched else 1
-    if args.ckpt_stride is None:
-        args.ckpt_stride = 10 if batched else 1
     if args.cpu_sample is None:
         args.cpu_sample = "2048x5" if args.workload == "da" else "16384x20"
     if batched:
@@ -540,7 +648,7 @@ def main():
                    "autotuned": tune is not None, "flops_per_particle_turn": fpt},
         "live_particle_turns_per_s": live,
         "lost_particles": int(lost.sum()),
-        "roofline": {"bound": "mfma", "pipe": "vector FP64 VALU (no MFMA: not a contraction); FP64 peak 78.6 TFLOP/s",
+        "roofline": {"bound": BOUND, "pipe": PIPE,
                      "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": traffic,
                      "avg_launch_ms": avg_launch_s * 1e3, "launches": launches},
@@ -555,9 +663,19 @@ def main():
         v_1, dt_1 = cpu_baseline(lat, beam, mk, aperture, n1, ts, 1)
         out["cpu_baseline"] = {"value": v_all, "unit": "particle-turns/s", "cores": cores, "kind": "port",
                                "sample": f"{ns} particles x {ts} turns of the same workload ({dt_all:.1f} s), "
-                                         "oracle C++ restatement, OpenMP over particles"}
+                                         f"oracle C++ restatement, OpenMP over particles, {cores_note(cores)}"}
         out["cpu_baseline_1thread"] = {"value": v_1, "unit": "particle-turns/s", "cores": 1, "kind": "port",
                                        "sample": f"{n1} particles x {ts} turns ({dt_1:.1f} s)"}
+    if args.workload == "dba" and args.parity_turns > 0:
+        out["parity_kernel"] = parity_leg(ctx, tab, init, work, args, fpt, n)
+    if args.workload == "dba" and not args.no_batched:
+        # the metric's second half: the batched sweep, forward and adjoint, one graph launch each
+        del init, work
+        _, res, _, _, _ = batched_results(args, fast, 1, 0)
+        out["batched"] = dict(res["forward"], config=res["config"])
+        out["batched_adjoint"] = res["batched_adjoint"]
+        if "vs_per_variant_launches" in res:
+            out["vs_per_variant_launches"] = res["vs_per_variant_launches"]
     print(json.dumps(out), flush=True)
 
 

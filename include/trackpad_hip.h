@@ -97,6 +97,24 @@ int tp_set_launch(tp_ctx* ctx, int block, int ppl, int turns_per_launch);
  * compiler's budget.  Results do not depend on the budget.  No reference counterpart. */
 int tp_set_waves(tp_ctx* ctx, int fwd_waves, int back_waves);
 
+/* Drift form of the fast build (TP_TRACK_FAST) for later tracking calls on ctx (tp_track,
+ * and the forward passes of tp_adjoint / tp_jacobian).  The fast build evaluates the exact
+ * drift's L / sqrt(pz^2) (src/tracking.jl:87-99) in one of two forms that round
+ * differently and are held to the same accuracy: TP_DRIFT_SERIES, a cubic series about
+ * 1 / sqrt(1 + 2 delta beti + delta^2) where (px^2 + py^2) / that < 2^-14, lane by lane,
+ * and the refined reciprocal square root elsewhere; TP_DRIFT_RSQRT, the refined reciprocal
+ * square root everywhere.  TP_DRIFT_AUTO (the default) takes the series when the particle
+ * set -- its size N, not the number of survivors -- fills at least half of the GPU's
+ * resident wave slots, where it is faster: the choice is fixed per particle set, so the
+ * launches of a call, survivor compaction and a run split into several calls never switch
+ * a particle's form.  Lattices with general kicks of order >= 3 always take the
+ * reciprocal square root; the batched kernels always take the series.  The parity build
+ * ignores the setting.  No reference counterpart. */
+#define TP_DRIFT_AUTO 0
+#define TP_DRIFT_SERIES 1
+#define TP_DRIFT_RSQRT 2
+int tp_set_drift(tp_ctx* ctx, int form);
+
 /* Kernel timing of the last call made with TP_TRACK_TIMED. */
 int tp_timing(tp_ctx* ctx, tp_timing_t* out);
 

@@ -194,9 +194,10 @@ def _f64(a):
 
 
 def ringpass(coords: np.ndarray, lat, beam, lost_flags: np.ndarray, nturns: int, *, turn: int = 0,
-             aperture=None, exact: bool = True, nthreads: int = 1, extended: bool = False):
+             aperture=None, exact: bool = True, nthreads: int = 1, extended: bool = False, quad: bool = False):
     """Oracle ringpass! on host arrays; returns (coords, lost_flags, lost_turn, lost_elem) copies.
-    ``extended`` runs the same algorithm in long double (results rounded to double)."""
+    ``extended`` runs the same algorithm in long double, ``quad`` in __float128 (results
+    rounded to double)."""
     c = np.asfortranarray(np.array(coords, dtype=np.float64))
     n = c.shape[0]
     lf = np.ascontiguousarray(np.array(lost_flags, dtype=np.int64))
@@ -204,10 +205,13 @@ def ringpass(coords: np.ndarray, lat, beam, lost_flags: np.ndarray, nturns: int,
     le = lt.copy()
     arr, ne = elems(lat)
     ap2 = 0.0 if aperture is None else float(aperture) ** 2
-    fn = lib().orc_ringpass_ld if extended else lib().orc_ringpass
-    fn(arr, ne, _f64(c), n, lf.ctypes.data_as(C.POINTER(C.c_int64)), beti(beam), ap2, int(exact),
-                       lt.ctypes.data_as(C.POINTER(C.c_int32)), le.ctypes.data_as(C.POINTER(C.c_int32)),
-                       int(turn), int(nturns), int(nthreads))
+    args = (arr, ne, _f64(c), n, lf.ctypes.data_as(C.POINTER(C.c_int64)), beti(beam), ap2, int(exact),
+            lt.ctypes.data_as(C.POINTER(C.c_int32)), le.ctypes.data_as(C.POINTER(C.c_int32)), int(turn), int(nturns),
+            int(nthreads))
+    if quad:
+        lib().orc_ringpass_q(*args, None, None)
+    else:
+        (lib().orc_ringpass_ld if extended else lib().orc_ringpass)(*args)
     return np.ascontiguousarray(c), lf, lt, le
 
 

@@ -1297,8 +1297,11 @@ int orc_loss_dir(const orc_elem* elems, int n_elem, const double* coords0, int64
     if (dk) el[i].k.d[0] = dk[i];
   }
   D bt(beti);
-  double L = 0.0, dL = 0.0;
-  int64_t alive = 0;
+  // particles in parallel, each into its own slot; the sums run in particle order, so the
+  // result does not depend on the thread count
+  std::vector<double> ls((size_t)(n > 0 ? n : 1), 0.0), dls((size_t)(n > 0 ? n : 1), 0.0);
+  std::vector<char> ok((size_t)(n > 0 ? n : 1), 0);
+#pragma omp parallel for schedule(dynamic, 1)
   for (int64_t p = 0; p < n; ++p) {
     R6<D> r;
     for (int k = 0; k < 6; ++k) { r.v[k] = D(coords0[p + k * n]); if (dr0) r.v[k].d[0] = dr0[p + k * n]; }
@@ -1310,10 +1313,18 @@ int orc_loss_dir(const orc_elem* elems, int n_elem, const double* coords0, int64
         r = rn;
       }
     if (lost) continue;
-    ++alive;
     D s = r.v[0] * r.v[0] + r.v[2] * r.v[2];
-    L += s.v;
-    dL += s.d[0];
+    ls[(size_t)p] = s.v;
+    dls[(size_t)p] = s.d[0];
+    ok[(size_t)p] = 1;
+  }
+  double L = 0.0, dL = 0.0;
+  int64_t alive = 0;
+  for (int64_t p = 0; p < n; ++p) {
+    if (!ok[(size_t)p]) continue;
+    ++alive;
+    L += ls[(size_t)p];
+    dL += dls[(size_t)p];
   }
   *loss = L;
   *dloss = dL;

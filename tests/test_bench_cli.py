@@ -51,3 +51,25 @@ def test_workload_sizes_and_flops():
     assert (n, T, len(lat)) == (1_000_000, 5, 2001) and mk(8).shape == (8, 6)
     beam, lat, n, T, mk, ap = bench.workload("dba", 4096, 100)
     assert (n, T) == (4096, 100) and mk(16).shape == (16, 6)
+
+
+def test_help_lists_the_metric_legs():
+    r = run_bench("--help")
+    assert r.returncode == 0
+    for opt in ("--parity-turns", "--compare-variants", "--no-batched", "--batched-ckpt-stride"):
+        assert opt in r.stdout
+
+
+def test_cpu_baseline_threads(monkeypatch):
+    """The CPU baseline runs on every core the process may use, or on the share that
+    OMP_NUM_THREADS grants it, and says which."""
+    import bench
+
+    vis = bench.visible_cores()
+    monkeypatch.delenv("OMP_NUM_THREADS", raising=False)
+    assert bench.host_cores() == vis and "all visible" in bench.cores_note(vis)
+    monkeypatch.setenv("OMP_NUM_THREADS", "1")
+    assert bench.host_cores() == 1
+    if vis > 1:
+        assert "OMP_NUM_THREADS" in bench.cores_note(1)
+    assert bench.BOUND == "vector_fp64"

@@ -71,7 +71,9 @@ struct TpKArgs {
   // Persistent launch: waves take tiles of 64 x PPL list entries from this queue counter
   // (cleared before the launch) until it runs dry; null = one tile per wave of the grid.
   int32_t* counter;
-  int32_t ser;  // set by the launcher: series drift (the launch fills the SIMDs)
+  // Fast build: series drift (1) or the refined reciprocal square root (0), chosen once per
+  // call by the host (tp_set_drift); variants with general kicks always take the latter.
+  int32_t ser;
 };
 
 // One lattice variant of a batch: where its element table sits in the concatenated
@@ -160,8 +162,9 @@ hipError_t tp_launch_batch_exact(const TpBatchArgs& b, bool ckpt, hipStream_t st
 hipError_t tp_launch_batch_fast(const TpBatchArgs& b, bool ckpt, hipStream_t stream);
 int tp_batch_occupancy_exact(int variant, int exact_h, bool ckpt, int block, int waves);
 // Resident blocks per CU of the tracking kernel that a launch shape selects.
-int tp_track_occupancy_exact(int variant, int exact_h, int block, int ppl, int waves);
-int tp_track_occupancy_fast(int variant, int exact_h, int block, int ppl, int waves);
+// ser: the occupancy of the series-drift kernel (fast build, variants without general kicks).
+int tp_track_occupancy_exact(int variant, int exact_h, int block, int ppl, int waves, int ser);
+int tp_track_occupancy_fast(int variant, int exact_h, int block, int ppl, int waves, int ser);
 int tp_batch_occupancy_fast(int variant, int exact_h, bool ckpt, int block, int waves);
 hipError_t tp_launch_batch_back_exact(const TpBatchArgs& b, hipStream_t stream);
 hipError_t tp_launch_batch_back_fast(const TpBatchArgs& b, hipStream_t stream);

@@ -26,6 +26,7 @@ struct tp_ctx {
   int ppl = 1;
   int tpl = 0;  // turns per launch, 0 = all
   int fwd_waves = 0, back_waves = 0;  // register budgets (0 = the compiler's)
+  int drift = TP_DRIFT_AUTO;           // drift form of the fast build (tp_set_drift)
   int cus = 0;                         // compute units of the device
   int32_t* d_counter = nullptr;        // work-queue counter of persistent tracking launches
   tp_timing_t timing{};
@@ -107,6 +108,12 @@ struct tp_batch {
   } while (0)
 
 namespace tpr {
+
+// Drift form of a fast-build tracking call (TpKArgs::ser): forced by tp_set_drift, or (auto)
+// the series when the particle set -- its size n, not its survivors -- fills at least half of
+// the resident wave slots of the series kernel.  Fixed per particle set, so every launch of a
+// call, and every call on the set, takes the same form.
+int drift_form(const tp_ctx* ctx, int variant, int exact_h, int64_t n, bool fast);
 
 // Sets the thread's error message (tp_last_error) and returns code.
 int fail(int code, const char* fmt, ...);

@@ -579,7 +579,7 @@ TP_DEV void run_turns(St* s, Cx* c, bool* alive, const int32_t* __restrict__ seq
 // if it must); WPE = 0 leaves the budget to the compiler.
 #define TP_WAVES(WPE) __attribute__((amdgpu_waves_per_eu((WPE) > 0 ? (WPE) : 1)))
 
-// SER: the series drift (a launch that fills the SIMDs; the launcher decides per launch).
+// SER: the series drift (the caller decides once per call: TpKArgs::ser).
 template <bool EXH, unsigned FEAT, int PPL, int WPE, bool SER>
 __global__ __launch_bounds__(256) TP_WAVES(WPE) void track_kernel(const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
                                                     const double* __restrict__ pool, double* __restrict__ coords,
@@ -765,17 +765,26 @@ static void launch_v(const TpKArgs& a, int ppl, unsigned grid, int block, hipStr
   launch_vs<EXH, FEAT, false>(a, ppl, grid, block, stream);
 }
 
-template <bool EXH, unsigned FEAT>
-static const void* track_fn(int ppl, int waves) {
+template <bool EXH, unsigned FEAT, bool SER>
+static const void* track_fn_s(int ppl, int waves) {
   const void* fn = nullptr;
   if (ppl == 2) {
-    fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::track_kernel<EXH, FEAT, 2, 0, false>);
+    fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::track_kernel<EXH, FEAT, 2, 0, SER>);
   } else {
-#define TP_CALL(W) fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::track_kernel<EXH, FEAT, 1, W, false>)
+#define TP_CALL(W) fn = reinterpret_cast<const void*>(tpk::TPK_BUILD::track_kernel<EXH, FEAT, 1, W, SER>)
     TP_WPE_SWITCH(waves, TP_CALL)
 #undef TP_CALL
   }
   return fn;
+}
+
+// The kernel a launch with these arguments runs (launch_v's choice), for occupancy queries.
+template <bool EXH, unsigned FEAT>
+static const void* track_fn(int ppl, int waves, int ser) {
+  if constexpr (TP_FAST && (FEAT & FT_GENKICK) == 0) {
+    if (ser) return track_fn_s<EXH, FEAT, true>(ppl, waves);
+  }
+  return track_fn_s<EXH, FEAT, false>(ppl, waves);
 }
 
 template <bool EXH, unsigned FEAT, bool CKPT>
@@ -838,9 +847,9 @@ int TP_BATCH_OCC(int variant, int exact_h, bool ckpt, int block, int waves) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) == hipSuccess ? nb : 0;
 }
 
-int TP_TRACK_OCC(int variant, int exact_h, int block, int ppl, int waves) {
+int TP_TRACK_OCC(int variant, int exact_h, int block, int ppl, int waves, int ser) {
   const void* fn = nullptr;
-#define TP_CALL(E, F) fn = track_fn<E, F>(ppl, waves)
+#define TP_CALL(E, F) fn = track_fn<E, F>(ppl, waves, ser)
   TP_BATCH_DISPATCH(TP_CALL)
 #undef TP_CALL
   int nb = 0;
@@ -852,10 +861,6 @@ hipError_t TP_LAUNCH(const TpKArgs& a_in, int block, int ppl, int max_grid, hipS
   TpKArgs a = a_in;
   const int64_t per_block = (int64_t)block * ppl;
   int64_t grid64 = (a.n + per_block - 1) / per_block;
-  // Series drift when the launch's waves fill at least half of what the GPU holds at once
-  // (max_grid blocks of block/64 waves); measured on the DBA ring: +6 % at 1e6 particles,
-  // -17 % at 4,096 (64 waves, each alone on its SIMD).
-  a.ser = (max_grid > 0 && 2 * ((a.n + 64 * ppl - 1) / (64 * ppl)) >= (int64_t)max_grid * (block / 64)) ? 1 : 0;
   if (a.counter && max_grid > 0 && grid64 > max_grid) grid64 = max_grid;  // persistent grid
   if (grid64 > 0x7fffffff) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)grid64;

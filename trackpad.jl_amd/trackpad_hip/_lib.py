@@ -12,7 +12,8 @@ import os
 import threading
 
 __all__ = ["lib", "TrackPadHIPError", "check", "LIB_PATH", "TP_TRACK_FAST", "TP_TRACK_ASYNC",
-           "TP_TRACK_TIMED", "TimingT", "LaunchCfg", "TP_TUNE_ADJOINT"]
+           "TP_TRACK_TIMED", "TimingT", "LaunchCfg", "TP_TUNE_ADJOINT", "TP_DRIFT_AUTO", "TP_DRIFT_SERIES",
+           "TP_DRIFT_RSQRT"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TRACKPAD_HIP_LIB", os.path.join(os.path.dirname(_HERE), "lib", "libtrackpad_hip.so"))
@@ -20,6 +21,10 @@ LIB_PATH = os.environ.get("TRACKPAD_HIP_LIB", os.path.join(os.path.dirname(_HERE
 TP_TRACK_FAST = 0x1
 TP_TRACK_ASYNC = 0x2
 TP_TRACK_TIMED = 0x4
+
+TP_DRIFT_AUTO = 0
+TP_DRIFT_SERIES = 1
+TP_DRIFT_RSQRT = 2
 
 _STATUS = {0: "TP_OK", -1: "TP_EINVAL", -2: "TP_EUNSUPPORTED", -3: "TP_EHIP", -4: "TP_ENOMEM",
            -5: "TP_ETABLE"}
@@ -83,6 +88,7 @@ _SIGS = {
     "tp_batch_destroy": ([_P], C.c_int),
     "tp_batch_set_launch": ([_P, C.c_int, C.c_int, C.c_int], C.c_int),
     "tp_set_waves": ([_P, C.c_int, C.c_int], C.c_int),
+    "tp_set_drift": ([_P, C.c_int], C.c_int),
     "tp_autotune": ([_P, _P, _P, C.c_int32, C.c_uint32, C.c_double, C.c_char_p, C.c_void_p], C.c_int),
     "tp_batch_autotune": ([_P, _P, _PI32, C.c_int32, C.c_uint32, C.c_char_p, C.c_void_p], C.c_int),
     "tp_batch_track": ([_P, _P, _PI32, _PI32, C.c_uint32], C.c_int),

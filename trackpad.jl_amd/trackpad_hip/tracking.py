@@ -80,6 +80,14 @@ class HIPContext:
         """Register budgets (resident waves per SIMD) of the tracking and reverse kernels."""
         L.check(self._lib.tp_set_waves(self.handle, int(fwd_waves), int(back_waves)))
 
+    def set_drift(self, form: str = "auto") -> None:
+        """Drift form of the fast build for later calls: "auto" (the series for particle sets
+        that fill the GPU, fixed per set), "series" or "rsqrt" (tp_set_drift)."""
+        code = {"auto": L.TP_DRIFT_AUTO, "series": L.TP_DRIFT_SERIES, "rsqrt": L.TP_DRIFT_RSQRT}.get(form)
+        if code is None:
+            raise ValueError(f"drift form {form!r} is not 'auto', 'series' or 'rsqrt'")
+        L.check(self._lib.tp_set_drift(self.handle, code))
+
     def autotune(self, table: "DeviceTable", particles: "HIPParticles", probe_turns: int = 2, *, fast: bool = True,
                  target_launch_ms: float = 0.0, cache: Optional[str] = None) -> dict:
         """Time the candidate launch shapes on a copy of ``particles`` and apply the fastest
