@@ -20,8 +20,11 @@ formulations that are independent of the oracle's code where the mathematics all
 Element constants that the reference computes in double before tracking (SL = L/n and
 the step lengths, irho = angle/L, edge tangents) are formed in double here too, so the
 fixture restates what the reference computes, not a different rounding of the inputs.
-Forest's fringe (bend_fringe) and yrot have no independent form here: they are covered
-by the 50-digit restatement of the same formula in ``forest_*`` cases only.
+Forest's fringe (bend_fringe) has no independent form here: it is covered by the 50-digit
+restatement of the same formula in ``forest_*`` cases (and by symplecticity and its
+derivative identity in tests/test_oracle_pinning.py).  yrot and bend_edge, with the fringe
+off, are pinned by the plane geometry of a hard-edge magnet with rotated faces
+(``wedge_geometry``, the ``ExactSBendWedge*`` cases).
 """
 
 import json
@@ -150,6 +153,64 @@ def body_geometry(r, irho, L, bt):
         dphi -= 2 * mp.pi
     s_over_d = R * dphi / dd                # horizontal arc length / d
     return [t - rho, px1, y + py * s_over_d, py, z + dp1 * s_over_d - mp.mpf(L) * bt, d]
+
+
+def _cross(a, b):
+    return a[0] * b[1] - a[1] * b[0]
+
+
+def wedge_geometry(r, irho, L, e1, e2, bt):
+    """Hard-edge sector dipole whose entrance and exit faces are rotated by e1 and e2 (pure
+    dipole, irho > 0), by plane geometry, from the entrance reference plane to the exit
+    reference plane: a straight line to the entrance face, the particle's circle from there to
+    the exit face, a straight line to the exit reference plane.  The frame is the one of
+    ``body_geometry``: bend centre at the origin, reference circle of radius rho, entrance
+    reference point (rho, 0) moving along +Y, x outward.  The entrance face passes through
+    the reference point at angle e1 to the radial line (outer particles reach it later for
+    e1 > 0, the rectangular-bend sense: e1 = e2 = angle/2 makes the faces parallel), the exit
+    face at angle angle - e2.  The reference computes the same hard-edge magnet as Forest's
+    yrot(e1), bend_edge(-e1), body, bend_edge(-e2), yrot(e2) (src/tracking.jl:901-973 with
+    the fringe off)."""
+    x, px, y, py, z, d = r
+    rho = 1 / mp.mpf(irho)
+    th = mp.mpf(irho) * mp.mpf(L)
+    dp1 = bt + d
+    dd = mp.sqrt(dp1 * dp1 - py * py)
+    pz = mp.sqrt(dd * dd - px * px)
+    R = dd * rho
+    P0 = (rho, mp.mpf(0))
+    P1 = (rho * mp.cos(th), rho * mp.sin(th))
+    er1 = (mp.cos(th), mp.sin(th))
+    t1 = (-mp.sin(th), mp.cos(th))
+    f0 = (mp.cos(mp.mpf(e1)), mp.sin(mp.mpf(e1)))
+    f1 = (mp.cos(th - mp.mpf(e2)), mp.sin(th - mp.mpf(e2)))
+    # straight line from the entrance reference plane to the entrance face
+    A = (rho + x, mp.mpf(0))
+    u0 = (px / dd, pz / dd)
+    s0 = _cross((P0[0] - A[0], P0[1] - A[1]), f0) / _cross(u0, f0)
+    Q0 = (A[0] + s0 * u0[0], A[1] + s0 * u0[1])
+    # circle through the field to the exit face (the crossing nearest the reference exit)
+    C = (Q0[0] - R * u0[1], Q0[1] + R * u0[0])
+    b = f1[0] * (P1[0] - C[0]) + f1[1] * (P1[1] - C[1])
+    c = (P1[0] - C[0]) ** 2 + (P1[1] - C[1]) ** 2 - R * R
+    disc = mp.sqrt(b * b - c)
+    t = min((-b + disc, -b - disc), key=abs)
+    Q1 = (P1[0] + t * f1[0], P1[1] + t * f1[1])
+    a0 = mp.atan2(Q0[1] - C[1], Q0[0] - C[0])
+    a1 = mp.atan2(Q1[1] - C[1], Q1[0] - C[0])
+    dphi = a1 - a0
+    while dphi < -mp.pi:
+        dphi += 2 * mp.pi
+    while dphi > mp.pi:
+        dphi -= 2 * mp.pi
+    u1 = (-(Q1[1] - C[1]) / R, (Q1[0] - C[0]) / R)
+    # straight line from the exit face to the exit reference plane
+    s2 = -((Q1[0] - P1[0]) * t1[0] + (Q1[1] - P1[1]) * t1[1]) / (u1[0] * t1[0] + u1[1] * t1[1])
+    Q2 = (Q1[0] + s2 * u1[0], Q1[1] + s2 * u1[1])
+    S = s0 + R * dphi + s2                  # horizontal path length
+    x1 = (Q2[0] - P1[0]) * er1[0] + (Q2[1] - P1[1]) * er1[1]
+    px1 = dd * (u1[0] * er1[0] + u1[1] * er1[1])
+    return [x1, px1, y + py * S / dd, py, z + dp1 * S / dd - mp.mpf(L) * bt, d]
 
 
 def edge_lin(r, irho, e, fint, gap, method, is_exit):
@@ -291,6 +352,8 @@ def run_case(kind, p, r, bt):
         r = bend_edge(r, irho, -e2, bt)
         r = bend_fringe(r, -irho, p["gk"], bt)
         return yrot(r, e2, bt)
+    if kind == "wedge":
+        return wedge_geometry(r, p["angle"] / p["L"], p["L"], p["e1"], p["e2"], bt)
     if kind == "rf":
         L = p["L"]
         beta = 1 / mp.mpf(bt)
@@ -327,6 +390,19 @@ CASES = [
                                        "e2": math.pi / 18, "n": 10, "gk": 0.02}),
     ("RFCavity", "rf", {"L": 0.3, "volt": 2.0e6, "freq": 500.0e6, "lag": 0.002, "philag": 0.3,
                         "energy": 3.5e9}),
+    # linear edge methods 2 (SOLEIL) and 3 (THOMX), entrance and exit, with fringe
+    # (src/tracking.jl:174-178, 207-213; the 50-digit values restate the edge formula)
+    ("SBendSOLEIL", "sbend", {"L": 1.5, "angle": 0.157, "e1": 0.02, "e2": 0.03, "n": 10, "fint1": 0.5,
+                              "fint2": 0.4, "gap": 0.03, "fbe": 2, "fbx": 2}),
+    ("SBendTHOMX", "sbend", {"L": 1.5, "angle": 0.157, "e1": 0.02, "e2": 0.03, "n": 10, "fint1": 0.5,
+                             "fint2": 0.4, "gap": 0.03, "fbe": 3, "fbx": 3}),
+    ("SBendMixedMethods", "sbend", {"L": 0.9, "angle": -0.12, "e1": -0.01, "e2": 0.04, "n": 8, "fint1": 0.6,
+                                    "fint2": 0.5, "gap": 0.05, "fbe": 3, "fbx": 2}),
+    # Forest's rotated faces with the fringe off, against the plane geometry of a hard-edge
+    # wedge magnet (an independent construction: wedge_geometry)
+    ("ExactSBendWedge", "wedge", {"L": 0.9, "angle": 0.15, "e1": 0.03, "e2": 0.02, "n": 0}),
+    ("ExactSBendWedgeRect", "wedge", {"L": 1.0, "angle": 0.3, "e1": 0.15, "e2": 0.15, "n": 10}),
+    ("ExactSBendWedgeMixed", "wedge", {"L": 1.2, "angle": 0.2, "e1": -0.05, "e2": 0.08, "n": 4}),
 ]
 
 

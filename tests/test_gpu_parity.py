@@ -428,3 +428,46 @@ def test_host_entry_points(ctx):
     ringpass_(c, lat, beam, lf, 3, ctx=ctx)
     oc, _, _, _ = O.ringpass(p10(), lat, beam, np.zeros(10, np.int64), 3)
     assert np.array_equal(c, oc)
+
+
+@pytest.mark.parametrize("fbe,fbx", [(2, 2), (3, 3), (3, 2), (2, 3)])
+def test_edge_methods_soleil_thomx(ctx, fbe, fbx):
+    """Linear edge methods 2 (SOLEIL: the vertical lens divided by 1 + delta) and 3 (THOMX: the
+    edge angle seen with the slope px / (1 + delta), + at the entrance, - at the exit),
+    src/tracking.jl:174-178, 207-213, with fringe (fint gap != 0) on both faces: one pass of
+    the reference's 10 x 6 input and of a wider beam (|px| to 5e-3, delta to 2 %), parity build
+    within the reference tests' 1e-15 and the fast build within the oracle-pinning bounds; a
+    ring of such bends with quads for 1 and 50 turns, parity build within 1e-12 / 1e-9 of the
+    oracle."""
+    from trackpad_hip import SBend
+
+    beam = Beam(ENERGY_VAL)
+    bend = SBend(1.5, 0.157, 0.02, 0.03, num_int_steps=10, fint1=0.5, fint2=0.4, gap=0.03,
+                 fringe_bend_entrance=fbe, fringe_bend_exit=fbx)
+    lat = Lattice([bend])
+    rng = np.random.default_rng(fbe * 10 + fbx)
+    wide = np.column_stack([rng.normal(0, 2e-3, 64), rng.normal(0, 5e-3, 64), rng.normal(0, 2e-3, 64),
+                            rng.normal(0, 5e-3, 64), rng.normal(0, 1e-3, 64), rng.uniform(-0.02, 0.02, 64)])
+    for c0 in (p10(), wide):
+        n = len(c0)
+        oc, olf, _, _ = O.linepass(c0, lat, beam, np.zeros(n, np.int64))
+        gc, glf, _, _ = gpu_track(ctx, lat, beam, c0, 1)
+        assert np.array_equal(glf, olf)
+        assert np.max(np.abs(gc - oc)) < PARITY_ATOL * max(1.0, np.abs(c0).max() / 1e-3)
+        fc, flf, _, _ = gpu_track(ctx, lat, beam, c0, 1, fast=True)
+        assert np.array_equal(flf, olf)
+        assert np.all(np.max(np.abs(fc - oc), axis=0) <= TOL_PLAIN * 10 * max(1.0, np.abs(c0).max() / 1e-3))
+    # the methods differ from Brown's on this input (the cases exercise them)
+    brown = Lattice([SBend(1.5, 0.157, 0.02, 0.03, num_int_steps=10, fint1=0.5, fint2=0.4, gap=0.03)])
+    ob = O.linepass(wide, brown, beam, np.zeros(64, np.int64))[0]
+    om = O.linepass(wide, lat, beam, np.zeros(64, np.int64))[0]
+    assert np.max(np.abs(ob[:, 3] - om[:, 3])) > 1e-9
+    cell = [Drift(1.0), Quadrupole(0.4, 1.1), Drift(0.5), bend, Drift(0.5), Quadrupole(0.4, -1.0), Drift(1.0)]
+    ring = Lattice(cell * 6)
+    c0 = W.gaussian_particles(256, 9, 1e-4, 1e-5, 1e-4, 1e-5, 1e-3)
+    for nturns, rtol in ((1, 1e-12), (50, 1e-9)):
+        oc, olf, olt, ole = O.ringpass(c0, ring, beam, np.zeros(256, np.int64), nturns, nthreads=8)
+        gc, glf, glt, gle = gpu_track(ctx, ring, beam, c0, nturns)
+        assert np.array_equal(glf, olf) and np.array_equal(glt, olt)
+        ok, err = ring_close(gc, oc, rtol)
+        assert ok, (nturns, err)

@@ -56,7 +56,8 @@ def _elem(c):
         return ThinMultipole(0.0, p["a"], p["b"], max_order=p["m"])
     if k == "sbend":
         return SBend(p["L"], p["angle"], p["e1"], p["e2"], num_int_steps=p["n"], fint1=p.get("fint1", 0.0),
-                     fint2=p.get("fint2", 0.0), gap=p.get("gap", 0.0))
+                     fint2=p.get("fint2", 0.0), gap=p.get("gap", 0.0), fringe_bend_entrance=p.get("fbe", 1),
+                     fringe_bend_exit=p.get("fbx", 1))
     if k == "esbend_body":
         return ExactSBendLinearEdge(p["L"], p["angle"], 0.0, 0.0, num_int_steps=p["n"], fringe_bend_entrance=0,
                                     fringe_bend_exit=0)
@@ -67,6 +68,9 @@ def _elem(c):
         return ExactSBend(p["L"], p["angle"], p["e1"], p["e2"], num_int_steps=p["n"], gk=p["gk"])
     if k == "rf":
         return RFCavity(p["L"], p["volt"], p["freq"], p["lag"], philag=p["philag"], energy=p["energy"])
+    if k == "wedge":
+        return ExactSBend(p["L"], p["angle"], p["e1"], p["e2"], num_int_steps=p["n"], fringe_bend_entrance=0,
+                          fringe_bend_exit=0)
     raise ValueError(k)
 
 
@@ -77,7 +81,7 @@ def test_oracle_vs_mpmath(case):
     e = _elem(c)
     ref = np.array([[float(v) for v in row] for row in c["out"]])
     out = np.array([O.pass_elem(e, r, beam) for r in GOLD["particles"]])
-    tol = TOL_EXACT_BEND if c["kind"] in ("esbend_body", "esbend_le", "forest_esbend") else TOL_PLAIN
+    tol = TOL_EXACT_BEND if c["kind"] in ("esbend_body", "esbend_le", "forest_esbend", "wedge") else TOL_PLAIN
     err = np.max(np.abs(out - ref), axis=0)
     assert np.all(err <= tol), (err, tol)
     # and the map is not the identity (the fixture exercises it)
