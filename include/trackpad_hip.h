@@ -162,6 +162,20 @@ int tp_particles_device_ptrs(tp_particles* p, double** coords, int32_t** lost_tu
 int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns,
              uint32_t flags);
 
+/* tp_track without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory during a tracking call (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments, results
+ * (bit for bit) and errors as tp_track; the survivor lists and the persistent launches'
+ * queue counter, which tp_track allocates with the particle set on first use, live instead
+ * in a library-owned static device array of 8 MiB per device (held until the process ends),
+ * so particle sets larger than TP_NOALLOC_MAX_N particles are refused with TP_EINVAL.  Every
+ * call on one device shares that array: calls that overlap on the same device (from
+ * different contexts or threads, or TP_TRACK_ASYNC calls on different streams) race, and the
+ * caller must serialise them. */
+#define TP_NOALLOC_MAX_N 1048576
+int tp_track_noalloc(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns,
+                     uint32_t flags);
+
 /* Wait for all work enqueued on the context's stream. */
 int tp_sync(tp_ctx* ctx);
 

@@ -15,7 +15,15 @@
 #include <string>
 #include <vector>
 
+#include <mutex>
+
 #include "tp_runtime.h"
+
+// Scratch of tp_track_noalloc: word 0 is the persistent launches' queue counter, words
+// 64 .. 64 + 2 TP_NOALLOC_MAX_N the two survivor lists and their count.  One copy per device,
+// held until the process ends (8 MiB).  Host code never names it directly: its device address
+// comes from hipGetSymbolAddress (scratch_ptr below).
+static __device__ __align__(256) int32_t scratch[64 + 2 * (size_t)TP_NOALLOC_MAX_N + 64];
 
 namespace tpr {
 
@@ -116,6 +124,24 @@ int set_dev(tp_ctx* ctx) {
   return TP_OK;
 }
 
+// Device address of `scratch` on `device` (which must be current), looked up once per device
+// and kept: each device holds its own copy of the array at its own address.
+int scratch_ptr(int device, int32_t** out) {
+  static std::mutex mu;
+  static std::vector<int32_t*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((size_t)device >= addr.size()) addr.resize((size_t)device + 1, nullptr);
+  if (!addr[(size_t)device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(TP_EHIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device, hipGetErrorName(e));
+    addr[(size_t)device] = static_cast<int32_t*>(ptr);
+  }
+  *out = addr[(size_t)device];
+  return TP_OK;
+}
+
 void ctx_release(tp_ctx* ctx) {
   if (ctx->refs.fetch_sub(1) != 1) return;
   (void)hipSetDevice(ctx->device);
@@ -179,6 +205,9 @@ int tp_ctx_create(int device, tp_ctx** out) {
     delete c;
     return fail(TP_EHIP, "tp_ctx_create: %s", hipGetErrorName(e));
   }
+  // the static scratch of tp_track_noalloc, looked up once per device at set-up; a failure
+  // here leaves it unset and tp_track_noalloc reports it (the other entry points do not need it)
+  if (scratch_ptr(device, &c->d_scratch) != TP_OK) c->d_scratch = nullptr;
   *out = c;
   return TP_OK;
 }
@@ -495,13 +524,21 @@ int tp_particles_device_ptrs(tp_particles* p, double** coords, int32_t** lost_tu
   return TP_OK;
 }
 
-int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, uint32_t flags) {
-  if (!ctx || !t || !p) return fail(TP_EINVAL, "tp_track: NULL argument");
-  if (t->ctx != ctx || p->ctx != ctx) return fail(TP_EINVAL, "tp_track: table and particles belong to another context");
-  if (ctx->closed) return fail(TP_EINVAL, "tp_track: context destroyed");
-  if (nturns < 0) return fail(TP_EINVAL, "tp_track: nturns = %d", nturns);
-  if (flags & ~(TP_TRACK_FAST | TP_TRACK_ASYNC | TP_TRACK_TIMED)) return fail(TP_EINVAL, "tp_track: unknown flags 0x%x", flags);
-  if ((int64_t)turn0 + (int64_t)nturns > 0x7fffffffLL) return fail(TP_EINVAL, "tp_track: turn range overflows int32");
+}  // extern "C"
+
+namespace {
+
+// tp_track and tp_track_noalloc.  lists: 2 n + 1 int32 of device memory for the survivor
+// lists and their count, or null to use (and on first need allocate) the particle set's own;
+// counter: the persistent launches' queue counter, or null for the context's.
+int track_impl(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, uint32_t flags,
+               int32_t* lists_buf, int32_t* counter, const char* who) {
+  if (!ctx || !t || !p) return fail(TP_EINVAL, "%s: NULL argument", who);
+  if (t->ctx != ctx || p->ctx != ctx) return fail(TP_EINVAL, "%s: table and particles belong to another context", who);
+  if (ctx->closed) return fail(TP_EINVAL, "%s: context destroyed", who);
+  if (nturns < 0) return fail(TP_EINVAL, "%s: nturns = %d", who, nturns);
+  if (flags & ~(TP_TRACK_FAST | TP_TRACK_ASYNC | TP_TRACK_TIMED)) return fail(TP_EINVAL, "%s: unknown flags 0x%x", who, flags);
+  if ((int64_t)turn0 + (int64_t)nturns > 0x7fffffffLL) return fail(TP_EINVAL, "%s: turn range overflows int32", who);
   int rc = set_dev(ctx);
   if (rc) return rc;
   TpKArgs a{};
@@ -525,7 +562,7 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
   a.row_turn0 = t->hdr.row_turn0;
   if (a.n_rows > 1 && nturns > 0 &&
       ((int64_t)turn0 < a.row_turn0 || (int64_t)turn0 + nturns > (int64_t)a.row_turn0 + a.n_rows))
-    return fail(TP_EINVAL, "tp_track: turns [%d, %lld) outside the table's time-varying rows [%d, %lld)", turn0,
+    return fail(TP_EINVAL, "%s: turns [%d, %lld) outside the table's time-varying rows [%d, %lld)", who, turn0,
                 (long long)turn0 + nturns, a.row_turn0, (long long)a.row_turn0 + a.n_rows);
   const bool timed = (flags & TP_TRACK_TIMED) != 0;
   const int chunk = ctx->tpl > 0 ? ctx->tpl : (nturns > 0 ? nturns : 1);
@@ -534,9 +571,12 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
   // following launches.  It needs a count read back per launch, so asynchronous calls and
   // single launches skip it.
   const bool compact = !(flags & TP_TRACK_ASYNC) && nturns > chunk && p->n <= 0x7fffffffLL;
-  if (compact && !p->d_order) TP_HIP(hipMalloc(&p->d_order, (size_t)(2 * p->n + 1) * sizeof(int32_t)));
-  int32_t* lists[2] = {p->d_order, p->d_order ? p->d_order + p->n : nullptr};
-  int32_t* d_count = p->d_order ? p->d_order + 2 * p->n : nullptr;
+  if (compact && !lists_buf && !p->d_order)
+    TP_HIP(hipMalloc(&p->d_order, (size_t)(2 * p->n + 1) * sizeof(int32_t)));
+  int32_t* lbuf = lists_buf ? lists_buf : p->d_order;
+  int32_t* lists[2] = {lbuf, lbuf ? lbuf + p->n : nullptr};
+  int32_t* d_count = lbuf ? lbuf + 2 * p->n : nullptr;
+  int32_t* queue = counter ? counter : ctx->d_counter;
   int cur = -1;  // list in use (-1: all particles)
   // A launch with more waves than the device holds at once runs persistent: a resident
   // grid whose waves take 64-particle tiles from a queue (TpKArgs::counter).
@@ -561,7 +601,7 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
     // first launch
     hipError_t e = tp_launch_fill_i32(d_count, 0, 1, ctx->stream);
     if (e == hipSuccess) e = tp_launch_compact_alive(nullptr, (int32_t)a.n, p->d_lt, lists[0], d_count, ctx->stream);
-    if (e != hipSuccess) return fail(TP_EHIP, "tp_track: compaction launch failed: %s", hipGetErrorName(e));
+    if (e != hipSuccess) return fail(TP_EHIP, "%s: compaction launch failed: %s", who, hipGetErrorName(e));
     int32_t alive = 0;
     TP_HIP(hipMemcpyAsync(&alive, d_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     TP_HIP(hipStreamSynchronize(ctx->stream));
@@ -587,14 +627,14 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
       }
       a.e_last = e_end < a.n_seq ? e_end : 0;
       const int64_t blocks = (a.n + (int64_t)ctx->block * ctx->ppl - 1) / ((int64_t)ctx->block * ctx->ppl);
-      a.counter = (max_grid > 0 && blocks > max_grid) ? ctx->d_counter : nullptr;
+      a.counter = (max_grid > 0 && blocks > max_grid) ? queue : nullptr;
       hipError_t e = hipSuccess;
       if (a.counter) e = tp_launch_fill_i32(a.counter, 0, 1, ctx->stream);
-      if (e != hipSuccess) return fail(TP_EHIP, "tp_track: queue reset failed: %s", hipGetErrorName(e));
+      if (e != hipSuccess) return fail(TP_EHIP, "%s: queue reset failed: %s", who, hipGetErrorName(e));
       if (timed) TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
       e = fast ? tp_launch_track_fast(a, ctx->block, ctx->ppl, max_grid, ctx->stream)
                : tp_launch_track_exact(a, ctx->block, ctx->ppl, max_grid, ctx->stream);
-      if (e != hipSuccess) return fail(TP_EHIP, "tp_track: kernel launch failed: %s", hipGetErrorName(e));
+      if (e != hipSuccess) return fail(TP_EHIP, "%s: kernel launch failed: %s", who, hipGetErrorName(e));
       ++launches;
       if (timed) {
         TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
@@ -618,7 +658,7 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
         if (e == hipSuccess)
           e = tp_launch_compact_alive(cur < 0 ? nullptr : lists[cur], (int32_t)a.n, p->d_lt, lists[nxt], d_count,
                                       ctx->stream);
-        if (e != hipSuccess) return fail(TP_EHIP, "tp_track: compaction launch failed: %s", hipGetErrorName(e));
+        if (e != hipSuccess) return fail(TP_EHIP, "%s: compaction launch failed: %s", who, hipGetErrorName(e));
         int32_t alive = 0;
         TP_HIP(hipMemcpyAsync(&alive, d_count, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
         TP_HIP(hipStreamSynchronize(ctx->stream));
@@ -647,6 +687,30 @@ int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int
   }
   if (!(flags & TP_TRACK_ASYNC)) TP_HIP(hipStreamSynchronize(ctx->stream));
   return TP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tp_track(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns, uint32_t flags) {
+  return track_impl(ctx, t, p, turn0, nturns, flags, nullptr, nullptr, "tp_track");
+}
+
+int tp_track_noalloc(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, int32_t nturns,
+                     uint32_t flags) {
+  if (!ctx || !t || !p) return fail(TP_EINVAL, "tp_track_noalloc: NULL argument");
+  if (p->n > (int64_t)TP_NOALLOC_MAX_N)
+    return fail(TP_EINVAL, "tp_track_noalloc: %lld particles exceed the static scratch maximum of %d",
+                (long long)p->n, (int)TP_NOALLOC_MAX_N);
+  if (!ctx->d_scratch) {
+    const int rc = set_dev(ctx);
+    if (rc) return rc;
+    if (scratch_ptr(ctx->device, &ctx->d_scratch) != TP_OK) return TP_EHIP;  // message set by scratch_ptr
+  }
+  // word 0: queue counter; from word 64: list 0 | list 1 | count (each call clears what it
+  // uses: fill_i32 before every persistent launch and every compaction)
+  return track_impl(ctx, t, p, turn0, nturns, flags, ctx->d_scratch + 64, ctx->d_scratch, "tp_track_noalloc");
 }
 
 int tp_sync(tp_ctx* ctx) {

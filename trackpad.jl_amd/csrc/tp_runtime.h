@@ -29,6 +29,7 @@ struct tp_ctx {
   int drift = TP_DRIFT_AUTO;           // drift form of the fast build (tp_set_drift)
   int cus = 0;                         // compute units of the device
   int32_t* d_counter = nullptr;        // work-queue counter of persistent tracking launches
+  int32_t* d_scratch = nullptr;        // device address of the static scratch (tp_track_noalloc)
   tp_timing_t timing{};
 };
 

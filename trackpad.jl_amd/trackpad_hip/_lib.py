@@ -13,7 +13,7 @@ import threading
 
 __all__ = ["lib", "TrackPadHIPError", "check", "LIB_PATH", "TP_TRACK_FAST", "TP_TRACK_ASYNC",
            "TP_TRACK_TIMED", "TimingT", "LaunchCfg", "TP_TUNE_ADJOINT", "TP_DRIFT_AUTO", "TP_DRIFT_SERIES",
-           "TP_DRIFT_RSQRT"]
+           "TP_DRIFT_RSQRT", "TP_NOALLOC_MAX_N"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TRACKPAD_HIP_LIB", os.path.join(os.path.dirname(_HERE), "lib", "libtrackpad_hip.so"))
@@ -25,6 +25,8 @@ TP_TRACK_TIMED = 0x4
 TP_DRIFT_AUTO = 0
 TP_DRIFT_SERIES = 1
 TP_DRIFT_RSQRT = 2
+
+TP_NOALLOC_MAX_N = 1048576
 
 _STATUS = {0: "TP_OK", -1: "TP_EINVAL", -2: "TP_EUNSUPPORTED", -3: "TP_EHIP", -4: "TP_ENOMEM",
            -5: "TP_ETABLE"}
@@ -80,6 +82,7 @@ _SIGS = {
     "tp_particles_download": ([_P, _PD, _PI64, _PI32, _PI32], C.c_int),
     "tp_particles_device_ptrs": ([_P, C.POINTER(_PD), C.POINTER(_PI32), C.POINTER(_PI32), _PI64], C.c_int),
     "tp_track": ([_P, _P, _P, C.c_int32, C.c_int32, C.c_uint32], C.c_int),
+    "tp_track_noalloc": ([_P, _P, _P, C.c_int32, C.c_int32, C.c_uint32], C.c_int),
     "tp_sync": ([_P], C.c_int),
     "tp_adjoint": ([_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _PD, _PI64, _PD, _PD], C.c_int),
     "tp_jacobian": ([_P, _P, _P, C.c_int32, C.c_int32, C.c_int32, C.c_uint32, _PD], C.c_int),

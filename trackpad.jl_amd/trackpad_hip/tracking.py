@@ -274,11 +274,13 @@ def linepass_(coords: np.ndarray, lat: Lattice, beam: Beam, lost_flags: np.ndarr
 
 def track_(particles: HIPParticles, lat, beam: Optional[Beam] = None, nturns: int = 1, *,
            turn: int = 0, fast: bool = False, timed: bool = False, aperture: Optional[float] = None,
-           time: float = 0.0, dt_turn: float = 0.0) -> None:
+           time: float = 0.0, dt_turn: float = 0.0, no_alloc: bool = False) -> None:
     """Track device-resident particles for ``nturns`` turns (``track!`` of the Julia package).
 
     ``lat`` is a :class:`Lattice` (compiled with ``beam``) or an already uploaded
-    :class:`DeviceTable`.
+    :class:`DeviceTable`.  ``no_alloc`` calls tp_track_noalloc: the same results without any
+    device allocation in the call (at most TP_NOALLOC_MAX_N particles; overlapping calls on
+    one device must be serialised by the caller).
     """
     ctx = particles.ctx
     if isinstance(lat, DeviceTable):
@@ -288,5 +290,5 @@ def track_(particles: HIPParticles, lat, beam: Optional[Beam] = None, nturns: in
             raise ValueError("track_ needs a Beam to compile a Lattice")
         tab = ctx.table(compile_table(lat, beam, aperture=aperture, turn=turn, nturns=nturns, time=time,
                                       dt_turn=dt_turn))
-    L.check(ctx._lib.tp_track(ctx.handle, tab.handle, particles.handle, int(turn), int(nturns),
-                              _flags(fast, timed)))
+    fn = ctx._lib.tp_track_noalloc if no_alloc else ctx._lib.tp_track
+    L.check(fn(ctx.handle, tab.handle, particles.handle, int(turn), int(nturns), _flags(fast, timed)))
