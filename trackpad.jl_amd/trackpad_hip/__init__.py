@@ -25,6 +25,7 @@ from .tracking import (  # noqa: F401
 
 from .adjoint import Gradients, ringpass_grad, set_strengths, strength_positions  # noqa: F401
 from .batch import BatchGradients, HIPBatch  # noqa: F401
+from .madx import MadxError, read_madx, write_madx  # noqa: F401
 from .optics import (  # noqa: F401
     fastfindm66, fastfindm66_refpts, find_closed_orbit_4d, find_closed_orbit_6d, findm66, findm66_refpts, getchrom,
     gettune, jacobians, one_turn_map, tune_from_map,
