@@ -92,6 +92,68 @@ TP_DEV void esbend_body_back(St& s, Ad& a, const Cx& c, const double* __restrict
   const double L1 = p[1], L2 = p[2], K1 = p[3], K2 = p[4], L1b = p[5], L2b = p[6];
   const double c1 = p[7], s1 = p[8], c2 = p[9], s2 = p[10];
   const double* poly = p + 24;
+#if TP_FAST
+  if constexpr (KK == TP_KICK_NONE) {
+    // Pure dipole: the forward kernel applies every sub-body as a rotation of (px, pzmx),
+    // pzmx = pz - 1 - x irho, with x formed at the end and the path terms telescoped (see
+    // esbend_body).  Undone as a whole: the inverse rotations in reverse order (the step is a
+    // palindrome) bring (px, pzmx) back to the element's entry, and, the rotations being
+    // orthogonal, the same matrices carry the adjoint of (px, pzmx) back -- one loop for
+    // both.  The element map, with B = beti + delta, dd = B^2 - 1 - py^2 and
+    // pz(a) = sqrt(B^2 - a^2 - py^2):
+    //   b0 = pz(px) - 1 - x irho, (aN, bN) = R (px, b0), x' = (pz(aN) - 1 - bN) rho,
+    //   S = Ltot + rho (asin(px/d2) - asin(aN/d2)), y' = y + py S, z' = z + B S - Lbtot,
+    //   px' = aN, d2 = sqrt(B^2 - py^2), d asin(a/d2)/da = 1/pz(a).
+    const double xo = s.x, aN = s.px, py = s.py, B = c.B;
+    const double Ltot = (double)ns * (2.0 * L1 + 2.0 * L2);
+    const double Lbtot = (double)ns * (2.0 * L1b + 2.0 * L2b);
+    const double dd = __builtin_fma(-py, py, (B - 1.0) * (B + 1.0));
+    const double pzN = pz, ipzN = ipz;  // pxyz of the output px (the caller's)
+    const double bN = __builtin_fma(-xo, irho, __builtin_fma(-aN, aN, dd) * rcp_nr(pzN + 1.0));
+    if (__builtin_expect(__builtin_fma(aN, aN, bN * bN) < 0.999999 * (d2 * d2), 1)) {
+      // adjoint parts that need only the output side
+      const double aS = a.y * py + a.z * B;   // d/dS
+      const double adif = aS * rho;           // d/d(asin difference)
+      const double apzN = a.x * rho;          // x' = (pzN - 1 - bN) rho
+      double ga = a.px - adif * ipzN - apzN * aN * ipzN;  // d/d aN
+      double gb = -apzN;                                   // d/d bN
+      double ra = aN, rb = bN;
+      for (int st = 0; st < ns; ++st) {
+#define TP_IROT(CS, SN)                                   \
+  {                                                       \
+    const double ra_ = __builtin_fma(ra, CS, -(rb * SN)); \
+    rb = __builtin_fma(ra, SN, rb * CS);                  \
+    ra = ra_;                                             \
+    const double ga_ = __builtin_fma(ga, CS, -(gb * SN)); \
+    gb = __builtin_fma(ga, SN, gb * CS);                  \
+    ga = ga_;                                             \
+  }
+        TP_IROT(c1, s1) TP_IROT(c2, s2) TP_IROT(c2, s2) TP_IROT(c1, s1)
+#undef TP_IROT
+      }
+      double ipz0;
+      const double pz0 = pxyz_r(c, ra, py, ipz0);
+      const double S = __builtin_fma(asin_diff(ra, pz0, aN, pzN, d2, id2), rho, Ltot);
+      const double ad2 = adif * id2 * (aN * ipzN - ra * ipz0);
+      // b0 = pz0 - 1 - x irho
+      const double apz0 = gb;
+      const double ax = -gb * irho;
+      ga = ga + adif * ipz0 - apz0 * ra * ipz0;
+      const double apy = a.py + a.y * S - apzN * py * ipzN - apz0 * py * ipz0 - ad2 * py * id2;
+      const double aB = a.z * S + (apzN * ipzN + apz0 * ipz0 + ad2 * id2) * B;
+      s.x = (__builtin_fma(-ra, ra, dd) * rcp_nr(pz0 + 1.0) - rb) * rho;
+      s.px = ra;
+      s.y = __builtin_fma(-py, S, s.y);
+      s.z = s.z - __builtin_fma(B, S, -Lbtot);
+      a.x = ax;
+      a.px = ga;
+      a.py = apy;
+      a.d = a.d + aB;
+      return;
+    }
+    // a grazing particle: the forward kernel took the reference's sub-bodies; so does this
+  }
+#endif
   // The four bodies of a step (lengths L1 L2 L2 L1, a palindrome) one at a time: a rolled
   // loop keeps one body's reverse sweep live instead of four interleaved ones.
   for (int st = 0; st < ns; ++st) {

@@ -195,19 +195,57 @@ TP_DEV void esbend_body(St* s, const Cx* c, const double* __restrict__ p, int ns
 #endif
 #if TP_FAST
   if constexpr (KK == TP_KICK_NONE) {
-    // identity kicks: py and delta stay fixed through the step, so y and z take the step's
-    // summed path term once (ebody<false>)
-    const double Lsb = 2.0 * L1b + 2.0 * L2b;
-    for (int st = 0; st < nsteps; ++st) {
+    // Pure dipole (identity kicks): py, delta and hence d2 stay fixed, and each sub-body of
+    // exact_bend_body (src/tracking.jl:858-894) is a rotation of (px, pzmx), pzmx = pz -
+    // (1 + x irho), by the sub-step's angle: px' = px cs + pzmx sn, and from the body's own
+    // x' = (pzn - pzmx cs + px sn - 1)/irho, pzmx' = pzn - (1 + x' irho) = pzmx cs - px sn.
+    // Every sub-step is applied as that rotation (4 instructions instead of a square root,
+    // a series arcsine and the O(1) cancellation of x'); x is formed once, at the end, from
+    // pz - 1 = (dp1^2 - 1 - px^2 - py^2) / (pz + 1) without cancellation, and the path terms
+    // asin(px_k/d2) - asin(px_k+1/d2) of the sub-bodies telescope to one difference.
+    const double Ltot = (double)nsteps * (2.0 * L1 + 2.0 * L2);
+    const double Lbtot = (double)nsteps * (2.0 * L1b + 2.0 * L2b);
 #pragma unroll
-      for (int k = 0; k < PPL; ++k) {
-        double D = ebody<false>(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k], pz[k]);
-        D = D + ebody<false>(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k], pz[k]);
-        D = D + ebody<false>(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k], pz[k]);
-        D = D + ebody<false>(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k], pz[k]);
-        s[k].y = s[k].y + s[k].py * D;
-        s[k].z = s[k].z + __builtin_fma(c[k].B, D, -Lsb);
+    for (int k = 0; k < PPL; ++k) {
+      const double Bm1 = c[k].B - 1.0;                 // exact: dp1 - 1 of the reference's dp1
+      const double BBm1 = Bm1 * (c[k].B + 1.0);        // dp1^2 - 1
+      const double dd = __builtin_fma(-s[k].py, s[k].py, BBm1);
+      const double px0 = s[k].px, pz0 = pz[k];
+      double a = px0;
+      double b = __builtin_fma(-s[k].x, irho, __builtin_fma(-px0, px0, dd) * rcp_nr(pz0 + 1.0));
+      // The rotation keeps a^2 + b^2; below d2^2 every sub-body has |px| < d2, so none of the
+      // reference's clamps (pz at 0, arcsine arguments at +-1) acts and the rotations are its
+      // map.  A particle past that (a grazing angle) takes the reference's sub-bodies one by one.
+      if (__builtin_expect(!(__builtin_fma(a, a, b * b) < 0.999999 * (d2[k] * d2[k])), 0)) {
+        const double Lsb = 2.0 * L1b + 2.0 * L2b;
+        for (int st = 0; st < nsteps; ++st) {
+          double D = ebody<false>(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k], pz[k]);
+          D = D + ebody<false>(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k], pz[k]);
+          D = D + ebody<false>(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k], pz[k]);
+          D = D + ebody<false>(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k], pz[k]);
+          s[k].y = s[k].y + s[k].py * D;
+          s[k].z = s[k].z + __builtin_fma(c[k].B, D, -Lsb);
+        }
+        continue;
       }
+      for (int st = 0; st < nsteps; ++st) {
+#define TP_ROT(CS, SN)                                   \
+  {                                                      \
+    const double a_ = __builtin_fma(a, CS, b * SN);      \
+    b = __builtin_fma(b, CS, -(a * SN));                 \
+    a = a_;                                              \
+  }
+        TP_ROT(c1, s1) TP_ROT(c2, s2) TP_ROT(c2, s2) TP_ROT(c1, s1)
+#undef TP_ROT
+      }
+      const double pzn = pxyz(c[k], a, s[k].py);
+      const double pn1 = __builtin_fma(-a, a, dd) * rcp_nr(pzn + 1.0);
+      const double S = __builtin_fma(asin_diff(px0, pz0, a, pzn, d2[k], id2[k]), rho, Ltot);
+      s[k].x = (pn1 - b) * rho;
+      s[k].px = a;
+      s[k].y = __builtin_fma(s[k].py, S, s[k].y);
+      s[k].z = s[k].z + __builtin_fma(c[k].B, S, -Lbtot);
+      pz[k] = pzn;
     }
     return;
   }
