@@ -100,8 +100,8 @@ int tp_set_waves(tp_ctx* ctx, int fwd_waves, int back_waves);
 /* Drift form of the fast build (TP_TRACK_FAST) for later tracking calls on ctx (tp_track,
  * and the forward passes of tp_adjoint / tp_jacobian).  The fast build evaluates the exact
  * drift's L / sqrt(pz^2) (src/tracking.jl:87-99) in one of two forms that round
- * differently and are held to the same accuracy: TP_DRIFT_SERIES, a cubic series about
- * 1 / sqrt(1 + 2 delta beti + delta^2) where (px^2 + py^2) / that < 2^-14, lane by lane,
+ * differently and are held to the same accuracy: TP_DRIFT_SERIES, a quadratic series about
+ * 1 / sqrt(1 + 2 delta beti + delta^2) where (px^2 + py^2) / that < 2^-18, lane by lane,
  * and the refined reciprocal square root elsewhere; TP_DRIFT_RSQRT, the refined reciprocal
  * square root everywhere.  TP_DRIFT_AUTO (the default) takes the series when the particle
  * set -- its size N, not the number of survivors -- fills at least half of the GPU's

@@ -93,7 +93,7 @@ def test_dba_ring_fast_per_plane(ctx, form, nturns, tol):
 @pytest.mark.parametrize("form", ["series", "rsqrt"])
 def test_fast_drift_forms_across_angle_range(ctx, form):
     """Each drift form forced for the whole call: transverse momenta from 1e-7 to 0.3 --
-    dense around the series' bound (px^2 + py^2)/A = 2^-14, with waves that mix both sides --
+    dense around the series' bound (px^2 + py^2)/A = 2^-18, with waves that mix both sides --
     through drifts and thick multipoles, delta in [-0.05, 0.05]; one pass, tolerance 1e-12
     per plane under the rule above; a particle's bits do not depend on its wave-mates (a
     permuted beam gives the permuted result)."""
@@ -103,7 +103,7 @@ def test_fast_drift_forms_across_angle_range(ctx, form):
     rng = np.random.default_rng(17)
     n = 4096
     mag = np.concatenate([np.exp(rng.uniform(np.log(1e-7), np.log(0.3), n // 2)),
-                          2.0 ** -7 * (1.0 + rng.uniform(-0.02, 0.02, n // 2))])
+                          2.0 ** -9 * (1.0 + rng.uniform(-0.02, 0.02, n // 2))])
     ang = rng.uniform(0.0, 2.0 * np.pi, n)
     c0 = np.zeros((n, 6))
     c0[:, 0] = rng.normal(0.0, 1e-3, n)

@@ -150,44 +150,6 @@ def test_dba_ring(ctx, nturns, rtol):
     assert ok, (ef, eo)
 
 
-def test_fast_drift_across_angle_range(ctx):
-    """The fast build's drifts switch, lane by lane, from a cubic series about 1/sqrt(A) to
-    the refined reciprocal square root when (px^2 + py^2)/A reaches 2^-14 (|p| = 2^-7
-    on momentum).  Transverse momenta from 1e-7 to 0.3 -- dense around the switch, and
-    waves that mix both sides -- through drifts and thick multipoles (Yoshida steps), for
-    delta in [-0.05, 0.05]: accuracy parity with the reference path against the long-double
-    run of the same algorithm, and each particle's result independent of its wave-mates
-    (the same particles in another order give the same bits)."""
-    beam = Beam(W.ENERGY)
-    from trackpad_hip import Sextupole
-    lat = Lattice([Drift(0.7), Quadrupole(0.4, 1.2, num_int_steps=3), Drift(0.3), Sextupole(0.2, 30.0, num_int_steps=2),
-                   Drift(1.1), Quadrupole(0.4, -1.1, num_int_steps=3), Drift(0.5)])
-    rng = np.random.default_rng(17)
-    n = 4096
-    mag = np.concatenate([np.exp(rng.uniform(np.log(1e-7), np.log(0.3), n // 2)),
-                          2.0 ** -7 * (1.0 + rng.uniform(-0.02, 0.02, n // 2))])
-    ang = rng.uniform(0.0, 2.0 * np.pi, n)
-    c0 = np.zeros((n, 6))
-    c0[:, 0] = rng.normal(0.0, 1e-3, n)
-    c0[:, 2] = rng.normal(0.0, 1e-3, n)
-    c0[:, 4] = rng.normal(0.0, 1e-3, n)
-    c0[:, 5] = rng.uniform(-0.05, 0.05, n)
-    c0[:, 1] = mag * np.cos(ang) * (1.0 + c0[:, 5])
-    c0[:, 3] = mag * np.sin(ang) * (1.0 + c0[:, 5])
-    lf0 = np.zeros(n, np.int64)
-    oc, olf, _, _ = O.linepass(c0, lat, beam, lf0)
-    ld = O.linepass(c0, lat, beam, lf0, extended=True)[0]
-    fc, flf, _, _ = gpu_track(ctx, lat, beam, c0, 1, fast=True)
-    assert np.array_equal(flf, olf)
-    alive = olf == 0
-    assert alive.sum() > n // 2
-    ok, ef, eo = accuracy_ratio_ok(fc[alive], oc[alive], ld[alive])
-    assert ok, (ef, eo)
-    perm = rng.permutation(n)
-    pc, _, _, _ = gpu_track(ctx, lat, beam, c0[perm], 1, fast=True)
-    assert np.array_equal(pc, fc[perm], equal_nan=True)
-
-
 def test_forest_exact_bend_ring(ctx):
     """Reference ExactSBend (yrot, Forest fringe, wedge) with e1, e2 != 0 in a ring of
     quads: 5 turns, tolerance 1e-12."""

@@ -37,15 +37,15 @@ struct St {
 // pz2 = ((1 + (2 delta) beti) + delta^2) - px^2 - py^2 and of (beti + delta)
 // (src/tracking.jl:90, 98); BB = dp1^2 of pxyz (src/tracking.jl:718) with dp1 = beti+delta;
 // C = 1 + delta of the linearised drift (src/tracking.jl:102).
-// The fast build adds R = 1/sqrt(A) and IA = k/A, k = sqrt(3)/2, for the drift's series
-// form (inv_pz); the out-of-line maps take only the base (CxE), so the additions are not
-// passed to them.
+// The fast build adds R = 1/sqrt(A), H = 1/(2A), H3 = 3/(4A) and WB = 2^-18 A for the
+// drift's series form (inv_pz); the out-of-line maps take only the base (CxE), so the
+// additions are not passed to them.
 struct CxE {
   double A, B, BB, C;
 };
 struct Cx : CxE {
 #if TP_FAST
-  double R, IA;
+  double R, H, H3, WB;
 #endif
 };
 
@@ -57,7 +57,9 @@ TP_DEV void mkctx(const St& s, Cx& c, double beti) {
 #if TP_FAST
   // correctly rounded: these are formed once per change of delta, not per drift
   c.R = 1.0 / sqrt(c.A);
-  c.IA = 0.8660254037844386 / c.A;
+  c.H = 0.5 / c.A;
+  c.H3 = 1.5 * c.H;
+  c.WB = c.A * 0x1p-18;  // exact
 #endif
 }
 
@@ -73,26 +75,24 @@ TP_DEV double rsqrt_nr(double v) {
   const double e = __builtin_fma(-t, y, 1.0);
   return __builtin_fma(y * e, __builtin_fma(e, 0.375, 0.5), y);
 }
-// L / pz of the exact-Hamiltonian drift, pz = sqrt(A - px^2 - py^2), A = (1 + delta)^2 form
-// hoisted in Cx.  With w = (px^2 + py^2) / A,
-//   L / pz = (L / sqrt(A)) (1 - w)^(-1/2) = (L R) (1 + w/2 + 3w^2/8 + 5w^3/16 + ...),
-// and for w < 2^-14 the terms after w^3 sum to below 35/128 w^4 / (1 - w) < 2^-57 relative,
-// so the cubic gives L / pz within about two ulp from full-rate FMAs only (the hardware
-// reciprocal square root of rsqrt_nr issues at a quarter of the FMA rate).  The cubic runs
-// on v = k w, k = sqrt(3)/2 (IA = k/A), which makes the w^2 coefficient the inline constant
-// 1/2:
-//   L / pz = L R + v (Q1 L R + L R v (1/2 + Q3 v)),  Q1 = 1/(2k),  Q3 = 5/(16 k^3),
-// so the one remaining constant sits in a scalar register and L R, Q1 L R are per element
-// and particle (hoisted out of a magnet's steps); the coefficients' roundings weigh in with
-// a factor v < 2^-14.  Larger angles (and NaN, pz2 <= 0, which has w >= 1) take rsqrt_nr
-// lane by lane, so each lane's result depends only on its own coordinates.
+// L / pz of the exact-Hamiltonian drift, pz = sqrt(A - w), w = px^2 + py^2, A = the
+// (1 + delta)^2 form hoisted in Cx.  With R = 1/sqrt(A),
+//   L / pz = L R (1 - w/A)^(-1/2) = L R (1 + (1/2)(w/A) + (3/8)(w/A)^2 + (5/16)(w/A)^3 + ...),
+// and for w < 2^-18 A the terms after (w/A)^2 sum to below (5/16) 2^-54 / (1 - 2^-18) <
+// 2^-55.6 relative, a quarter of an ulp, so the quadratic
+//   L / pz = Lr + w (c1 + w c2),  Lr = L R,  c1 = Lr H = L R / (2A),  c2 = c1 H3 = 3 L R / (8 A^2)
+// gives L / pz within about two ulp from two FMAs; Lr, c1, c2 are per element and
+// particle (hoisted out of a magnet's steps), and their roundings weigh in with a factor
+// w/A < 2^-18.  |p| < 2^-9 rad covers the beams of the ring workloads (|p| ~ 1e-4); larger
+// angles (and NaN, pz2 <= 0, where w >= A) take rsqrt_nr lane by lane, so each lane's
+// result depends only on its own coordinates.
 TP_DEV double inv_pz(const St& s, const Cx& c, double L) {
-  const double v = (s.px * s.px + s.py * s.py) * c.IA;
+  const double w = __builtin_fma(s.px, s.px, s.py * s.py);
   const double Lr = L * c.R;
-  const double t = __builtin_fma(v, 0.4811252243246881, 0.5);                       // Q3, 1/2
-  double NormL = __builtin_fma(v, __builtin_fma(Lr * v, t, Lr * 0.5773502691896257), Lr);  // Q1
-  // bound k 2^-14 rounded down to 20 fraction bits (a 32-bit literal of the compare)
-  if (__builtin_expect(!(v < 0x1.bb67ap-15), 0)) NormL = L * rsqrt_nr(c.A - s.px * s.px - s.py * s.py);
+  const double c1 = Lr * c.H;
+  const double c2 = c1 * c.H3;
+  double NormL = __builtin_fma(w, __builtin_fma(w, c2, c1), Lr);
+  if (__builtin_expect(!(w < c.WB), 0)) NormL = L * rsqrt_nr(c.A - w);
   return NormL;
 }
 // sqrt(v) = v / sqrt(v) = t (1 - e)^(-1/2) with t = v y, from the same estimate
