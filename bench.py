@@ -71,15 +71,22 @@ HBM_PEAK = 8.0e12
 
 # HBM bytes per particle of one tracking-kernel launch, measured with rocprofv3 --pmc on the
 # 1e6-particle DBA launch of 100 turns, per register budget of the kernel (waves/SIMD; 0 = the
-# compiler's, 4 waves): FETCH_SIZE x 1.998 (the counter reads 23,457 KiB for a plain 48 MB
-# device copy) + WRITE_SIZE x 1.00 (exact on the same copy), per 1e6 particles.  Budget 0:
-# 25,578.8 + 47,491.3 KiB (616 KiB above the 46,875 KiB of coordinates are the tile queue's
-# atomics), independent of the turns per launch since the state stays in registers in
-# between; budgets 5 and 6 add the part of their register spills (32 and 96 B per lane) that
-# leaves the L2 over those 100 turns.
-MEASURED_BYTES_PER_PARTICLE_LAUNCH = {("dba", 0): (25578.8 * 1.998 + 47491.3) * 1024 / 1e6,
-                                      ("dba", 5): (27054.6 * 1.998 + 57852.3) * 1024 / 1e6,
-                                      ("dba", 6): (33381.9 * 1.998 + 104208.7) * 1024 / 1e6}
+# compiler's, 4 waves): FETCH_SIZE x 46,875 / 23,456.4 (the counter reads 23,456.4 KiB for a
+# plain 48 MB device copy in the same run) + WRITE_SIZE x 1.00 (exact on the same copy), per
+# 1e6 particles.  Budget 0: 25,561.9 + 47,491.3 KiB (616 KiB above the 46,875 KiB of
+# coordinates are the tile queue's atomics), independent of the turns per launch since the
+# state stays in registers in between; budgets 5 and 6 add the part of their register spills
+# (32 and 96 B per lane) that leaves the L2 over those 100 turns.
+_FETCH_SCALE = 46875.0 / 23456.4
+MEASURED_BYTES_PER_PARTICLE_LAUNCH = {("dba", 0): (25561.9 * _FETCH_SCALE + 47491.3) * 1024 / 1e6,
+                                      ("dba", 5): (27147.3 * _FETCH_SCALE + 57933.4) * 1024 / 1e6,
+                                      ("dba", 6): (27761.1 * _FETCH_SCALE + 109751.8) * 1024 / 1e6}
+
+# FP64 flops the fast DBA kernel executes per particle-turn, from rocprofv3 PMC on the same
+# launch (2 SQ_INSTS_VALU_FMA_F64 + SQ_INSTS_VALU_MUL_F64 + SQ_INSTS_VALU_ADD_F64 per 64-particle
+# tile-turn): fewer than the reference's 313,587 because the fast build does not repeat work
+# (pure-dipole bend sub-bodies as rotations, drifts as a two-FMA series).
+MEASURED_EXECUTED_FLOPS_PER_PARTICLE_TURN = {("dba", "fast"): 177_215}
 
 
 def visible_cores() -> int:
@@ -651,8 +658,16 @@ def main():
         "roofline": {"bound": BOUND, "pipe": PIPE,
                      "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": traffic,
-                     "avg_launch_ms": avg_launch_s * 1e3, "launches": launches},
+                     "avg_launch_ms": avg_launch_s * 1e3, "launches": launches,
+                     "flops": "algorithmic: the reference's FP64 operations per particle-turn, counted as written"},
     }
+    xf = MEASURED_EXECUTED_FLOPS_PER_PARTICLE_TURN.get((args.workload, args.kernel))
+    if xf is not None:
+        ex = xf * live_per_launch / avg_launch_s
+        out["roofline"]["executed"] = {
+            "flops_per_particle_turn": xf, "TFLOPs": ex / 1e12, "frac": ex / FP64_PEAK,
+            "note": "FP64 flops the kernel executes (rocprofv3 PMC per tile-turn, a constant of this kernel) "
+                    "at this run's rate: fewer than the reference's because the fast build does not repeat work"}
     if args.streamed_turns > 0:
         out["streamed"] = streamed(ctx, tab, init, work, args, fast, fpt, n, n_seq)
     if not args.no_cpu_baseline:
