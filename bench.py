@@ -34,7 +34,7 @@ the variants' full turn counts: --compare-variants variants spread over the beam
 timed both as one batch and as plain launches, and the 256-variant per-variant time
 extrapolated from them (time per turn interpolated in log N).
 
-Workload ``da``: the dynamic-aperture scan -- the 2,000-element ring (80 DBA cells, every
+Workload ``da``: the dynamic-aperture scan -- the 2,000-magnet ring (80 DBA cells + RF, every
 magnet with order-10 normal and skew errors) on the 1024 x 1024 x 3 grid (3,145,728
 particles); one step is --turns turns (default 20: the full 10,000-turn scan is ~45 minutes
 of GPU time, and the rate is independent of the turn count).

@@ -40,66 +40,6 @@ int fail(int code, const char* fmt, ...) {
 }
 
 
-// Length in doubles of a record's parameter block (trackpad_table.h).
-int64_t block_len(int tag, int m, int nsteps) {
-  const int64_t poly = 2 * (int64_t)(m + 1);
-  switch (tag) {
-    case TP_MARKER: return 0;
-    case TP_DRIFT: return 2;
-    case TP_MPOLE: return 6 + poly;
-    case TP_THINMP: return poly;
-    case TP_SBEND: return 17 + poly;
-    case TP_ESBEND:
-    case TP_ESBEND_LE: return 24 + poly;
-    case TP_RFCAV: return 7;
-    case TP_SOLENOID: return 2;
-    case TP_CORRECTOR: return 6;
-    case TP_TRANSLATION: return 3;
-    case TP_YROTATION: return 3;
-    case TP_CRAB: return 6;
-    case TP_ACCEL: return 4;
-    case TP_LONGRF: return 1;
-    case TP_BOOST:
-    case TP_INVBOOST: return 2;
-    case TP_GAUSSBB: return 3 + 3 * (int64_t)nsteps;
-    case TP_RLCWAKE: return 6;
-    case TP_TABWAKE: return 3 + 2 * (int64_t)nsteps;
-    case TP_LBEND: return 13;
-    case TP_NAN: return 0;
-    case TP_WIGGLER: return -2;  // length depends on the block's harmonic counts
-    default: return -1;
-  }
-}
-
-int64_t poly_off(int tag) {
-  switch (tag) {
-    case TP_MPOLE: return 6;
-    case TP_THINMP: return 0;
-    case TP_SBEND: return 17;
-    case TP_ESBEND:
-    case TP_ESBEND_LE: return 24;
-    default: return -1;
-  }
-}
-
-// Kick specialisation from the structural zeros of the polynomial (indices <= m).
-int classify_kick(const double* poly, int m) {
-  bool nz_b1 = false, nz_b2 = false, nz_other = false;
-  for (int i = 0; i <= m; ++i) {
-    const double b = poly[2 * i], a = poly[2 * i + 1];
-    if (a != 0.0) nz_other = true;
-    if (b != 0.0) {
-      if (i == 1) nz_b1 = true;
-      else if (i == 2) nz_b2 = true;
-      else nz_other = true;
-    }
-  }
-  if (nz_other || (nz_b1 && nz_b2)) return TP_KICK_GEN;
-  if (nz_b1) return TP_KICK_QUAD;
-  if (nz_b2) return TP_KICK_SEXT;
-  return TP_KICK_NONE;
-}
-
 uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
   const unsigned char* c = static_cast<const unsigned char*>(p);
   for (size_t i = 0; i < n; ++i) {
@@ -260,96 +200,11 @@ int tp_timing(tp_ctx* ctx, tp_timing_t* out) {
 
 }  // extern "C"
 
-namespace {
-
-// Parse and validate a table blob into t (host side); fills the kick specialisation and
-// the feature mask.  Returns TP_OK or a TP_E* code with the message set.
-int parse_table(const void* bytes, size_t nbytes, tp_table* t) {
-  if (!bytes) return fail(TP_EINVAL, "table: bytes is NULL");
-  if (nbytes < sizeof(tp_table_header)) return fail(TP_ETABLE, "table: %zu bytes is shorter than the header", nbytes);
-  tp_table_header h;
-  std::memcpy(&h, bytes, sizeof h);
-  if (h.magic != TP_TABLE_MAGIC) return fail(TP_ETABLE, "table: bad magic 0x%08x", h.magic);
-  if (h.version != TP_TABLE_VERSION) return fail(TP_ETABLE, "table: version %u, library reads %u", h.version, TP_TABLE_VERSION);
-  if (h.n_rows > 1000000u) return fail(TP_ETABLE, "table: %u sequence rows", h.n_rows);
-  const size_t rows = h.n_rows > 1 ? h.n_rows : 1;
-  if (rows > 1 && (int64_t)h.row_turn0 + (int64_t)rows > 0x7fffffffLL)
-    return fail(TP_ETABLE, "table: sequence rows overflow the turn range");
-  const size_t n_pos = rows * (size_t)h.n_seq;
-  const size_t seq_bytes = (n_pos * 4 + 7) / 8 * 8;
-  const size_t need = sizeof h + seq_bytes + (size_t)h.n_rec * sizeof(tp_rec) + (size_t)h.pool_len * 8;
-  if (need != nbytes) return fail(TP_ETABLE, "table: size %zu, header implies %zu", nbytes, need);
-  if (h.n_seq > 0 && h.n_rec == 0) return fail(TP_ETABLE, "table: %u positions but no records", h.n_seq);
-  if (!(std::isfinite(h.beti) && h.beti >= 1.0)) return fail(TP_ETABLE, "table: beti %g is not >= 1", h.beti);
-  if (!(h.coord_limit > 0.0) || !(h.angle_limit > 0.0) || !(h.aperture_r2 >= 0.0))
-    return fail(TP_ETABLE, "table: bad loss limits");
-  t->hdr = h;
-  const char* b = (const char*)bytes;
-  t->seq.resize(n_pos);
-  std::memcpy(t->seq.data(), b + sizeof h, n_pos * 4);
-  t->recs.resize(h.n_rec);
-  std::memcpy(t->recs.data(), b + sizeof h + seq_bytes, (size_t)h.n_rec * sizeof(tp_rec));
-  t->pool.resize(h.pool_len);
-  std::memcpy(t->pool.data(), b + sizeof h + seq_bytes + (size_t)h.n_rec * sizeof(tp_rec), (size_t)h.pool_len * 8);
-  for (size_t i = 0; i < n_pos; ++i)
-    if (t->seq[i] < 0 || (uint32_t)t->seq[i] >= h.n_rec)
-      return fail(TP_ETABLE, "table: seq[%zu] = %d out of range", i, t->seq[i]);
-  unsigned feat = 0;
-  if (!(h.flags & TP_H_EXACT_HAMILTONIAN)) feat |= FT_LINH;
-  for (uint32_t i = 0; i < h.n_rec; ++i) {
-    tp_rec& r = t->recs[i];
-    if (r.tag < 0 || r.tag >= TP_NTAGS) return fail(TP_EUNSUPPORTED, "table: record %u: unknown tag %d", i, r.tag);
-    if (r.max_order < 0 || r.max_order > TP_MAXORD)
-      return fail(TP_EUNSUPPORTED, "table: record %u: max_order %d outside [0, 10]", i, r.max_order);
-    if (r.nsteps < 0 || r.nsteps > 1000000) return fail(TP_ETABLE, "table: record %u: nsteps %d", i, r.nsteps);
-    if (r.tag == TP_GAUSSBB && r.nsteps < 1) return fail(TP_ETABLE, "table: record %u: beam-beam without slices", i);
-    if (r.tag == TP_TABWAKE && r.nsteps < 2) return fail(TP_ETABLE, "table: record %u: wake table shorter than 2", i);
-    int64_t len = block_len(r.tag, r.max_order, r.nsteps);
-    if (len == -2) {  // TP_WIGGLER: 9 + 11 per harmonic
-      if (r.off < 0 || (int64_t)r.off + 9 > (int64_t)h.pool_len)
-        return fail(TP_ETABLE, "table: record %u: parameter block outside the pool", i);
-      const double nh = t->pool[(size_t)r.off + 3], nv = t->pool[(size_t)r.off + 4];
-      if (!(nh >= 0.0 && nv >= 0.0 && nh + nv <= 64.0) || r.nsteps < 0)
-        return fail(TP_ETABLE, "table: record %u: wiggler with %g + %g harmonics", i, nh, nv);
-      len = 9 + 11 * ((int64_t)nh + (int64_t)nv);
-    }
-    if (r.off < 0 || (int64_t)r.off + len > (int64_t)h.pool_len)
-      return fail(TP_ETABLE, "table: record %u: parameter block outside the pool", i);
-    if (r.flags & TP_F_MISALIGN) {
-      if (r.tag == TP_MARKER || r.tag == TP_RFCAV ||
-          (r.tag >= TP_TRANSLATION && r.tag != TP_LBEND && r.tag != TP_WIGGLER))
-        return fail(TP_ETABLE, "table: record %u: misalignment on an element without one", i);
-      if (r.moff < 0 || (int64_t)r.moff + 84 > (int64_t)h.pool_len)
-        return fail(TP_ETABLE, "table: record %u: misalignment block outside the pool", i);
-      feat |= FT_MISALIGN;
-    }
-    if (r.gorder < -1 || r.gorder > r.max_order) return fail(TP_ETABLE, "table: record %u: gorder %d", i, r.gorder);
-    r.kick = TP_KICK_NONE;
-    const int64_t po = poly_off(r.tag);
-    if (po >= 0) {
-      r.kick = classify_kick(t->pool.data() + r.off + po, r.max_order);
-      if (r.kick == TP_KICK_GEN) feat |= FT_GENKICK;
-    }
-    if (r.tag == TP_SBEND) feat |= FT_SBEND;
-    if (r.tag == TP_ESBEND) feat |= FT_ESBEND;
-    if (r.tag == TP_ESBEND_LE) feat |= FT_ESBEND_LE;
-    if (r.tag == TP_RFCAV) feat |= FT_RF;
-    if (r.tag >= TP_SOLENOID) feat |= FT_EXTRA;
-  }
-  t->features = feat;
-  if ((feat & ~TP_FEAT_A) == 0) t->variant = TP_VARIANT_A;
-  else if ((feat & ~TP_FEAT_B) == 0) t->variant = TP_VARIANT_B;
-  else if ((feat & ~TP_FEAT_C) == 0) t->variant = TP_VARIANT_C;
-  else t->variant = TP_VARIANT_ALL;
-  return TP_OK;
-}
-
-}  // namespace
 
 extern "C" {
 
 int tp_table_check(const void* bytes, size_t nbytes, uint32_t* features) {
-  tp_table t;
+  tp_table_host t;
   int rc = parse_table(bytes, nbytes, &t);
   if (rc) return rc;
   if (features) *features = t.features;

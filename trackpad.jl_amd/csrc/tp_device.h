@@ -6,39 +6,10 @@
 #include <hip/hip_runtime.h>
 
 #include "trackpad_table.h"
+#include "tp_table_parse.h"
 
-// Kick specialisations, chosen per record at table load from the structural zeros of
-// the polynomial.  Every specialisation evaluates the same products and sums as the
-// full Horner recurrence of src/tracking.jl:290-298 minus the terms that are a product
-// with an exact zero, so finite nonzero results are bit-identical; only the sign of an
-// exactly-zero result can differ.
-enum tp_kick {
-  TP_KICK_NONE = 0,  // all coefficients zero: the kick is the identity
-  TP_KICK_QUAD = 1,  // m = 1, only b1 nonzero
-  TP_KICK_SEXT = 2,  // m = 2, only b2 nonzero
-  TP_KICK_GEN = 3    // anything else: full Horner over max_order
-};
-
-// Kernel feature bits: a kernel variant contains only the element maps its mask names.
-enum : unsigned {
-  FT_SBEND = 1u,      // expanded-Hamiltonian bends (TP_SBEND)
-  FT_ESBEND = 2u,     // Forest exact bends with yrot / fringe / wedge (TP_ESBEND)
-  FT_ESBEND_LE = 4u,  // exact bend body with linear edges (TP_ESBEND_LE)
-  FT_RF = 8u,         // RF cavities
-  FT_MISALIGN = 16u,  // t1/t2/r1/r2 misalignments
-  FT_GENKICK = 32u,   // general Horner kicks up to order 10
-  FT_LINH = 64u,      // linearised drift (USE_EXACT_HAMILTONIAN = false)
-  FT_EXTRA = 128u     // the other element families (tags TP_SOLENOID .. TP_NAN, tp_extra.h)
-};
-// Compiled variants, smallest first; the host picks the first whose mask covers the table.
-#define TP_FEAT_A 0u                                              // drift/multipole lattices
-#define TP_FEAT_B (FT_ESBEND_LE | FT_RF)                          // rings of exact bends + RF
-#define TP_FEAT_C (FT_ESBEND_LE | FT_RF | FT_SBEND | FT_GENKICK)  // + high-order multipoles
-#define TP_FEAT_ALL 255u
-// The reverse kernels' largest variant: every family (the host refuses the wiggler); it is
-// selected for Forest exact bends, r1/r2 rotations, the linearised drift and tags 8-21.
-#define TP_FEAT_ADJ_ALL TP_FEAT_ALL
-enum { TP_VARIANT_A = 0, TP_VARIANT_B = 1, TP_VARIANT_C = 2, TP_VARIANT_ALL = 3 };
+// tp_kick, the FT_* feature bits, TP_FEAT_* and TP_VARIANT_* live beside the host-side table
+// parser that assigns them.
 
 struct TpKArgs {
   const int32_t* seq;

@@ -33,19 +33,14 @@ struct tp_ctx {
   tp_timing_t timing{};
 };
 
-struct tp_table {
+// The parsed host copy (hdr, features, variant, recs, seq, pool) and the device copy.
+struct tp_table : tpr::tp_table_host {
   tp_ctx* ctx = nullptr;
   void* dev = nullptr;
   size_t nbytes = 0;
-  tp_table_header hdr{};
   const int32_t* d_seq = nullptr;
   const tp_rec* d_recs = nullptr;
   const double* d_pool = nullptr;
-  unsigned features = 0;
-  int variant = TP_VARIANT_ALL;
-  std::vector<tp_rec> recs;      // host copy with the kick specialisation filled in
-  std::vector<int32_t> seq;      // host copy (n_rows x n_seq)
-  std::vector<double> pool;      // host copy
   int32_t* d_slots = nullptr;    // gradient slot of each position (adjoint), built on first use
   int32_t n_slots = 0;
   double* d_minv = nullptr;      // r1/r2 inverses at the matrices' pool offsets (adjoint), on first use
@@ -116,8 +111,6 @@ namespace tpr {
 // call, and every call on the set, takes the same form.
 int drift_form(const tp_ctx* ctx, int variant, int exact_h, int64_t n, bool fast);
 
-// Sets the thread's error message (tp_last_error) and returns code.
-int fail(int code, const char* fmt, ...);
 int set_dev(tp_ctx* ctx);
 void ctx_release(tp_ctx* ctx);
 void particles_release(tp_particles* p);

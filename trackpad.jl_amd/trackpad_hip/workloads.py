@@ -7,7 +7,7 @@ that the oracle, the tests and the benchmark all track the same inputs:
                   10,000 particles, 1,000 turns, R = 0.02 m aperture, E = 3 GeV.
 * ``dba``      -- 20 DBA cells of 25 elements (2 exact sector bends L = 1.5 m, angle 2pi/40,
                   with linear edge/fringe maps; 6 thick quads |k1| in [0.5, 2.4]; 4 thick
-                  sextupoles |k2| in {20, 30}; 13 drifts) plus one RF cavity per turn
+                  sextupoles, k2 = -32 (SD) and +27 (SF); 13 drifts) plus one RF cavity per turn
                   (V = 1 MV, h = 400, E = 3 GeV), 1e6 particles, 1,000 turns.
 * ``da``       -- 80 DBA cells (bend angle 2pi/160), every magnet with random normal and
                   skew multipole errors up to order 10, a 1024 x 1024 (x, y) grid over
@@ -120,13 +120,13 @@ def dba_particles(n: int = 1_000_000, seed: int = 2, beta_xy: Tuple[float, float
 
 
 # Periodic beta functions at the start of the 20-cell DBA ring (symmetry point), from the
-# oracle's one-turn Jacobian (tests/test_workloads.py recomputes and checks them).
+# oracle's one-turn Jacobian at the origin (tests/test_workloads.py recomputes them).
 DBA_BETA = (2.922440729213387, 6.123935623898734)
 
 
 def da_ring(ncell: int = 80, seed: int = 7, err: float = 1e-4, r_ref: float = 0.01,
             beam: Optional[Beam] = None, steps: int = 10) -> Lattice:
-    """2,000-element DA ring: 80 DBA cells (bend angle 2pi/160) + RF, every magnet with
+    """2,001-element DA ring: 80 DBA cells (bend angle 2pi/160) + RF, every magnet with
     normal and skew errors b_n, a_n = N(0, err) x (main field at r_ref) / r_ref^n for
     n = 0..10.  The main-order normal error is folded into the magnet's k (the reference
     overwrites that polynomial slot with k, src/tracking.jl:395, 446)."""
