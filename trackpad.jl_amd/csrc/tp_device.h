@@ -11,6 +11,10 @@
 // tp_kick, the FT_* feature bits, TP_FEAT_* and TP_VARIANT_* live beside the host-side table
 // parser that assigns them.
 
+// Dynamic LDS of a tracking launch per wave of its block (tp_track_kernels.hip: the edge memo
+// of each particle a lane holds).
+#define TP_LDS_PER_WAVE(ppl) ((size_t)(ppl) * 128 * 8)
+
 struct TpKArgs {
   const int32_t* seq;
   const tp_rec* recs;

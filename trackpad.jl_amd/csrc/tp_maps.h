@@ -312,16 +312,57 @@ TP_DEV void bkick(St& s, const double* __restrict__ poly, int m, double K, doubl
 // ---------------------------------------------------------------------------------------
 // Linear dipole edge, src/tracking.jl:160-221.  blk = [fx, fy_const, e, fc].
 // ---------------------------------------------------------------------------------------
-TP_DEV void edge_lin(St& s, int method, const double* __restrict__ blk, double irho, bool is_exit) {
+// Memo of the fringe term tan(e - fc / (1 + delta)) of edge methods 1 and 2: the bends of a
+// ring usually share e and fc, and delta changes at few elements per turn, so each particle
+// keeps its last value with the delta it was formed from (in LDS, one slot per lane, the
+// lane's own), and the wave keeps the bit patterns of the (e, fc) the slots belong to.  A
+// hit returns the bits the call would return: the same function of the same arguments.
+// Every lane of the wave passes every edge (dead lanes too), so the slots of all lanes
+// always belong to the wave's key.
+struct EdgeMemo {
+  uint64_t ke, kf;         // bits of e and fc (wave-uniform)
+  double* __restrict__ w;  // the wave's slots for this particle (wave-uniform address): w[lane]
+                           // delta of the lane's value (NaN: empty), w[64 + lane] the value
+};
+
+TP_DEV int lane_id() { return (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+// The same, formed where it is used: the compiler cannot hoist it out of the element loop,
+// where a lane index held across the loop costs a register the hot paths need.
+TP_DEV int lane_id_here() {
+  int l;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+  return l;
+}
+
+TP_DEV double edge_tan(EdgeMemo& mm, const St& s, double e, double fc) {
+  const uint64_t be = __builtin_bit_cast(uint64_t, e), bf = __builtin_bit_cast(uint64_t, fc);
+  const bool same = be == mm.ke && bf == mm.kf;
+  mm.ke = be;
+  mm.kf = bf;
+  double* __restrict__ slot = mm.w + lane_id_here();
+  double t;
+  if (same && slot[0] == s.d) {
+    t = slot[64];
+  } else {
+    t = tan_lib(e - fc / (1.0 + s.d));
+    slot[0] = s.d;
+    slot[64] = t;
+  }
+  return t;
+}
+
+template <bool MEMO = false>
+TP_DEV void edge_lin(St& s, int method, const double* __restrict__ blk, double irho, bool is_exit,
+                     EdgeMemo* mm = nullptr) {
   const double fx = blk[0], fyc = blk[1], e = blk[2], fc = blk[3];
   double fy;
   if (method == 2) {
-    double t = (fc == 0.0) ? fyc : irho * tan_lib(e - fc / (1.0 + s.d));
+    double t = (fc == 0.0) ? fyc : irho * (MEMO ? edge_tan(*mm, s, e, fc) : tan_lib(e - fc / (1.0 + s.d)));
     fy = t / (1.0 + s.d);
   } else if (method == 3) {
     fy = is_exit ? irho * tan_lib(e - fc - s.px / (1.0 + s.d)) : irho * tan_lib(e - fc + s.px / (1.0 + s.d));
   } else {  // Brown (1) and the fallback
-    fy = (fc == 0.0) ? fyc : irho * tan_lib(e - fc / (1.0 + s.d));
+    fy = (fc == 0.0) ? fyc : irho * (MEMO ? edge_tan(*mm, s, e, fc) : tan_lib(e - fc / (1.0 + s.d)));
   }
   const double px_new = s.px + s.x * fx;
   s.py = s.py - s.y * fy;

@@ -375,7 +375,7 @@ TP_DEV void esbend_dispatch(St* s, const Cx* c, const double* __restrict__ p, in
 // a SIMD; it holds two more values per particle and two per element, so the variants with
 // general kicks (up to 22 coefficients in registers) use the reciprocal square root always.
 template <bool EXH, unsigned FEAT, int PPL, bool SER_OK>
-TP_DEV void apply_elem(St* s, Cx* c, const tp_rec& rec, const double* __restrict__ pool, double beti) {
+TP_DEV void apply_elem(St* s, Cx* c, EdgeMemo* mm, const tp_rec& rec, const double* __restrict__ pool, double beti) {
   constexpr bool SER = SER_OK && (FEAT & FT_GENKICK) == 0;
   const double* __restrict__ p = pool + rec.off;
   const uint32_t fl = rec.flags;
@@ -416,14 +416,14 @@ TP_DEV void apply_elem(St* s, Cx* c, const tp_rec& rec, const double* __restrict
         const double irho = p[6];
         if (fin != 0) {
 #pragma unroll
-          for (int k = 0; k < PPL; ++k) edge_lin(s[k], fin, p + 9, irho, false);
+          for (int k = 0; k < PPL; ++k) edge_lin<true>(s[k], fin, p + 9, irho, false, &mm[k]);
         }
 #define TP_CALL(KK) sbend_steps<EXH, KK, PPL, SER>(s, c, p, ns, m, beti)
         TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
         if (fout != 0) {
 #pragma unroll
-          for (int k = 0; k < PPL; ++k) edge_lin(s[k], fout, p + 13, irho, true);
+          for (int k = 0; k < PPL; ++k) edge_lin<true>(s[k], fout, p + 13, irho, true, &mm[k]);
         }
       }
       break;
@@ -468,14 +468,14 @@ TP_DEV void apply_elem(St* s, Cx* c, const tp_rec& rec, const double* __restrict
         const double irho = p[0];
         if (fin != 0) {
 #pragma unroll
-          for (int k = 0; k < PPL; ++k) edge_lin(s[k], fin, p + 11, irho, false);
+          for (int k = 0; k < PPL; ++k) edge_lin<true>(s[k], fin, p + 11, irho, false, &mm[k]);
         }
 #define TP_CALL(KK) esbend_dispatch<KK, PPL>(s, c, p, ns, m, dl)
         TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
         if (fout != 0) {
 #pragma unroll
-          for (int k = 0; k < PPL; ++k) edge_lin(s[k], fout, p + 15, irho, true);
+          for (int k = 0; k < PPL; ++k) edge_lin<true>(s[k], fout, p + 15, irho, true, &mm[k]);
         }
       }
       break;
@@ -562,7 +562,7 @@ TP_DEV void store_state(double* q, int64_t ld, const St& s) {
 // after an element has its entry state, turn (a.turn0 + t) and position written out and
 // is dead from then on; the wave leaves early once every particle in it is dead.
 template <bool EXH, unsigned FEAT, int PPL, bool SER_OK>
-TP_DEV void run_turns(St* s, Cx* c, bool* alive, const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
+TP_DEV void run_turns(St* s, Cx* c, EdgeMemo* mm, bool* alive, const int32_t* __restrict__ seq, const tp_rec* __restrict__ recs,
                       const double* __restrict__ pool, double* __restrict__ coords, int32_t* __restrict__ lost_turn,
                       int32_t* __restrict__ lost_elem, const TpKArgs& a, const int64_t* idx, int32_t t_begin,
                       int32_t t_end) {
@@ -590,7 +590,7 @@ TP_DEV void run_turns(St* s, Cx* c, bool* alive, const int32_t* __restrict__ seq
       St s0[PPL];
 #pragma unroll
       for (int k = 0; k < PPL; ++k) s0[k] = s[k];
-      apply_elem<EXH, FEAT, PPL, SER_OK>(s, c, rec, pool, a.beti);
+      apply_elem<EXH, FEAT, PPL, SER_OK>(s, c, mm, rec, pool, a.beti);
       bool lostnow[PPL];
       bool anylost = false;
 #pragma unroll
@@ -606,10 +606,33 @@ TP_DEV void run_turns(St* s, Cx* c, bool* alive, const int32_t* __restrict__ seq
             lost_turn[idx[k]] = a.turn0 + t;
             lost_elem[idx[k]] = e;
             alive[k] = false;
+            // the dead lane goes on as the reference particle: no NaN left in it to send
+            // its wave through the slow paths that only NaN or large angles take
+            s[k] = St{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            mkctx(s[k], c[k], a.beti);
           }
         }
       }
     }
+  }
+}
+
+// Per-wave LDS of the tracking kernels (dynamic, sized by the launch: TP_LDS_PER_WAVE(ppl)
+// bytes per wave of the block): for each particle per lane k, the edge memo's 128 doubles
+// (64 deltas, 64 values).  Each lane uses only its own slots, so no barrier is needed.
+TP_DEV double* wave_lds(double* lds, int ppl) {
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x) >> 6;
+  return lds + (size_t)wave * ppl * (TP_LDS_PER_WAVE(1) / 8);
+}
+
+template <unsigned FEAT, int PPL>
+TP_DEV void edge_memo_init(EdgeMemo* mm, double* wl) {
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) {
+    mm[k].ke = 0;
+    mm[k].kf = 0;
+    mm[k].w = wl + 128 * k;
+    if constexpr ((FEAT & (FT_SBEND | FT_ESBEND_LE)) != 0) mm[k].w[lane_id()] = __builtin_nan("");
   }
 }
 
@@ -630,6 +653,10 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void track_kernel(const int32_t*
   const bool persist = a.counter != nullptr;
   const int lane = threadIdx.x & 63;
   const int64_t n_tiles = (a.n + 64 * PPL - 1) / (64 * PPL);
+  extern __shared__ double tp_lds[];
+  double* wl = wave_lds(tp_lds, PPL);
+  EdgeMemo mm[PPL];
+  edge_memo_init<FEAT, PPL>(mm, wl);
   for (;;) {
     int64_t i0, stride;
     if (persist) {
@@ -655,7 +682,7 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void track_kernel(const int32_t*
     Cx c[PPL];
     bool alive[PPL];
     load_lanes<PPL>(s, c, alive, coords, lost_turn, a.ld, a.beti, idx, valid);
-    run_turns<EXH, FEAT, PPL, SER>(s, c, alive, seq, recs, pool, coords, lost_turn, lost_elem, a, idx, 0, a.nturns);
+    run_turns<EXH, FEAT, PPL, SER>(s, c, mm, alive, seq, recs, pool, coords, lost_turn, lost_elem, a, idx, 0, a.nturns);
 #pragma unroll
     for (int k = 0; k < PPL; ++k)
       if (alive[k]) store_state(coords + idx[k], a.ld, s[k]);
@@ -677,6 +704,10 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_kernel(const TpVarDes
                                                     const double* __restrict__ pool, double* __restrict__ ckpt,
                                                     double* __restrict__ item_loss) {
   const int lane = threadIdx.x & 63;
+  extern __shared__ double tp_lds[];
+  double* wl = wave_lds(tp_lds, 1);
+  EdgeMemo mm[1];
+  edge_memo_init<FEAT, 1>(mm, wl);
   for (;;) {
     int32_t it = 0;
     if (lane == 0) it = atomicAdd(counter, 1);
@@ -710,7 +741,7 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_kernel(const TpVarDes
       for (int32_t j = 0; j < d.nseg; ++j) {
         const int32_t tb = j * d.ckpt_stride;
         const int32_t te = (j == d.nseg - 1) ? d.nturns : tb + d.ckpt_stride;
-        run_turns<EXH, FEAT, 1, true>(s, c, alive, vseq, vrecs, vpool, d.coords, d.lost_turn, d.lost_elem, a, idx, tb, te);
+        run_turns<EXH, FEAT, 1, true>(s, c, mm, alive, vseq, vrecs, vpool, d.coords, d.lost_turn, d.lost_elem, a, idx, tb, te);
         store_state(ck + (int64_t)j * 384, 64, s[0]);
       }
       const double l2 = alive[0] ? s[0].x * s[0].x + s[0].y * s[0].y : 0.0;
@@ -721,7 +752,7 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_kernel(const TpVarDes
         item_loss[2 * (int64_t)it + 1] = cnt;
       }
     } else {
-      run_turns<EXH, FEAT, 1, true>(s, c, alive, vseq, vrecs, vpool, d.coords, d.lost_turn, d.lost_elem, a, idx, 0,
+      run_turns<EXH, FEAT, 1, true>(s, c, mm, alive, vseq, vrecs, vpool, d.coords, d.lost_turn, d.lost_elem, a, idx, 0,
                               d.nturns);
     }
     if (alive[0]) store_state(d.coords + li, d.n, s[0]);
@@ -780,11 +811,13 @@ constexpr bool has_budgets() {
 template <bool EXH, unsigned FEAT, bool SER>
 static void launch_vs(const TpKArgs& a, int ppl, unsigned grid, int block, hipStream_t stream) {
   if (ppl == 2) {
-    hipLaunchKernelGGL((tpk::TPK_BUILD::track_kernel<EXH, FEAT, 2, 0, SER>), dim3(grid), dim3(block), 0, stream,
+    hipLaunchKernelGGL((tpk::TPK_BUILD::track_kernel<EXH, FEAT, 2, 0, SER>), dim3(grid), dim3(block),
+                       TP_LDS_PER_WAVE(2) * (block / 64), stream,
                        TP_KPTRS(a), a);
   } else {
 #define TP_CALL(W)                                                                                               \
-  hipLaunchKernelGGL((tpk::TPK_BUILD::track_kernel<EXH, FEAT, 1, W, SER>), dim3(grid), dim3(block), 0, stream, \
+  hipLaunchKernelGGL((tpk::TPK_BUILD::track_kernel<EXH, FEAT, 1, W, SER>), dim3(grid), dim3(block),      \
+                     TP_LDS_PER_WAVE(1) * (block / 64), stream,                                            \
                      TP_KPTRS(a), a)
     TP_WPE_SWITCH(a.waves, TP_CALL)
 #undef TP_CALL
@@ -837,7 +870,8 @@ static const void* batch_fn(int waves) {
 template <bool EXH, unsigned FEAT, bool CKPT>
 static void launch_batch_v(const TpBatchArgs& b, hipStream_t stream) {
 #define TP_CALL(W)                                                                                               \
-  hipLaunchKernelGGL((tpk::TPK_BUILD::batch_kernel<EXH, FEAT, CKPT, W>), dim3(b.grid), dim3(b.block), 0, stream, \
+  hipLaunchKernelGGL((tpk::TPK_BUILD::batch_kernel<EXH, FEAT, CKPT, W>), dim3(b.grid), dim3(b.block),      \
+                     TP_LDS_PER_WAVE(1) * (b.block / 64), stream,                                             \
                      b.desc, b.items, b.n_items, b.counter, b.seq, b.recs, b.pool, b.ckpt, b.item_loss)
   TP_WPE_SWITCH(b.waves, TP_CALL)
 #undef TP_CALL
@@ -882,7 +916,8 @@ int TP_BATCH_OCC(int variant, int exact_h, bool ckpt, int block, int waves) {
 #undef TP_CALL
   }
   int nb = 0;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) == hipSuccess ? nb : 0;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, TP_LDS_PER_WAVE(1) * (block / 64)) == hipSuccess
+             ? nb : 0;
 }
 
 int TP_TRACK_OCC(int variant, int exact_h, int block, int ppl, int waves, int ser) {
@@ -891,7 +926,8 @@ int TP_TRACK_OCC(int variant, int exact_h, int block, int ppl, int waves, int se
   TP_BATCH_DISPATCH(TP_CALL)
 #undef TP_CALL
   int nb = 0;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, 0) == hipSuccess ? nb : 0;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, TP_LDS_PER_WAVE(ppl) * (block / 64)) ==
+                 hipSuccess ? nb : 0;
 }
 
 hipError_t TP_LAUNCH(const TpKArgs& a_in, int block, int ppl, int max_grid, hipStream_t stream) {
