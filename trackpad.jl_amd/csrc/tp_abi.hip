@@ -767,8 +767,9 @@ int tp_adjoint(tp_ctx* ctx, const tp_table* t, tp_particles* p, int32_t turn0, i
     tm->n_slots = ns;
   }
   const int32_t n_slots = tm->n_slots;
-  if ((size_t)4 * n_slots * sizeof(double) > 64 * 1024)
-    return fail(TP_EUNSUPPORTED, "tp_adjoint: %d strength slots exceed the 64 KiB LDS budget (max 2048)", n_slots);
+  // LDS of the reverse kernel: a row of slots per wave and the waves' edge memo (128 doubles)
+  if ((size_t)4 * ((size_t)n_slots + 128) * sizeof(double) > 64 * 1024)
+    return fail(TP_EUNSUPPORTED, "tp_adjoint: %d strength slots exceed the 64 KiB LDS budget (max 1920)", n_slots);
   const int64_t n = p->n;
   const int32_t K = ckpt_stride > 0 ? ckpt_stride : 1;
   const int32_t nseg = (nturns + K - 1) / K;
@@ -1362,7 +1363,7 @@ int tp_batch_adjoint(tp_ctx* ctx, tp_batch* b, const int32_t* turn0, const int32
   if (flags & ~(TP_TRACK_FAST | TP_TRACK_TIMED)) return fail(TP_EINVAL, "tp_batch_adjoint: unsupported flags 0x%x", flags);
   if (ckpt_stride < 0) return fail(TP_EINVAL, "tp_batch_adjoint: ckpt_stride = %d", ckpt_stride);
   if (!b->no_adjoint.empty()) return fail(TP_EUNSUPPORTED, "tp_batch_adjoint: %s", b->no_adjoint.c_str());
-  if ((size_t)(b->block / 64) * b->max_slots * sizeof(double) > 64 * 1024)
+  if ((size_t)(b->block / 64) * ((size_t)b->max_slots + 128) * sizeof(double) > 64 * 1024)
     return fail(TP_EUNSUPPORTED, "tp_batch_adjoint: %d strength slots exceed the 64 KiB LDS budget", b->max_slots);
   const int32_t K = ckpt_stride > 0 ? ckpt_stride : 1;
   rc = batch_plan(b, 2, turn0, nturns, K, flags & TP_TRACK_FAST);

@@ -265,8 +265,11 @@ TP_DEV void ebody_back(St& s, Ad& a, const Cx& c, double irho, double rho, doubl
 // ---------------------------------------------------------------------------------------
 // Linear dipole edge (src/tracking.jl:160-221): inverse + VJP.  blk = [fx, fyc, e, fc].
 // ---------------------------------------------------------------------------------------
+// MEMO: the fringe term tan(e - fc/(1 + delta)) from the particle's edge memo (edge_tan: the
+// same bits as the call).
+template <bool MEMO = false>
 TP_DEV void edge_fy(const St& s, int method, const double* __restrict__ blk, double irho, bool is_exit, double& fy,
-                    double& dfy_px, double& dfy_d) {
+                    double& dfy_px, double& dfy_d, EdgeMemo* mm = nullptr) {
   const double fyc = blk[1], e = blk[2], fc = blk[3];
   const double opd = 1.0 + s.d;
   dfy_px = 0.0;
@@ -286,7 +289,7 @@ TP_DEV void edge_fy(const St& s, int method, const double* __restrict__ blk, dou
     t = fyc;
     dt_d = 0.0;
   } else {
-    const double tt = tan_lib(e - fc / opd);
+    const double tt = MEMO ? edge_tan(*mm, s, e, fc) : tan_lib(e - fc / opd);
     t = irho * tt;
     dt_d = irho * (1.0 + tt * tt) * fc / (opd * opd);
   }
@@ -299,11 +302,13 @@ TP_DEV void edge_fy(const St& s, int method, const double* __restrict__ blk, dou
   }
 }
 
-TP_DEV void edge_back(St& s, Ad& a, int method, const double* __restrict__ blk, double irho, bool is_exit) {
+template <bool MEMO = false>
+TP_DEV void edge_back(St& s, Ad& a, int method, const double* __restrict__ blk, double irho, bool is_exit,
+                      EdgeMemo* mm = nullptr) {
   const double fx = blk[0];
   s.px = s.px - s.x * fx;
   double fy, dfy_px, dfy_d;
-  edge_fy(s, method, blk, irho, is_exit, fy, dfy_px, dfy_d);
+  edge_fy<MEMO>(s, method, blk, irho, is_exit, fy, dfy_px, dfy_d, mm);
   s.py = s.py + s.y * fy;
   // px' = px + x fx; py' = py - y fy(px, delta)
   a.x = a.x + a.px * fx;

@@ -188,7 +188,7 @@ TP_DEV void sub6(St& s, const double* __restrict__ t) {
 // holds the inverses of the r1 / r2 matrices at the pool offsets of the matrices (variant
 // with FT_MISALIGN); lin selects the linearised drift (variant with FT_LINH).
 template <unsigned FEAT, bool SER_OK>
-TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double* __restrict__ pool,
+TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, EdgeMemo* mm, const tp_rec& rec, const double* __restrict__ pool,
                             const double* __restrict__ minv, double beti, bool lin, double& gacc) {
   // the series drift needs two more hoisted values; kernels with general kicks never take it
   constexpr bool SER = SER_OK && (FEAT & FT_GENKICK) == 0;
@@ -222,22 +222,22 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
       if constexpr ((FEAT & FT_SBEND) != 0) {
         const int fin = (fl >> TP_F_FIN_SHIFT) & 0xF, fout = (fl >> TP_F_FOUT_SHIFT) & 0xF;
         const double irho = p[6];
-        if (fout != 0) edge_back(s, a, fout, p + 13, irho, true);
+        if (fout != 0) edge_back<true>(s, a, fout, p + 13, irho, true, mm);
 #define TP_CALL(KK) sbend_back<KK, SER>(s, a, c, p, ns, m, beti, lin)
         TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
-        if (fin != 0) edge_back(s, a, fin, p + 9, irho, false);
+        if (fin != 0) edge_back<true>(s, a, fin, p + 9, irho, false, mm);
       }
       break;
     case TP_ESBEND_LE:
       if constexpr ((FEAT & FT_ESBEND_LE) != 0) {
         const int fin = (fl >> TP_F_FIN_SHIFT) & 0xF, fout = (fl >> TP_F_FOUT_SHIFT) & 0xF;
         const double irho = p[0];
-        if (fout != 0) edge_back(s, a, fout, p + 15, irho, true);
+        if (fout != 0) edge_back<true>(s, a, fout, p + 15, irho, true, mm);
 #define TP_CALL(KK) esbend_body_back<KK>(s, a, c, p, ns, m)
         TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
-        if (fin != 0) edge_back(s, a, fin, p + 11, irho, false);
+        if (fin != 0) edge_back<true>(s, a, fin, p + 11, irho, false, mm);
       }
       break;
     case TP_ESBEND:
@@ -300,7 +300,7 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, const tp_rec& rec, const double
 // lattice of turn r + row_turn0 in sequence row r; the turns undone here are turn_last,
 // turn_last - 1, ..., each with its own row.
 template <unsigned FEAT, bool SER_OK>
-TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq_rows, const tp_rec* __restrict__ recs,
+TP_DEV void back_turns(St& s, Ad& l, Cx& c, EdgeMemo* mm, const int32_t* __restrict__ seq_rows, const tp_rec* __restrict__ recs,
                        const double* __restrict__ pool, const double* __restrict__ minv,
                        const int32_t* __restrict__ slot_of_pos, int32_t n_seq, int32_t nturns, int32_t turn_last,
                        int32_t n_rows, int32_t row_turn0, double beti, bool lin, double* row, int lane) {
@@ -321,7 +321,7 @@ TP_DEV void back_turns(St& s, Ad& l, Cx& c, const int32_t* __restrict__ seq_rows
         if (e >= 2) ri_next = seq[e - 2];
       }
       double gacc = 0.0;
-      apply_elem_back<FEAT, SER_OK>(s, l, c, rec, pool, minv, beti, lin, gacc);
+      apply_elem_back<FEAT, SER_OK>(s, l, c, mm, rec, pool, minv, beti, lin, gacc);
       if (slot >= 0) {
         const double w = wave_sum(gacc);
         if (lane == 0) row[slot] += w;
@@ -341,9 +341,11 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void adjoint_kernel(const int32_
                                                       const double* __restrict__ end_state, double* __restrict__ lam,
                                                       const int32_t* __restrict__ lost_turn,
                                                       double* __restrict__ partial, TpAdjArgs a) {
-  extern __shared__ double lds[];  // [waves per block][n_slots]
+  extern __shared__ double lds[];  // [waves per block][n_slots], then the edge memo's [waves][128]
   const int nw = blockDim.x / 64, wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   for (int k = threadIdx.x; k < nw * a.n_slots; k += blockDim.x) lds[k] = 0.0;
+  EdgeMemo mm{0, 0, lds + nw * a.n_slots + 128 * __builtin_amdgcn_readfirstlane(wave)};
+  mm.w[lane] = __builtin_nan("");
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool alive = (i < a.n) && (lost_turn[i] == -1);
@@ -372,7 +374,7 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void adjoint_kernel(const int32_
   Cx c;
   mkctx(s, c, a.beti);
   const bool lin = (FEAT & FT_LINH) != 0 && a.exact_h == 0;
-  back_turns<FEAT, false>(s, l, c, seq, recs, pool, a.minv, slot_of_pos, a.n_seq, a.nturns, a.turn_last, a.n_rows,
+  back_turns<FEAT, false>(s, l, c, &mm, seq, recs, pool, a.minv, slot_of_pos, a.n_seq, a.nturns, a.turn_last, a.n_rows,
                    a.row_turn0, a.beti, lin, lds + wave * a.n_slots, lane);
   if (i < a.n) {
     double* r = lam + i;
@@ -407,10 +409,12 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_back_kernel(const TpV
                                                          const double* __restrict__ ckpt,
                                                          double* __restrict__ item_partial, int32_t max_slots,
                                                          int32_t exact_h) {
-  extern __shared__ double lds[];  // [waves per block][max_slots]
+  extern __shared__ double lds[];  // [waves per block][max_slots], then the edge memo's [waves][128]
   const int wave = threadIdx.x / 64, lane = threadIdx.x & 63;
   const bool lin = (FEAT & FT_LINH) != 0 && exact_h == 0;
   double* row = lds + wave * max_slots;
+  EdgeMemo mm{0, 0, lds + (blockDim.x / 64) * max_slots + 128 * __builtin_amdgcn_readfirstlane(wave)};
+  mm.w[lane] = __builtin_nan("");
   for (;;) {
     int32_t it = 0;
     if (lane == 0) it = atomicAdd(counter, 1);
@@ -436,7 +440,7 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_back_kernel(const TpV
       }
       mkctx(s, c, d.beti);
       const int32_t nt = (j == d.nseg - 1) ? d.nturns - j * d.ckpt_stride : d.ckpt_stride;
-      back_turns<FEAT, true>(s, l, c, seq + d.seq_off, recs + d.rec_off, pool + d.pool_off,
+      back_turns<FEAT, true>(s, l, c, &mm, seq + d.seq_off, recs + d.rec_off, pool + d.pool_off,
                        minv ? minv + d.pool_off : nullptr, slots + d.pos_off, d.n_seq, nt,
                        d.turn0 + j * d.ckpt_stride + nt - 1, d.n_rows, d.row_turn0, d.beti, lin, row, lane);
     }
@@ -504,7 +508,7 @@ __global__ void reduce_kernel(const double* __restrict__ partial, int nblocks, i
 
 template <unsigned FEAT>
 static void launch_v(const TpAdjArgs& a, unsigned grid, hipStream_t stream) {
-  const size_t lds = (size_t)4 * (size_t)(a.n_slots > 0 ? a.n_slots : 1) * sizeof(double);
+  const size_t lds = (size_t)4 * ((size_t)(a.n_slots > 0 ? a.n_slots : 1) + 128) * sizeof(double);
 #define TP_CALL(W)                                                                                          \
   hipLaunchKernelGGL((adjoint_kernel<FEAT, W>), dim3(grid), dim3(256), lds, stream, a.seq, a.recs, a.pool, \
                      a.slot_of_pos, a.end_state, a.lam, a.lost_turn, a.partial, a)
@@ -514,7 +518,7 @@ static void launch_v(const TpAdjArgs& a, unsigned grid, hipStream_t stream) {
 
 template <unsigned FEAT>
 static void launch_back_v(const TpBatchArgs& b, hipStream_t stream) {
-  const size_t lds = (size_t)(b.block / 64) * (size_t)(b.max_slots > 0 ? b.max_slots : 1) * sizeof(double);
+  const size_t lds = (size_t)(b.block / 64) * ((size_t)(b.max_slots > 0 ? b.max_slots : 1) + 128) * sizeof(double);
 #define TP_CALL(W)                                                                                             \
   hipLaunchKernelGGL((batch_back_kernel<FEAT, W>), dim3(b.grid), dim3(b.block), lds, stream, b.desc, b.items, \
                      b.n_items, b.counter, b.seq, b.recs, b.pool, b.minv, b.slots, b.ckpt, b.item_partial,       \
@@ -547,7 +551,7 @@ hipError_t TP_BATCH_BACK_LAUNCH(const TpBatchArgs& b, hipStream_t stream) {
 }
 
 int TP_BATCH_BACK_OCC(int variant, int block, int max_slots, int waves) {
-  const size_t lds = (size_t)(block / 64) * (size_t)(max_slots > 0 ? max_slots : 1) * sizeof(double);
+  const size_t lds = (size_t)(block / 64) * ((size_t)(max_slots > 0 ? max_slots : 1) + 128) * sizeof(double);
   const void* fn = nullptr;
   switch (variant) {
     case TP_VARIANT_A: fn = tpk::TPK_BUILD::back_fn<TP_FEAT_A>(waves); break;
