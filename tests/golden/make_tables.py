@@ -14,9 +14,28 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "trackpad.jl_amd")]
 
+import math  # noqa: E402
+
 import lattices as LT  # noqa: E402
-from trackpad_hip import Beam, Lattice, compile_table  # noqa: E402
+from trackpad_hip import Beam, Lattice, Quadrupole, Sextupole, compile_table, timed  # noqa: E402
 from trackpad_hip import workloads as W  # noqa: E402
+
+# The turn range of the time-varying golden table: rows for turns 5..8, the first at
+# time 2e-4 s, each later one 1e-6 s after the previous (as ringpass! advances time).
+TV_TURN, TV_NTURNS, TV_TIME, TV_DT = 5, 4, 2e-4, 1e-6
+
+
+def dba_ring_time_varying():
+    """The DBA workload ring with every sextupole ramped by 0.2 % per turn and the first
+    QF1 modulated at 50 Hz in time (test/runtests.jl builds the same lattice in Julia)."""
+    els = list(W.dba_ring().elements)
+    for i, e in enumerate(els):
+        if isinstance(e, Sextupole):
+            els[i] = timed(e, k2=lambda t, n, k0=e.k2: k0 * (1.0 + 0.002 * n))
+    q = els[1]
+    assert isinstance(q, Quadrupole)
+    els[1] = timed(q, k1=lambda t, n, k0=q.k1: k0 * (1.0 + 1e-3 * math.sin(2 * math.pi * 50.0 * t)))
+    return Lattice(els, name="DBA_TV")
 
 
 def cases():
@@ -27,6 +46,9 @@ def cases():
     yield "verify_elements", compile_table(Lattice([e for _, e in LT.element_cases()]), b35)
     yield "fodo_workload", compile_table(W.fodo_lattice(), Beam(W.ENERGY), aperture=0.02)
     yield "dba_workload", compile_table(W.dba_ring(), Beam(W.ENERGY))
+    yield "families_ring", compile_table(LT.families_ring(), Beam(3e9))
+    yield "dba_time_varying", compile_table(dba_ring_time_varying(), Beam(W.ENERGY), turn=TV_TURN,
+                                            nturns=TV_NTURNS, time=TV_TIME, dt_turn=TV_DT)
 
 
 def main():

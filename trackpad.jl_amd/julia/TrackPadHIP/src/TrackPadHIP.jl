@@ -14,8 +14,7 @@ module TrackPadHIP
 
 using Libdl
 using TrackPad
-using TrackPad: AbstractElement, Beam, Lattice, Drift, Marker, Quadrupole, Sextupole, Octupole,
-                ThinMultipole, SBend, ExactSBend, RFCavity, TimeVaryingElement
+using TrackPad: AbstractElement, Beam, Lattice, TimeVaryingElement
 
 export HIPContext, HIPTable, HIPParticles, HIPBatch, compile_table, track!, linepass_hip!, ringpass_hip!,
        ringpass_grad, ExactSBendLinearEdge, download, jacobian, autotune!, set_waves!, set_launch!, batch_grad,
@@ -49,10 +48,12 @@ end
 mutable struct HIPContext
     handle::Ptr{Cvoid}
     device::Int
+    tables::Dict{Vector{UInt8}, Any}      # uploaded tables by blob (HIPTable values)
+    memo::WeakKeyDict{Any, Any}           # lattice element vector -> (options, elements, HIPTable)
     function HIPContext(device::Integer = 0)
         h = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:tp_ctx_create, libtp()), Cint, (Cint, Ptr{Ptr{Cvoid}}), device, h))
-        c = new(h[], Int(device))
+        c = new(h[], Int(device), Dict{Vector{UInt8}, Any}(), WeakKeyDict{Any, Any}())
         finalizer(c) do x
             ccall((:tp_ctx_destroy, libtp()), Cint, (Ptr{Cvoid},), x.handle)
         end
@@ -88,6 +89,48 @@ mutable struct HIPTable
 end
 HIPTable(lat::Lattice, beam::Beam; ctx::HIPContext = default_context(), kw...) =
     HIPTable(ctx, compile_table(lat, beam; kw...))
+
+# An element whose fields are all immutable values is unchanged exactly when it is the same
+# (===) object; elements holding vectors (wake samples, slice weights, wiggler harmonics,
+# WithMultipoles) could change in place, so their lattices are compiled on every call.
+immutable_fields(e) = !any(f -> ismutable(getfield(e, f)), fieldnames(typeof(e)))
+
+"""
+    table_for(ctx, lat, beam; aperture = nothing, turn = 0, nturns = nothing, time = 0.0,
+              dt_turn = 0.0) -> HIPTable
+
+The device table of `lat` for a tracking call, uploaded once per context and reused:
+
+* a static lattice whose elements are the same objects as in an earlier call with the same
+  beam and aperture gets that call's table without being compiled again;
+* otherwise the lattice is compiled (time-varying elements resolved per turn for
+  `turn .. turn + nturns - 1`, as `ringpass!` resolves them, src/lattice.jl:305-317) and the
+  blob is looked up among the context's uploaded tables (at most 64 are kept).
+"""
+function table_for(ctx::HIPContext, lat::Lattice, beam::Beam; aperture = nothing, turn::Integer = 0,
+                   nturns::Union{Nothing, Integer} = nothing, time::Real = 0.0, dt_turn::Real = 0.0)
+    els = lat.elements
+    varying = any(w -> base(w) isa TimeVaryingElement, els)
+    memo_ok = !varying && all(immutable_fields, els)
+    opts = (beam, aperture === nothing ? nothing : Float64(aperture))
+    if memo_ok
+        m = get(ctx.memo, els, nothing)
+        if m !== nothing && m[1] == opts && length(m[2]) == length(els) &&
+           all(i -> m[2][i] === els[i], eachindex(els))
+            return m[3]::HIPTable
+        end
+    end
+    blob = compile_table(lat, beam; aperture = aperture, turn = turn, nturns = nturns, time = time,
+                         dt_turn = dt_turn)
+    tab = get(ctx.tables, blob, nothing)
+    if tab === nothing
+        length(ctx.tables) >= 64 && empty!(ctx.tables)
+        tab = HIPTable(ctx, blob)
+        ctx.tables[blob] = tab
+    end
+    memo_ok && (ctx.memo[els] = (opts, collect(Any, els), tab))
+    return tab::HIPTable
+end
 
 mutable struct HIPParticles
     handle::Ptr{Cvoid}
@@ -153,50 +196,71 @@ function track!(p::HIPParticles, tab::HIPTable; nturns::Integer = 1, turn::Integ
                 p.ctx.handle, tab.handle, p.handle, turn, nturns, fast ? TP_TRACK_FAST : UInt32(0)))
     return p
 end
-track!(p::HIPParticles, lat::Lattice, beam::Beam; aperture = nothing, kw...) =
-    track!(p, HIPTable(p.ctx, compile_table(lat, beam; aperture = aperture)); kw...)
-
-# Methods of the reference's functions for device particles (src/lattice.jl:260, 305).
-TrackPad.linepass!(p::HIPParticles, lat::Lattice, beam::Beam; time::Real = 0, turn::Integer = 0, kw...) =
-    (track!(p, lat, beam; nturns = 1, turn = turn, kw...); nothing)
-TrackPad.ringpass!(p::HIPParticles, lat::Lattice, beam::Beam, nturns::Int; time::Real = 0,
-                   dt_turn::Real = 0, turn::Integer = 0, kw...) =
-    (track!(p, lat, beam; nturns = nturns, turn = turn, kw...); nothing)
 
 """
-    ringpass_hip!(coords, lat, beam, lost_flags, nturns; turn = 0, fast = false, aperture = nothing)
+    track!(p::HIPParticles, lat, beam; nturns = 1, turn = 0, time = 0.0, dt_turn = 0.0,
+           fast = false, aperture = nothing)
+
+Track device-resident particles through `lat` for `nturns` turns from turn `turn`; turn k
+sees the lattice at time `time` advanced by `dt_turn` k times (only time-varying elements
+depend on it).  The table comes from `table_for` (compiled and uploaded once per lattice).
+"""
+function track!(p::HIPParticles, lat::Lattice, beam::Beam; nturns::Integer = 1, turn::Integer = 0,
+                time::Real = 0.0, dt_turn::Real = 0.0, fast::Bool = false, aperture = nothing)
+    tab = table_for(p.ctx, lat, beam; aperture = aperture, turn = turn, nturns = nturns, time = time,
+                    dt_turn = dt_turn)
+    return track!(p, tab; nturns = nturns, turn = turn, fast = fast)
+end
+
+# Methods of the reference's functions for device particles (src/lattice.jl:260, 305), with
+# its keywords: linepass! resolves time-varying elements at (time, turn), ringpass! at
+# (time + k dt_turn, turn + k) for turn k.
+TrackPad.linepass!(p::HIPParticles, lat::Lattice, beam::Beam; time::Real = 0, turn::Integer = 0, kw...) =
+    (track!(p, lat, beam; nturns = 1, turn = turn, time = time, kw...); nothing)
+TrackPad.ringpass!(p::HIPParticles, lat::Lattice, beam::Beam, nturns::Int; time::Real = 0,
+                   dt_turn::Real = 0, turn::Integer = 0, kw...) =
+    (track!(p, lat, beam; nturns = nturns, turn = turn, time = time, dt_turn = dt_turn, kw...); nothing)
+
+"""
+    ringpass_hip!(coords, lat, beam, lost_flags, nturns; time = 0.0, dt_turn = 0.0, turn = 0,
+                  fast = false, aperture = nothing)
 
 `ringpass!` on host arrays through the GPU: mutates `coords` and `lost_flags` in place with the
 reference's semantics (flags equal to 1 are skipped; newly lost particles get 1 and keep their
-element-entry coordinates).
+element-entry coordinates; time-varying elements resolved per turn as `ringpass!` does).
 """
 function ringpass_hip!(coords::Matrix{Float64}, lat::Lattice, beam::Beam, lost_flags::Vector{Int},
-                       nturns::Integer; turn::Integer = 0, fast::Bool = false, aperture = nothing,
-                       ctx::HIPContext = default_context())
+                       nturns::Integer; time::Real = 0.0, dt_turn::Real = 0.0, turn::Integer = 0,
+                       fast::Bool = false, aperture = nothing, ctx::HIPContext = default_context())
     n = size(coords, 1)
     size(coords, 2) == 6 || throw(ArgumentError("coords must be N x 6"))
     length(lost_flags) == n || throw(DimensionMismatch("lost_flags must have N entries"))
-    tab = HIPTable(ctx, compile_table(lat, beam; aperture = aperture))
+    nturns >= 0 || throw(ArgumentError("nturns = $nturns < 0"))
+    tab = table_for(ctx, lat, beam; aperture = aperture, turn = turn, nturns = nturns, time = time,
+                    dt_turn = dt_turn)
     GC.@preserve coords lost_flags check(ccall((:tp_track_host, libtp()), Cint,
         (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Int64}, Ptr{Int32}, Ptr{Int32}, Int32, Int32, UInt32),
         ctx.handle, tab.handle, coords, n, lost_flags, C_NULL, C_NULL, turn, nturns,
         fast ? TP_TRACK_FAST : UInt32(0)))
     return nothing
 end
-linepass_hip!(coords::Matrix{Float64}, lat::Lattice, beam::Beam, lost_flags::Vector{Int}; kw...) =
-    ringpass_hip!(coords, lat, beam, lost_flags, 1; kw...)
+linepass_hip!(coords::Matrix{Float64}, lat::Lattice, beam::Beam, lost_flags::Vector{Int};
+              time::Real = 0.0, turn::Integer = 0, kw...) =
+    ringpass_hip!(coords, lat, beam, lost_flags, 1; time = time, turn = turn, kw...)
 
 """
-    ringpass_grad(coords0, lat, beam, nturns; ckpt_stride = 1, fast = false)
-        -> (loss, dL_dk::Vector{Float64}, dL_dx0::Matrix{Float64}, n_alive)
+    ringpass_grad(coords0, lat, beam, nturns; turn = 0, time = 0.0, dt_turn = 0.0, ckpt_stride = 1,
+                  fast = false) -> (loss, dL_dk::Vector{Float64}, dL_dx0::Matrix{Float64}, n_alive)
 
 `loss = mean over surviving particles of (x^2 + y^2)` after `nturns` turns; `dL_dk[i]` is the
 derivative with respect to the main strength of lattice position `i` (k1 of a Quadrupole, k2
-of a Sextupole, k3 of an Octupole, 0 elsewhere); `dL_dx0` with respect to the initial beam.
+of a Sextupole, k3 of an Octupole, 0 elsewhere; summed over the turns of a time-varying
+position); `dL_dx0` with respect to the initial beam.
 """
 function ringpass_grad(coords0::Matrix{Float64}, lat::Lattice, beam::Beam, nturns::Integer;
-                       ckpt_stride::Integer = 1, fast::Bool = false, ctx::HIPContext = default_context())
-    tab = HIPTable(ctx, compile_table(lat, beam))
+                       turn::Integer = 0, time::Real = 0.0, dt_turn::Real = 0.0, ckpt_stride::Integer = 1,
+                       fast::Bool = false, ctx::HIPContext = default_context())
+    tab = table_for(ctx, lat, beam; turn = turn, nturns = nturns, time = time, dt_turn = dt_turn)
     p = HIPParticles(coords0; ctx = ctx)
     loss = Ref{Float64}(0)
     na = Ref{Int64}(0)
@@ -205,7 +269,7 @@ function ringpass_grad(coords0::Matrix{Float64}, lat::Lattice, beam::Beam, nturn
     check(ccall((:tp_adjoint, libtp()), Cint,
                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Int32, Int32, UInt32, Ptr{Float64}, Ptr{Int64},
                  Ptr{Float64}, Ptr{Float64}),
-                ctx.handle, tab.handle, p.handle, 0, nturns, ckpt_stride, fast ? TP_TRACK_FAST : UInt32(0),
+                ctx.handle, tab.handle, p.handle, turn, nturns, ckpt_stride, fast ? TP_TRACK_FAST : UInt32(0),
                 loss, na, dk, dx0))
     return loss[], dk[1:tab.n_seq], dx0, Int(na[])
 end
