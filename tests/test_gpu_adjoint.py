@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as O
-from trackpad_hip import (Beam, Drift, ExactSBend, Lattice, Quadrupole, Sextupole, TrackPadHIPError, ringpass_grad,
+from trackpad_hip import (Beam, Drift, ExactSBend, Lattice, Octupole, Quadrupole, Sextupole, TrackPadHIPError, ringpass_grad,
                           set_strengths, strength_positions)
 from trackpad_hip import workloads as W
 
@@ -112,6 +112,31 @@ def test_adjoint_deterministic_and_stride_independent(ctx):
         assert c.loss == a.loss
         np.testing.assert_allclose(c.dk, a.dk, rtol=1e-9, atol=1e-9 * np.max(np.abs(a.dk)))
         np.testing.assert_allclose(c.dx0, a.dx0, rtol=1e-8, atol=1e-9 * np.max(np.abs(a.dx0)))
+
+
+@pytest.mark.parametrize("fast", [False, True])
+def test_adjoint_zero_strength_magnets(ctx, fast):
+    """A magnet at zero strength is the identity map, but its strength gradient is not zero
+    (dpx'/dk1 = -K x): quadrupoles, a sextupole and an octupole switched off, beside a
+    sextupole with only a quadrupole error (its b2 gradient through the general path), each
+    against the oracle's forward-mode derivative, 1e-9 relative."""
+    beam = Beam(W.ENERGY)
+    sx_err = Sextupole(0.2, 0.0, num_int_steps=6, polynom_b=[0.0, 0.3, 0.0, 0.0])
+    lat = Lattice([Quadrupole(0.4, 1.2, num_int_steps=6), Drift(1.0), Sextupole(0.2, 0.0, num_int_steps=6),
+                   Drift(1.0), Quadrupole(0.4, -1.2, num_int_steps=6), Drift(1.0),
+                   Quadrupole(0.3, 0.0, num_int_steps=6), Octupole(0.2, 0.0, num_int_steps=4), sx_err,
+                   Drift(1.0), Sextupole(0.2, 5.0, num_int_steps=6), Drift(0.5)])
+    n, T = 64, 10
+    c0 = W.dba_particles(n, seed=27)
+    g = ringpass_grad(c0, lat, beam, T, fast=fast, ctx=ctx)
+    assert g.n_alive == n
+    for p in strength_positions(lat):
+        dk = np.zeros(len(lat))
+        dk[p] = 1.0
+        _, dl, na = O.loss_dir(lat, beam, c0, T, dk=dk)
+        ref = dl / na
+        assert ref != 0.0
+        assert abs(g.dk[p] - ref) <= 1e-9 * abs(ref), (p, g.dk[p], ref)
 
 
 def test_adjoint_survivors_only(ctx):

@@ -129,7 +129,16 @@ TP_DEV void wpow(const St& s, int g, double& wr, double& wi) {
 // strthinkick: inverse + VJP; gacc += dL/d b_g (main strength of the element).
 template <int KK>
 TP_DEV void kick_back(St& s, Ad& a, const double* __restrict__ poly, int m, double K, int g, double& gacc) {
-  if (KK == TP_KICK_NONE) return;
+  if (KK == TP_KICK_NONE) {
+    // the identity map, but a magnet at zero strength still has a strength gradient:
+    // d px' / d b_g = -K Re(w^g), d py' / d b_g = K Im(w^g)
+    if (g >= 0) {
+      double wr, wi;
+      wpow(s, g, wr, wi);
+      gacc = gacc + K * (a.py * wi - a.px * wr);
+    }
+    return;
+  }
   double re, im, U, V;
   field_and_slope<KK>(s, poly, m, re, im, U, V);
   s.px = s.px + K * re;
@@ -144,6 +153,91 @@ TP_DEV void kick_back(St& s, Ad& a, const double* __restrict__ poly, int m, doub
   a.x = ax;
   a.y = ay;
 }
+
+#if TP_FAST
+// ---------------------------------------------------------------------------------------
+// A Yoshida step of a thick multipole backwards, in the forms the fast forward kernel uses
+// (mpole_steps): drifts without their path-length terms (drift_xy), the step's path length
+// added to z at once, the sextupole kick with K b2 folded (kick_sext_kb).  Inside a step
+// neither z nor its adjoint is read or changed by a drift or a kick, so a.z B is formed once
+// per element and the a.z NormL terms of delta's adjoint once per step.
+// ---------------------------------------------------------------------------------------
+// drift_xy backwards: x and y undone; the adjoints of px, py and delta updated except the
+// a.z NormL term of delta.  azB = a.z B.  Returns NormL.
+template <bool SER>
+TP_DEV double drift_xy_back(St& s, Ad& a, const Cx& c, double L, double azB) {
+  const double rz = SER ? inv_pz(s, c, 1.0) : rsqrt_nr(c.A - s.px * s.px - s.py * s.py);
+  const double NormL = L * rz;
+  s.x = s.x - NormL * s.px;
+  s.y = s.y - NormL * s.py;
+  // d NormL / d px = NormL px / pz^2, / d py likewise, / d delta = -NormL B / pz^2
+  const double f = __builtin_fma(a.x, s.px, __builtin_fma(a.y, s.py, azB)) * (NormL * (rz * rz));
+  a.px = a.px + a.x * NormL + f * s.px;
+  a.py = a.py + a.y * NormL + f * s.py;
+  a.d = a.d - f * c.B;
+  return NormL;
+}
+
+// Quadrupole kick (px -= K b1 x, py += K b1 y) backwards; Kb = K b1.  gq accumulates
+// a.py y - a.px x, so that dL/d b1 = sum over the kicks of K gq.
+TP_DEV void kick_quad_back(St& s, Ad& a, double K, double b1, double Kb, double& gq) {
+  s.px = s.px + K * (b1 * s.x);
+  s.py = s.py - K * (b1 * s.y);
+  gq = __builtin_fma(a.py, s.y, __builtin_fma(-a.px, s.x, gq));
+  a.x = a.x - Kb * a.px;
+  a.y = a.y + Kb * a.py;
+}
+
+// Sextupole kick in the forward's folded form (kick_sext_kb, Kb = K b2) backwards: its four
+// FMAs undone in reverse order.  With rx = Kb x, iy = Kb y the Jacobian's kick block is
+// [[-2 rx, 2 iy], [2 iy, 2 rx]], so a.x += 2 tx, a.y += 2 ty with tx = a.py iy - a.px rx,
+// ty = a.px iy + a.py rx; and x tx + y ty = Kb dL/dKb, so dL/d b2 = (sum of x tx + y ty) / b2
+// over the element's kicks (gs accumulates the sum).
+TP_DEV void kick_sext_back(St& s, Ad& a, double Kb, double& gs) {
+  const double rx = Kb * s.x, iy = Kb * s.y;
+  s.py = __builtin_fma(-rx, s.y, s.py);
+  s.py = __builtin_fma(-iy, s.x, s.py);
+  s.px = __builtin_fma(-iy, s.y, s.px);
+  s.px = __builtin_fma(rx, s.x, s.px);
+  const double tx = __builtin_fma(a.py, iy, -(a.px * rx));
+  const double ty = __builtin_fma(a.px, iy, a.py * rx);
+  gs = __builtin_fma(s.x, tx, __builtin_fma(s.y, ty, gs));
+  a.x = __builtin_fma(2.0, tx, a.x);
+  a.y = __builtin_fma(2.0, ty, a.y);
+}
+
+// The steps of a Quadrupole / Sextupole / Octupole record whose kick is the identity, a pure
+// quadrupole or a pure sextupole, exact Hamiltonian, undone; grad: the element has a gradient
+// slot and its main strength is the kick's own coefficient (b1 of QUAD, b2 of SEXT).
+template <int KK, bool SER>
+TP_DEV void mpole_back_fast(St& s, Ad& a, const Cx& c, const double* __restrict__ p, int ns, bool grad,
+                            double& gacc) {
+  const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3];
+  const double Lsb = 2.0 * p[4] + 2.0 * p[5];  // the forward's 2 L1b + 2 L2b
+  const double azB = a.z * c.B;
+  const double b = KK == TP_KICK_QUAD ? p[6 + 2] : (KK == TP_KICK_SEXT ? p[6 + 4] : 0.0);
+  const double Kb1 = K1 * b, Kb2 = K2 * b;
+  double g1 = 0.0, g2 = 0.0;
+  for (int st = 0; st < ns; ++st) {
+    double S = drift_xy_back<SER>(s, a, c, L1, azB);
+    if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, K1, b, Kb1, g1);
+    if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb1, g1);
+    S = S + drift_xy_back<SER>(s, a, c, L2, azB);
+    if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, K2, b, Kb2, g2);
+    if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb2, g1);
+    S = S + drift_xy_back<SER>(s, a, c, L2, azB);
+    if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, K1, b, Kb1, g1);
+    if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb1, g1);
+    S = S + drift_xy_back<SER>(s, a, c, L1, azB);
+    s.z = s.z - __builtin_fma(S, c.B, -Lsb);
+    a.d = __builtin_fma(a.z, S, a.d);
+  }
+  if (grad) {
+    if constexpr (KK == TP_KICK_QUAD) gacc = __builtin_fma(K1, g1, __builtin_fma(K2, g2, gacc));
+    if constexpr (KK == TP_KICK_SEXT) gacc = gacc + g1 / b;
+  }
+}
+#endif
 
 // bndthinkick: inverse + VJP (no strength gradient for bends).
 template <int KK>

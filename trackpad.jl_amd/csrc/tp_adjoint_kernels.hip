@@ -41,6 +41,18 @@ namespace TPK_BUILD {
 template <int KK, bool SER>
 TP_DEV void mpole_back(St& s, Ad& a, const Cx& c, const double* __restrict__ p, int ns, int m, int g, double& gacc,
                        bool lin) {
+#if TP_FAST
+  if constexpr (KK != TP_KICK_GEN) {
+    // the forward's step forms (mpole_steps) undone as such; a strength gradient of another
+    // order than the kick's own coefficient (a magnet whose main strength is zero) takes the
+    // general path below
+    constexpr int gown = KK == TP_KICK_QUAD ? 1 : (KK == TP_KICK_SEXT ? 2 : -1);
+    if (!lin && (g < 0 || g == gown)) {
+      mpole_back_fast<KK, SER>(s, a, c, p, ns, g >= 0, gacc);
+      return;
+    }
+  }
+#endif
   const double L1 = p[0], L2 = p[1], K1 = p[2], K2 = p[3], L1b = p[4], L2b = p[5];
   const double* poly = p + 6;
   // the Yoshida step D1 K1 D2 K2 D2 K1 D1 is a palindrome: undo it in the same order
