@@ -178,14 +178,49 @@ TP_DEV double drift_xy_back(St& s, Ad& a, const Cx& c, double L, double azB) {
   return NormL;
 }
 
-// Quadrupole kick (px -= K b1 x, py += K b1 y) backwards; Kb = K b1.  gq accumulates
-// a.py y - a.px x, so that dL/d b1 = sum over the kicks of K gq.
-TP_DEV void kick_quad_back(St& s, Ad& a, double K, double b1, double Kb, double& gq) {
-  s.px = s.px + K * (b1 * s.x);
-  s.py = s.py - K * (b1 * s.y);
+// The series drift of one length, as the forward's inv_pz forms it per element and particle:
+// NormL = Lr + w (c1 + w c2), Lr = L R, c1 = Lr H, c2 = c1 H3; and d1 = 2 c1, d2 = 4 c2, so that
+// NormL / pz^2 = 2 dNormL/dw = d1 + w d2 -- the exact derivative of the map the forward
+// applied (lanes past the series bound take the refined rsqrt and its own derivative).
+struct DSer {
+  double Lr, c1, c2, d1, d2;
+};
+TP_DEV DSer dser(const Cx& c, double L) {
+  DSer k;
+  k.Lr = L * c.R;
+  k.c1 = k.Lr * c.H;
+  k.c2 = k.c1 * c.H3;
+  k.d1 = k.c1 + k.c1;
+  k.d2 = k.c2 * 4.0;
+  return k;
+}
+TP_DEV double drift_xy_back_ser(St& s, Ad& a, const Cx& c, double L, const DSer& k, double azB) {
+  const double w = __builtin_fma(s.px, s.px, s.py * s.py);
+  double NormL = __builtin_fma(w, __builtin_fma(w, k.c2, k.c1), k.Lr);
+  double g = __builtin_fma(w, k.d2, k.d1);
+  if (__builtin_expect(!(w < c.WB), 0)) {
+    const double rz = rsqrt_nr(c.A - w);
+    NormL = L * rz;
+    g = NormL * (rz * rz);
+  }
+  s.x = s.x - NormL * s.px;
+  s.y = s.y - NormL * s.py;
+  const double f = __builtin_fma(a.x, s.px, __builtin_fma(a.y, s.py, azB)) * g;
+  a.px = a.px + a.x * NormL + f * s.px;
+  a.py = a.py + a.y * NormL + f * s.py;
+  a.d = a.d - f * c.B;
+  return NormL;
+}
+
+// Quadrupole kick (px -= K b1 x, py += K b1 y) backwards with Kb = K b1 (one rounding of
+// the strength per element, below the rounding of the inverse drifts around it).  gq
+// accumulates a.py y - a.px x, so that dL/d b1 = sum over the kicks of K gq.
+TP_DEV void kick_quad_back(St& s, Ad& a, double Kb, double& gq) {
+  s.px = __builtin_fma(Kb, s.x, s.px);
+  s.py = __builtin_fma(-Kb, s.y, s.py);
   gq = __builtin_fma(a.py, s.y, __builtin_fma(-a.px, s.x, gq));
-  a.x = a.x - Kb * a.px;
-  a.y = a.y + Kb * a.py;
+  a.x = __builtin_fma(-Kb, a.px, a.x);
+  a.y = __builtin_fma(Kb, a.py, a.y);
 }
 
 // Sextupole kick in the forward's folded form (kick_sext_kb, Kb = K b2) backwards: its four
@@ -199,7 +234,7 @@ TP_DEV void kick_sext_back(St& s, Ad& a, double Kb, double& gs) {
   s.py = __builtin_fma(-iy, s.x, s.py);
   s.px = __builtin_fma(-iy, s.y, s.px);
   s.px = __builtin_fma(rx, s.x, s.px);
-  const double tx = __builtin_fma(a.py, iy, -(a.px * rx));
+  const double tx = __builtin_fma(-a.px, rx, a.py * iy);
   const double ty = __builtin_fma(a.px, iy, a.py * rx);
   gs = __builtin_fma(s.x, tx, __builtin_fma(s.y, ty, gs));
   a.x = __builtin_fma(2.0, tx, a.x);
@@ -218,20 +253,28 @@ TP_DEV void mpole_back_fast(St& s, Ad& a, const Cx& c, const double* __restrict_
   const double b = KK == TP_KICK_QUAD ? p[6 + 2] : (KK == TP_KICK_SEXT ? p[6 + 4] : 0.0);
   const double Kb1 = K1 * b, Kb2 = K2 * b;
   double g1 = 0.0, g2 = 0.0;
-  for (int st = 0; st < ns; ++st) {
-    double S = drift_xy_back<SER>(s, a, c, L1, azB);
-    if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, K1, b, Kb1, g1);
-    if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb1, g1);
-    S = S + drift_xy_back<SER>(s, a, c, L2, azB);
-    if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, K2, b, Kb2, g2);
-    if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb2, g1);
-    S = S + drift_xy_back<SER>(s, a, c, L2, azB);
-    if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, K1, b, Kb1, g1);
-    if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb1, g1);
-    S = S + drift_xy_back<SER>(s, a, c, L1, azB);
-    s.z = s.z - __builtin_fma(S, c.B, -Lsb);
-    a.d = __builtin_fma(a.z, S, a.d);
+#define TP_STEP_BACK(D1, D2)                                              \
+  for (int st = 0; st < ns; ++st) {                                       \
+    double S = D1;                                                        \
+    if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, Kb1, g1);      \
+    if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb1, g1);      \
+    S = S + D2;                                                           \
+    if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, Kb2, g2);      \
+    if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb2, g1);      \
+    S = S + D2;                                                           \
+    if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, Kb1, g1);      \
+    if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb1, g1);      \
+    S = S + D1;                                                           \
+    s.z = s.z - __builtin_fma(S, c.B, -Lsb);                              \
+    a.d = __builtin_fma(a.z, S, a.d);                                     \
   }
+  if constexpr (SER) {
+    const DSer k1 = dser(c, L1), k2 = dser(c, L2);
+    TP_STEP_BACK(drift_xy_back_ser(s, a, c, L1, k1, azB), drift_xy_back_ser(s, a, c, L2, k2, azB))
+  } else {
+    TP_STEP_BACK(drift_xy_back<false>(s, a, c, L1, azB), drift_xy_back<false>(s, a, c, L2, azB))
+  }
+#undef TP_STEP_BACK
   if (grad) {
     if constexpr (KK == TP_KICK_QUAD) gacc = __builtin_fma(K1, g1, __builtin_fma(K2, g2, gacc));
     if constexpr (KK == TP_KICK_SEXT) gacc = gacc + g1 / b;
