@@ -374,12 +374,14 @@ TP_DEV void esbend_dispatch(St* s, const Cx* c, const double* __restrict__ p, in
 // SER_OK: the launch may use the series drift (inv_pz), which pays where several waves share
 // a SIMD; it holds two more values per particle and two per element, so the variants with
 // general kicks (up to 22 coefficients in registers) use the reciprocal square root always.
-template <bool EXH, unsigned FEAT, int PPL, bool SER_OK>
+// DCH: the element may change delta (changes_delta); with DCH = false the code that renews
+// the delta hoists (the RF kick, the other families, misalignments) is not compiled in.
+template <bool EXH, unsigned FEAT, int PPL, bool SER_OK, bool DCH = true>
 TP_DEV void apply_elem(St* s, Cx* c, EdgeMemo* mm, const tp_rec& rec, const double* __restrict__ pool, double beti) {
   constexpr bool SER = SER_OK && (FEAT & FT_GENKICK) == 0;
   const double* __restrict__ p = pool + rec.off;
   const uint32_t fl = rec.flags;
-  if constexpr ((FEAT & FT_MISALIGN) != 0) {
+  if constexpr (DCH && (FEAT & FT_MISALIGN) != 0) {
     if (fl & TP_F_MISALIGN) {
       const double* mb = pool + rec.moff;
 #pragma unroll
@@ -485,7 +487,7 @@ TP_DEV void apply_elem(St* s, Cx* c, EdgeMemo* mm, const tp_rec& rec, const doub
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
           if (dr) drift<EXH, SER>(s[k], c[k], p[0], p[1]);
-          if (fl & TP_F_RF_KICK) {
+          if (DCH && (fl & TP_F_RF_KICK)) {
             const double phase = p[3] * ((s[k].z - p[4]) / 2.99792458e8) - p[5];
             s[k].d = s[k].d - p[2] * sin_lib(phase) / p[6];
             mkctx(s[k], c[k], beti);
@@ -497,7 +499,7 @@ TP_DEV void apply_elem(St* s, Cx* c, EdgeMemo* mm, const tp_rec& rec, const doub
     case TP_MARKER:
       break;
     default:  // the other element families, out of line (tp_extra.h)
-      if constexpr ((FEAT & FT_EXTRA) != 0) {
+      if constexpr (DCH && (FEAT & FT_EXTRA) != 0) {
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
           s[k] = extra_map<EXH>(s[k], rec.tag, fl, ns, p, beti);
@@ -506,7 +508,7 @@ TP_DEV void apply_elem(St* s, Cx* c, EdgeMemo* mm, const tp_rec& rec, const doub
       }
       break;
   }
-  if constexpr ((FEAT & FT_MISALIGN) != 0) {
+  if constexpr (DCH && (FEAT & FT_MISALIGN) != 0) {
     if (fl & TP_F_MISALIGN) {
       const double* mb = pool + rec.moff;
 #pragma unroll
@@ -519,12 +521,60 @@ TP_DEV void apply_elem(St* s, Cx* c, EdgeMemo* mm, const tp_rec& rec, const doub
   }
 }
 
-// check_lost (src/tracking.jl:49-52) plus the circular aperture; the aperture sum is
-// rounded explicitly so both builds take identical loss decisions.
-TP_DEV bool is_lost(const St& s, const TpKArgs& a) {
-  bool l = isnan(s.x) || fabs(s.x) > a.coord_limit || fabs(s.y) > a.coord_limit ||
-           fabs(s.px) > a.angle_limit || fabs(s.py) > a.angle_limit;
-  if (a.ap_r2 > 0.0) l = l || (__dadd_rn(__dmul_rn(s.x, s.x), __dmul_rn(s.y, s.y)) > a.ap_r2);
+// Whether an element changes delta, and with it the hoists in Cx: the RF kick, the other
+// families (their maps end with mkctx) and misaligned elements (t1 / t2 / r1 / r2 may move
+// delta).  Wave-uniform: a test of the record on the scalar unit.
+template <unsigned FEAT>
+TP_DEV bool changes_delta(const tp_rec& rec) {
+  if ((FEAT & FT_RF) != 0 && rec.tag == TP_RFCAV) return (rec.flags & TP_F_RF_KICK) != 0;
+  if ((FEAT & FT_EXTRA) != 0 && rec.tag >= TP_SOLENOID) return true;
+  if ((FEAT & FT_MISALIGN) != 0 && (rec.flags & TP_F_MISALIGN) != 0) return true;
+  return false;
+}
+
+// An element that changes delta.  Without misalignments only the RF kick and the other
+// families do, so only their cases are compiled here.
+template <bool EXH, unsigned FEAT, int PPL, bool SER_OK>
+TP_DEV void apply_elem_dch(St* s, Cx* c, EdgeMemo* mm, const tp_rec& rec, const double* __restrict__ pool,
+                           double beti) {
+  if constexpr ((FEAT & FT_MISALIGN) != 0) {
+    apply_elem<EXH, FEAT, PPL, SER_OK, true>(s, c, mm, rec, pool, beti);
+  } else {
+    constexpr bool SER = SER_OK && (FEAT & FT_GENKICK) == 0;
+    const double* __restrict__ p = pool + rec.off;
+    const uint32_t fl = rec.flags;
+    if ((FEAT & FT_RF) != 0 && rec.tag == TP_RFCAV) {  // src/tracking.jl:502-518
+      const bool dr = (fl & TP_F_RF_DRIFT) != 0;
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        if (dr) drift<EXH, SER>(s[k], c[k], p[0], p[1]);
+        const double phase = p[3] * ((s[k].z - p[4]) / 2.99792458e8) - p[5];
+        s[k].d = s[k].d - p[2] * sin_lib(phase) / p[6];
+        mkctx(s[k], c[k], beti);
+        if (dr) drift<EXH, SER>(s[k], c[k], p[0], p[1]);
+      }
+    } else if constexpr ((FEAT & FT_EXTRA) != 0) {  // the other families (tp_extra.h)
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) {
+        s[k] = extra_map<EXH>(s[k], rec.tag, fl, rec.nsteps, p, beti);
+        mkctx(s[k], c[k], beti);
+      }
+    }
+  }
+}
+
+// check_lost (src/tracking.jl:49-52): isnan(x) || |x| > 1 || |y| > 1 || |px| > 1 || |py| > 1
+// (limits from the table, validated positive), and the optional circular aperture (ap: the
+// table has one; wave-uniform).  isnan(x) || |x| > CL is !(|x| <= CL); the other tests are
+// ordered comparisons, false for NaN as in the reference.  The results are combined
+// without short circuits: four compares whose lane masks are ORed on the scalar unit.  The
+// aperture sum is rounded explicitly so both builds take identical loss decisions.
+TP_DEV bool is_lost(const St& s, const TpKArgs& a, bool ap) {
+  bool l = !(fabs(s.x) <= a.coord_limit);
+  l |= fabs(s.y) > a.coord_limit;
+  l |= fabs(s.px) > a.angle_limit;
+  l |= fabs(s.py) > a.angle_limit;
+  if (ap) l |= __dadd_rn(__dmul_rn(s.x, s.x), __dmul_rn(s.y, s.y)) > a.ap_r2;
   return l;
 }
 
@@ -566,6 +616,7 @@ TP_DEV void run_turns(St* s, Cx* c, EdgeMemo* mm, bool* alive, const int32_t* __
                       const double* __restrict__ pool, double* __restrict__ coords, int32_t* __restrict__ lost_turn,
                       int32_t* __restrict__ lost_elem, const TpKArgs& a, const int64_t* idx, int32_t t_begin,
                       int32_t t_end) {
+  const bool ap = __builtin_amdgcn_readfirstlane(a.ap_r2 > 0.0 ? 1 : 0) != 0;
   for (int32_t t = t_begin; t < t_end; ++t) {
     bool anyl = false;
 #pragma unroll
@@ -581,21 +632,14 @@ TP_DEV void run_turns(St* s, Cx* c, EdgeMemo* mm, bool* alive, const int32_t* __
     // round trip of scalar loads (its parameters) instead of three.
     int32_t ri_next = eb + 1 < ee ? seq_t[eb + 1] : 0;
     tp_rec rec_next = eb < ee ? recs[seq_t[eb]] : tp_rec{};
-    for (int32_t e = eb; e < ee; ++e) {
-      const tp_rec rec = rec_next;
-      if (e + 1 < ee) {
-        rec_next = recs[ri_next];
-        if (e + 2 < ee) ri_next = seq_t[e + 2];
-      }
-      St s0[PPL];
-#pragma unroll
-      for (int k = 0; k < PPL; ++k) s0[k] = s[k];
-      apply_elem<EXH, FEAT, PPL, SER_OK>(s, c, mm, rec, pool, a.beti);
+    // After an element: a particle lost there has its entry state s0, the turn and the
+    // position written out.
+    auto settle = [&](const St* s0, int32_t e) {
       bool lostnow[PPL];
       bool anylost = false;
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
-        lostnow[k] = alive[k] && is_lost(s[k], a);
+        lostnow[k] = alive[k] & is_lost(s[k], a, ap);
         anylost = anylost || lostnow[k];
       }
       if (__any(anylost)) {
@@ -606,13 +650,48 @@ TP_DEV void run_turns(St* s, Cx* c, EdgeMemo* mm, bool* alive, const int32_t* __
             lost_turn[idx[k]] = a.turn0 + t;
             lost_elem[idx[k]] = e;
             alive[k] = false;
-            // the dead lane goes on as the reference particle: no NaN left in it to send
-            // its wave through the slow paths that only NaN or large angles take
+            // the dead lane goes on from the zero state: no NaN left in it to send its wave
+            // through the slow paths that only NaN or large angles take.  Its delta hoists
+            // (c) stay those of the particle's last delta, finite unless its delta was,
+            // until the next element that changes delta renews them.  A dead lane's values
+            // never reach an output.
             s[k] = St{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            mkctx(s[k], c[k], a.beti);
           }
         }
       }
+    };
+    int32_t e = eb;
+    while (e < ee) {
+      // Elements that leave delta unchanged: the delta hoists (c) are invariant in this loop,
+      // so they keep their registers; an element that renews them (the RF kick, misaligned
+      // elements, the other families) ends the loop and runs below.  Renewing the hoists in
+      // the loop's own dispatch makes every iteration move them out of and back into their
+      // registers (16 moves per element).
+      for (; e < ee; ++e) {
+        const tp_rec rec = rec_next;
+        if (changes_delta<FEAT>(rec)) break;
+        if (e + 1 < ee) {
+          rec_next = recs[ri_next];
+          if (e + 2 < ee) ri_next = seq_t[e + 2];
+        }
+        St s0[PPL];
+#pragma unroll
+        for (int k = 0; k < PPL; ++k) s0[k] = s[k];
+        apply_elem<EXH, FEAT, PPL, SER_OK, false>(s, c, mm, rec, pool, a.beti);
+        settle(s0, e);
+      }
+      if (e >= ee) break;
+      const tp_rec rec = rec_next;
+      if (e + 1 < ee) {
+        rec_next = recs[ri_next];
+        if (e + 2 < ee) ri_next = seq_t[e + 2];
+      }
+      St s0[PPL];
+#pragma unroll
+      for (int k = 0; k < PPL; ++k) s0[k] = s[k];
+      apply_elem_dch<EXH, FEAT, PPL, SER_OK>(s, c, mm, rec, pool, a.beti);
+      settle(s0, e);
+      ++e;
     }
   }
 }
