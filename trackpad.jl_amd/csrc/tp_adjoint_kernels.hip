@@ -199,7 +199,9 @@ TP_DEV void sub6(St& s, const double* __restrict__ t) {
 // One element backwards: s is the element's output on entry, its input on return.  minv
 // holds the inverses of the r1 / r2 matrices at the pool offsets of the matrices (variant
 // with FT_MISALIGN); lin selects the linearised drift (variant with FT_LINH).
-template <unsigned FEAT, bool SER_OK>
+// DCH: the element may change delta (changes_delta); with DCH = false the code that renews
+// the delta hoists (the RF kick, the other families, misalignments) is not compiled in.
+template <unsigned FEAT, bool SER_OK, bool DCH = true>
 TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, EdgeMemo* mm, const tp_rec& rec, const double* __restrict__ pool,
                             const double* __restrict__ minv, double beti, bool lin, double& gacc) {
   // the series drift needs two more hoisted values; kernels with general kicks never take it
@@ -207,7 +209,7 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, EdgeMemo* mm, const tp_rec& rec
   const double* __restrict__ p = pool + rec.off;
   const uint32_t fl = rec.flags;
   const int m = rec.max_order, ns = rec.nsteps, g = rec.gorder;
-  if (fl & (TP_F_T2 | TP_F_R2)) {
+  if (DCH && (fl & (TP_F_T2 | TP_F_R2))) {
     if (fl & TP_F_T2) sub6(s, pool + rec.moff + 6);
     if constexpr ((FEAT & FT_MISALIGN) != 0) {
       if (fl & TP_F_R2) mat6_back(s, a, pool + rec.moff + 48, minv + rec.moff + 48);
@@ -278,7 +280,7 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, EdgeMemo* mm, const tp_rec& rec
       if constexpr ((FEAT & FT_RF) != 0) {
         const bool dr = (fl & TP_F_RF_DRIFT) != 0;
         if (dr) drift_back<SER>(s, a, c, p[0], p[1], lin);
-        if (fl & TP_F_RF_KICK) {
+        if (DCH && (fl & TP_F_RF_KICK)) {
           rf_kick_back(s, a, p);
           mkctx(s, c, beti);
         }
@@ -288,7 +290,7 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, EdgeMemo* mm, const tp_rec& rec
     case TP_MARKER:
       break;
     default:  // the other families (tags TP_SOLENOID ..; the host refuses the wiggler)
-      if constexpr ((FEAT & FT_EXTRA) != 0) {
+      if constexpr (DCH && (FEAT & FT_EXTRA) != 0) {
         const SA r = extra_back(s, a, rec.tag, fl, ns, p, beti, lin);
         s = r.s;
         a = r.a;
@@ -296,12 +298,39 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, EdgeMemo* mm, const tp_rec& rec
       }
       break;
   }
-  if (fl & (TP_F_T1 | TP_F_R1)) {
+  if (DCH && (fl & (TP_F_T1 | TP_F_R1))) {
     if constexpr ((FEAT & FT_MISALIGN) != 0) {
       if (fl & TP_F_R1) mat6_back(s, a, pool + rec.moff + 12, minv + rec.moff + 12);
     }
     if (fl & TP_F_T1) sub6(s, pool + rec.moff);
     mkctx(s, c, beti);
+  }
+}
+
+// An element that changes delta, backwards.  Without misalignments only the RF kick and the
+// other families do, so only their cases are compiled here.
+template <unsigned FEAT, bool SER_OK>
+TP_DEV void apply_elem_back_dch(St& s, Ad& a, Cx& c, EdgeMemo* mm, const tp_rec& rec,
+                                const double* __restrict__ pool, const double* __restrict__ minv, double beti,
+                                bool lin, double& gacc) {
+  if constexpr ((FEAT & FT_MISALIGN) != 0) {
+    apply_elem_back<FEAT, SER_OK, true>(s, a, c, mm, rec, pool, minv, beti, lin, gacc);
+  } else {
+    constexpr bool SER = SER_OK && (FEAT & FT_GENKICK) == 0;
+    const double* __restrict__ p = pool + rec.off;
+    const uint32_t fl = rec.flags;
+    if ((FEAT & FT_RF) != 0 && rec.tag == TP_RFCAV) {
+      const bool dr = (fl & TP_F_RF_DRIFT) != 0;
+      if (dr) drift_back<SER>(s, a, c, p[0], p[1], lin);
+      rf_kick_back(s, a, p);
+      mkctx(s, c, beti);
+      if (dr) drift_back<SER>(s, a, c, p[0], p[1], lin);
+    } else if constexpr ((FEAT & FT_EXTRA) != 0) {
+      const SA r = extra_back(s, a, rec.tag, fl, rec.nsteps, p, beti, lin);
+      s = r.s;
+      a = r.a;
+      mkctx(s, c, beti);
+    }
   }
 }
 
@@ -324,20 +353,40 @@ TP_DEV void back_turns(St& s, Ad& l, Cx& c, EdgeMemo* mm, const int32_t* __restr
     int32_t ri_next = n_seq > 1 ? seq[n_seq - 2] : 0;
     tp_rec rec_next = n_seq > 0 ? recs[seq[n_seq - 1]] : tp_rec{};
     int32_t slot_next = (slot_of_pos && n_seq > 0) ? slot_of_pos[n_seq - 1] : -1;
-    for (int32_t e = n_seq - 1; e >= 0; --e) {
-      const tp_rec rec = rec_next;
-      const int32_t slot = slot_next;
+    int32_t e = n_seq - 1;
+    auto advance = [&]() {
       if (e >= 1) {
         rec_next = recs[ri_next];
         slot_next = slot_of_pos ? slot_of_pos[e - 1] : -1;
         if (e >= 2) ri_next = seq[e - 2];
       }
-      double gacc = 0.0;
-      apply_elem_back<FEAT, SER_OK>(s, l, c, mm, rec, pool, minv, beti, lin, gacc);
+    };
+    auto add_grad = [&](int32_t slot, double gacc) {
       if (slot >= 0) {
         const double w = wave_sum(gacc);
         if (lane == 0) row[slot] += w;
       }
+    };
+    while (e >= 0) {
+      // elements that leave delta unchanged: the delta hoists are invariant in this loop
+      // (changes_delta); an element that renews them ends it and runs below
+      for (; e >= 0; --e) {
+        const tp_rec rec = rec_next;
+        const int32_t slot = slot_next;
+        if (changes_delta<FEAT>(rec)) break;
+        advance();
+        double gacc = 0.0;
+        apply_elem_back<FEAT, SER_OK, false>(s, l, c, mm, rec, pool, minv, beti, lin, gacc);
+        add_grad(slot, gacc);
+      }
+      if (e < 0) break;
+      const tp_rec rec = rec_next;
+      const int32_t slot = slot_next;
+      advance();
+      double gacc = 0.0;
+      apply_elem_back_dch<FEAT, SER_OK>(s, l, c, mm, rec, pool, minv, beti, lin, gacc);
+      add_grad(slot, gacc);
+      --e;
     }
   }
 }

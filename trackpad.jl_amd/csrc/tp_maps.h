@@ -524,10 +524,44 @@ TP_DEV void ebody_driftlimit(St& s, const Cx& c, double L, double Lb) {
 
 // Sum over the 64 lanes of a wave (butterfly; every lane gets the same total, formed in
 // the same order whatever the wave's position in the grid).
+// One DPP move of a double (two 32-bit halves): lanes whose source is outside the row, or
+// whose row is masked off, read zero.
+template <int CTRL, int ROWS>
+TP_DEV double dpp_f64(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWS, 0xf, true);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xf, true);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+// Sum over the 64 lanes of a wave (every lane active), returned to every lane: sums within
+// quads (quad_perm xor 1, xor 2), within rows of 16 (row_shr 4, 8: lane 15 of each row holds
+// the row's sum), across rows (row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2 and
+// 3: lane 63 holds the total), then lane 63 read back as a wave-uniform value.  A fixed
+// order of additions, so the result is bit-reproducible.
 TP_DEV double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v = v + dpp_f64<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+  v = v + dpp_f64<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+  v = v + dpp_f64<0x114, 0xf>(v);  // row_shr 4
+  v = v + dpp_f64<0x118, 0xf>(v);  // row_shr 8
+  v = v + dpp_f64<0x142, 0xa>(v);  // row_bcast 15
+  v = v + dpp_f64<0x143, 0xc>(v);  // row_bcast 31
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), 63);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Whether an element changes delta, and with it the delta hoists in Cx: the RF kick, the
+// other families (their maps end with mkctx) and misaligned elements (t1 / t2 / r1 / r2 may
+// move delta).  Wave-uniform: a test of the record on the scalar unit.  The element loops run
+// such elements outside their hot loop, where the hoists are invariant: renewing them in the
+// loop's own dispatch makes every iteration move them out of and back into their registers.
+template <unsigned FEAT>
+TP_DEV bool changes_delta(const tp_rec& rec) {
+  if ((FEAT & FT_RF) != 0 && rec.tag == TP_RFCAV) return (rec.flags & TP_F_RF_KICK) != 0;
+  if ((FEAT & FT_EXTRA) != 0 && rec.tag >= TP_SOLENOID) return true;
+  if ((FEAT & FT_MISALIGN) != 0 && (rec.flags & TP_F_MISALIGN) != 0) return true;
+  return false;
 }
 
 }  // namespace tpk

@@ -521,17 +521,6 @@ TP_DEV void apply_elem(St* s, Cx* c, EdgeMemo* mm, const tp_rec& rec, const doub
   }
 }
 
-// Whether an element changes delta, and with it the hoists in Cx: the RF kick, the other
-// families (their maps end with mkctx) and misaligned elements (t1 / t2 / r1 / r2 may move
-// delta).  Wave-uniform: a test of the record on the scalar unit.
-template <unsigned FEAT>
-TP_DEV bool changes_delta(const tp_rec& rec) {
-  if ((FEAT & FT_RF) != 0 && rec.tag == TP_RFCAV) return (rec.flags & TP_F_RF_KICK) != 0;
-  if ((FEAT & FT_EXTRA) != 0 && rec.tag >= TP_SOLENOID) return true;
-  if ((FEAT & FT_MISALIGN) != 0 && (rec.flags & TP_F_MISALIGN) != 0) return true;
-  return false;
-}
-
 // An element that changes delta.  Without misalignments only the RF kick and the other
 // families do, so only their cases are compiled here.
 template <bool EXH, unsigned FEAT, int PPL, bool SER_OK>
@@ -663,10 +652,8 @@ TP_DEV void run_turns(St* s, Cx* c, EdgeMemo* mm, bool* alive, const int32_t* __
     int32_t e = eb;
     while (e < ee) {
       // Elements that leave delta unchanged: the delta hoists (c) are invariant in this loop,
-      // so they keep their registers; an element that renews them (the RF kick, misaligned
-      // elements, the other families) ends the loop and runs below.  Renewing the hoists in
-      // the loop's own dispatch makes every iteration move them out of and back into their
-      // registers (16 moves per element).
+      // so they keep their registers; an element that renews them (changes_delta) ends the
+      // loop and runs below.
       for (; e < ee; ++e) {
         const tp_rec rec = rec_next;
         if (changes_delta<FEAT>(rec)) break;
