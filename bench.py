@@ -88,7 +88,7 @@ MEASURED_BYTES_PER_PARTICLE_LAUNCH = {("dba", 0): (25576.6 * _FETCH_SCALE + 4745
 # tile-turn): fewer than the reference's 313,587 because the fast build does not repeat work
 # (pure-dipole bend sub-bodies as rotations, drifts as a two-FMA series, the edges' fringe
 # term reused across the ring's identical bends).
-MEASURED_EXECUTED_FLOPS_PER_PARTICLE_TURN = {("dba", "fast"): 168_764}
+MEASURED_EXECUTED_FLOPS_PER_PARTICLE_TURN = {("dba", "fast"): 167_144}
 
 
 def visible_cores() -> int:

@@ -554,17 +554,22 @@ TP_DEV void apply_elem_dch(St* s, Cx* c, EdgeMemo* mm, const tp_rec& rec, const 
 
 // check_lost (src/tracking.jl:49-52): isnan(x) || |x| > 1 || |y| > 1 || |px| > 1 || |py| > 1
 // (limits from the table, validated positive), and the optional circular aperture (ap: the
-// table has one; wave-uniform).  isnan(x) || |x| > CL is !(|x| <= CL); the other tests are
-// ordered comparisons, false for NaN as in the reference.  The results are combined
-// without short circuits: four compares whose lane masks are ORed on the scalar unit.  The
-// aperture sum is rounded explicitly so both builds take identical loss decisions.
-TP_DEV bool is_lost(const St& s, const TpKArgs& a, bool ap) {
-  bool l = !(fabs(s.x) <= a.coord_limit);
-  l |= fabs(s.y) > a.coord_limit;
-  l |= fabs(s.px) > a.angle_limit;
-  l |= fabs(s.py) > a.angle_limit;
-  if (ap) l |= __dadd_rn(__dmul_rn(s.x, s.x), __dmul_rn(s.y, s.y)) > a.ap_r2;
-  return l;
+// table has one; wave-uniform), as the wave's lane mask.  isnan(x) || |x| > CL is
+// !(|x| <= CL); the other tests are ordered comparisons, false for NaN as in the reference.
+// Each compare's lane mask is ORed on the scalar unit, so the test costs its four compares.
+// The aperture sum is rounded explicitly so both builds take identical loss decisions.
+TP_DEV uint64_t lost_mask(const St& s, const TpKArgs& a, bool ap) {
+  uint64_t m = __builtin_amdgcn_ballot_w64(!(fabs(s.x) <= a.coord_limit));
+  m |= __builtin_amdgcn_ballot_w64(fabs(s.y) > a.coord_limit);
+  m |= __builtin_amdgcn_ballot_w64(fabs(s.px) > a.angle_limit);
+  m |= __builtin_amdgcn_ballot_w64(fabs(s.py) > a.angle_limit);
+  if (ap) {
+    // an empty volatile statement keeps this a branch on the uniform flag: otherwise the
+    // compiler computes the aperture test for every element of tables without one
+    __asm__ volatile("");
+    m |= __builtin_amdgcn_ballot_w64(__dadd_rn(__dmul_rn(s.x, s.x), __dmul_rn(s.y, s.y)) > a.ap_r2);
+  }
+  return m;
 }
 
 // Loads PPL particles per lane (idx / valid per particle); lanes past the beam's end and
@@ -606,11 +611,16 @@ TP_DEV void run_turns(St* s, Cx* c, EdgeMemo* mm, bool* alive, const int32_t* __
                       int32_t* __restrict__ lost_elem, const TpKArgs& a, const int64_t* idx, int32_t t_begin,
                       int32_t t_end) {
   const bool ap = __builtin_amdgcn_readfirstlane(a.ap_r2 > 0.0 ? 1 : 0) != 0;
-  for (int32_t t = t_begin; t < t_end; ++t) {
-    bool anyl = false;
+  // alive[k] as the wave's lane mask (scalar registers): the loss test of an element is then
+  // its compares and scalar mask logic, with no per-lane flag to test
+  uint64_t am[PPL];
 #pragma unroll
-    for (int k = 0; k < PPL; ++k) anyl = anyl || alive[k];
-    if (!__any(anyl)) break;
+  for (int k = 0; k < PPL; ++k) am[k] = __builtin_amdgcn_ballot_w64(alive[k]);
+  for (int32_t t = t_begin; t < t_end; ++t) {
+    uint64_t anyl = 0;
+#pragma unroll
+    for (int k = 0; k < PPL; ++k) anyl |= am[k];
+    if (anyl == 0) break;
     // the lattice of this turn: row (turn - row_turn0) of a time-varying table
     const int32_t* __restrict__ seq_t = seq + (a.n_rows > 1 ? (int64_t)(a.turn0 + t - a.row_turn0) * a.n_seq : 0);
     // elements of this turn: a launch may start inside its first turn and end inside its last
@@ -624,21 +634,22 @@ TP_DEV void run_turns(St* s, Cx* c, EdgeMemo* mm, bool* alive, const int32_t* __
     // After an element: a particle lost there has its entry state s0, the turn and the
     // position written out.
     auto settle = [&](const St* s0, int32_t e) {
-      bool lostnow[PPL];
-      bool anylost = false;
+      uint64_t lm[PPL];
+      uint64_t anylost = 0;
 #pragma unroll
       for (int k = 0; k < PPL; ++k) {
-        lostnow[k] = alive[k] & is_lost(s[k], a, ap);
-        anylost = anylost || lostnow[k];
+        lm[k] = lost_mask(s[k], a, ap) & am[k];
+        anylost |= lm[k];
       }
-      if (__any(anylost)) {
+      if (anylost != 0) {
+        const int lane = lane_id();
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
-          if (lostnow[k]) {
+          am[k] &= ~lm[k];
+          if ((lm[k] >> lane) & 1) {
             store_state(coords + idx[k], a.ld, s0[k]);
             lost_turn[idx[k]] = a.turn0 + t;
             lost_elem[idx[k]] = e;
-            alive[k] = false;
             // the dead lane goes on from the zero state: no NaN left in it to send its wave
             // through the slow paths that only NaN or large angles take.  Its delta hoists
             // (c) stay those of the particle's last delta, finite unless its delta was,
@@ -681,6 +692,9 @@ TP_DEV void run_turns(St* s, Cx* c, EdgeMemo* mm, bool* alive, const int32_t* __
       ++e;
     }
   }
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < PPL; ++k) alive[k] = ((am[k] >> lane) & 1) != 0;
 }
 
 // Per-wave LDS of the tracking kernels (dynamic, sized by the launch: TP_LDS_PER_WAVE(ppl)
