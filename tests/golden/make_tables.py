@@ -17,7 +17,7 @@ sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "trackpad.jl_amd
 import math  # noqa: E402
 
 import lattices as LT  # noqa: E402
-from trackpad_hip import Beam, Lattice, Quadrupole, Sextupole, compile_table, timed  # noqa: E402
+from trackpad_hip import Beam, Lattice, Quadrupole, Sextupole, compile_table, read_madx, timed, write_madx  # noqa: E402
 from trackpad_hip import workloads as W  # noqa: E402
 
 # The turn range of the time-varying golden table: rows for turns 5..8, the first at
@@ -38,6 +38,18 @@ def dba_ring_time_varying():
     return Lattice(els, name="DBA_TV")
 
 
+MADX = os.path.join(HERE, "madx")
+
+
+def madx_files():
+    """MAD-X text the Julia reader is checked on: the DBA workload ring as the Python writer
+    prints it.  (madx/hand_cell.madx, a hand-written cell with variables, deferred expressions,
+    inheritance, a sequence with refer = centre and from =, an RBEND given by its chord, a
+    multipole, a monitor, a kicker and a cavity given by its harmonic number, is written by
+    hand; its table is generated below.)"""
+    yield "dba_workload.madx", write_madx(W.dba_ring(), Beam(W.ENERGY))
+
+
 def cases():
     b35 = Beam(3.5e9)
     yield "verify_fodo_line", compile_table(LT.verify_fodo_line(), b35)
@@ -49,10 +61,16 @@ def cases():
     yield "families_ring", compile_table(LT.families_ring(), Beam(3e9))
     yield "dba_time_varying", compile_table(dba_ring_time_varying(), Beam(W.ENERGY), turn=TV_TURN,
                                             nturns=TV_NTURNS, time=TV_TIME, dt_turn=TV_DT)
+    yield "madx_hand_cell", compile_table(*read_madx(os.path.join(MADX, "hand_cell.madx"), bend="ExactSBendLinearEdge"))
 
 
 def main():
     os.makedirs(os.path.join(HERE, "tables"), exist_ok=True)
+    os.makedirs(MADX, exist_ok=True)
+    for name, text in madx_files():
+        with open(os.path.join(MADX, name), "w") as f:
+            f.write(text)
+        print(name, len(text))
     for name, blob in cases():
         with open(os.path.join(HERE, "tables", name + ".bin"), "wb") as f:
             f.write(blob)

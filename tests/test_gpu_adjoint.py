@@ -249,5 +249,16 @@ def test_adjoint_diff_workload(ctx):
         k0 = lat[p].k1 if isinstance(lat[p], Quadrupole) else lat[p].k2
         fd = (loss_gpu(set_strengths(lat, [p], [k0 + h])) - loss_gpu(set_strengths(lat, [p], [k0 - h]))) / (2 * h)
         assert abs(g.dk[p] - fd) <= 2e-4 * abs(fd), (p, g.dk[p], fd)
+    # the whole gradient at full size along one random direction of (strengths, initial
+    # coordinates), against the oracle's exact forward-mode derivative: relative 1e-9
+    rng = np.random.default_rng(8)
+    dk = np.zeros(len(lat))
+    dk[pos] = rng.normal(0, 1, len(pos))
+    dr0 = rng.normal(0, 1, (n, 6)) * np.array([5e-5, 2e-5, 1e-5, 2e-6, 1e-4, 1e-3])
+    _, dl, na = O.loss_dir(lat, beam, c0, T, dk=dk, dr0=dr0)
+    assert na == n
+    ref = dl / na
+    got = _directional(g, dk, dr0)
+    assert abs(got - ref) <= 1e-9 * abs(ref), (got, ref)
     print(f"adjoint diff workload: forward {g.forward_ms:.1f} ms, reverse {g.backward_ms:.1f} ms, "
           f"ratio {g.backward_ms / max(g.forward_ms, 1e-9):.2f}")

@@ -2,6 +2,7 @@
 sequence file against the lattice built from constructors, and the refusals (CPU only)."""
 
 import math
+import os
 
 import numpy as np
 import pytest
@@ -54,50 +55,10 @@ def test_round_trip_same_table(which, tmp_path):
     assert beam3.energy == pytest.approx(beam.energy, rel=1e-15)
 
 
-SEQ = """
-// a DBA-like cell written by hand: variables, deferred expressions, inheritance,
-// element references, a sequence with refer = centre and from =, and a line
-title, "test cell";
-option, -echo;
-! strengths
-kf := 2.1 * scale;   scale = 1.0;
-kd = -2.4;
-lq = 0.3;
-const nb = 40;
-ang = twopi / nb;
-real ksd = -32.0; ksf = 27.0;
-
-qf: quadrupole, l = lq, k1 := kf;
-qd: qf, k1 = kd;                      /* derives from qf: l stays */
-qf2: quadrupole, l = 0.4, k1 = 0.5, nst = 6;
-sd: sextupole, l = 0.2, k2 = ksd;
-sf: sd, k2 = ksf;
-b: sbend, l = 1.5, angle = ang, fint = 0.5, hgap = 0.015;
-rb: rbend, l = 2 * 0.75 * sin(ang/2) / (ang/2), angle = ang;   ! chord length of a 1.5 m arc
-mp: multipole, knl = {0, 0, 0.2}, ksl = {0, 1e-3};
-bpm: monitor;
-kick: hkicker, l = 0.1, kick = 2e-5;
-rf: rfcavity, l = 0, volt = 1.0, harmon = 4, lag = 0;
-qf->k1 := kf * 1.0;
-
-cell: sequence, l = 10.45, refer = centre;
-  qf, at = 0.15;
-  qd, at = 0.7;
-  b, at = 1.3, from = qd;           ! centre 2.0 m
-  sd, at = 2.85;
-  mp, at = 2.95;
-  qf2, at = 3.25;
-  sf, at = 3.55;
-  bpm, at = 3.65;
-  sf2: sextupole, l = 0.2, k2 = 20.0, at = 3.85;
-  rb, at = 4.9;
-  kick, at = 6.0;
-  rf, at = 10.45;
-endsequence;
-
-beam, particle = electron, pc = 3.0;
-use, sequence = cell;
-"""
+# a hand-written cell (tests/golden/madx/hand_cell.madx): variables, deferred expressions,
+# inheritance, element references, a sequence with refer = centre and from =, and a line
+with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "madx", "hand_cell.madx")) as _f:
+    SEQ = _f.read()
 
 
 def test_hand_written_sequence():
@@ -176,3 +137,23 @@ def test_writer_refuses_what_madx_cannot_hold():
         write_madx(Lattice([SBend(1.0, 0.1, fringe_bend_entrance=3)]), beam)
     with pytest.raises(MadxError, match="one bend model"):
         write_madx(Lattice([SBend(1.0, 0.1), ExactSBend(1.0, 0.1)]), beam)
+
+
+def test_golden_madx_files():
+    """The MAD-X fixtures the Julia reader is checked on (tests/golden/madx): the writer still
+    prints the DBA ring as committed, and both files read back to the committed tables."""
+    import sys
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_tables
+
+    g = os.path.join(here, "golden")
+    with open(os.path.join(g, "madx", "dba_workload.madx")) as f:
+        assert f.read() == dict(make_tables.madx_files())["dba_workload.madx"]
+    lat, beam = read_madx(os.path.join(g, "madx", "dba_workload.madx"), energy=W.ENERGY)
+    with open(os.path.join(g, "tables", "dba_workload.bin"), "rb") as f:
+        assert compile_table(lat, beam) == f.read()
+    lat, beam = read_madx(os.path.join(g, "madx", "hand_cell.madx"), bend="ExactSBendLinearEdge")
+    with open(os.path.join(g, "tables", "madx_hand_cell.bin"), "rb") as f:
+        assert compile_table(lat, beam) == f.read()

@@ -18,7 +18,7 @@ using TrackPad: AbstractElement, Beam, Lattice, TimeVaryingElement
 
 export HIPContext, HIPTable, HIPParticles, HIPBatch, compile_table, track!, linepass_hip!, ringpass_hip!,
        ringpass_grad, ExactSBendLinearEdge, download, jacobian, autotune!, set_waves!, set_launch!, batch_grad,
-       LaunchCfg
+       LaunchCfg, read_madx, MadxError
 
 const LIBPATH = Ref{String}(get(ENV, "TRACKPAD_HIP_LIB",
                                 joinpath(@__DIR__, "..", "..", "..", "lib", "libtrackpad_hip.so")))
@@ -456,5 +456,6 @@ function autotune!(b::HIPBatch, probe_turns::Vector{Int32}; ckpt_stride::Integer
 end
 
 include("table.jl")
+include("madx.jl")
 
 end # module TrackPadHIP
