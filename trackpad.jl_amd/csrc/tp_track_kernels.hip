@@ -642,7 +642,7 @@ TP_DEV void run_turns(St* s, Cx* c, EdgeMemo* mm, bool* alive, const int32_t* __
         anylost |= lm[k];
       }
       if (anylost != 0) {
-        const int lane = lane_id();
+        const int lane = lane_id_here();  // not held across the element loop
 #pragma unroll
         for (int k = 0; k < PPL; ++k) {
           am[k] &= ~lm[k];
