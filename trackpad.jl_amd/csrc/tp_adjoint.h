@@ -163,17 +163,21 @@ TP_DEV void kick_back(St& s, Ad& a, const double* __restrict__ poly, int m, doub
 // per element and the a.z NormL terms of delta's adjoint once per step.
 // ---------------------------------------------------------------------------------------
 // drift_xy backwards: x and y undone; the adjoints of px, py and delta updated except the
-// a.z NormL term of delta.  azB = a.z B.  Returns NormL.
-template <bool SER>
+// a.z NormL term of delta.  azB = a.z B.  Returns NormL.  PAIR: two drifts of length L with
+// nothing between them (drift_xy_pair) undone at once -- neither the momenta nor a.x, a.y
+// change in a drift, so the second sees the first's NormL and f, and the pair is one drift
+// with NormL and its derivative doubled.
+template <bool SER, bool PAIR = false>
 TP_DEV double drift_xy_back(St& s, Ad& a, const Cx& c, double L, double azB) {
   const double rz = SER ? inv_pz(s, c, 1.0) : rsqrt_nr(c.A - s.px * s.px - s.py * s.py);
   const double NormL = L * rz;
-  s.x = s.x - NormL * s.px;
-  s.y = s.y - NormL * s.py;
+  const double Nm = PAIR ? NormL + NormL : NormL;
+  s.x = s.x - Nm * s.px;
+  s.y = s.y - Nm * s.py;
   // d NormL / d px = NormL px / pz^2, / d py likewise, / d delta = -NormL B / pz^2
-  const double f = __builtin_fma(a.x, s.px, __builtin_fma(a.y, s.py, azB)) * (NormL * (rz * rz));
-  a.px = a.px + a.x * NormL + f * s.px;
-  a.py = a.py + a.y * NormL + f * s.py;
+  const double f = __builtin_fma(a.x, s.px, __builtin_fma(a.y, s.py, azB)) * (Nm * (rz * rz));
+  a.px = a.px + a.x * Nm + f * s.px;
+  a.py = a.py + a.y * Nm + f * s.py;
   a.d = a.d - f * c.B;
   return NormL;
 }
@@ -194,6 +198,8 @@ TP_DEV DSer dser(const Cx& c, double L) {
   k.d2 = k.c2 * 4.0;
   return k;
 }
+// PAIR: two drifts of length L undone at once, as in drift_xy_back.
+template <bool PAIR = false>
 TP_DEV double drift_xy_back_ser(St& s, Ad& a, const Cx& c, double L, const DSer& k, double azB) {
   const double w = __builtin_fma(s.px, s.px, s.py * s.py);
   double NormL = __builtin_fma(w, __builtin_fma(w, k.c2, k.c1), k.Lr);
@@ -203,11 +209,13 @@ TP_DEV double drift_xy_back_ser(St& s, Ad& a, const Cx& c, double L, const DSer&
     NormL = L * rz;
     g = NormL * (rz * rz);
   }
-  s.x = s.x - NormL * s.px;
-  s.y = s.y - NormL * s.py;
-  const double f = __builtin_fma(a.x, s.px, __builtin_fma(a.y, s.py, azB)) * g;
-  a.px = a.px + a.x * NormL + f * s.px;
-  a.py = a.py + a.y * NormL + f * s.py;
+  const double Nm = PAIR ? NormL + NormL : NormL;
+  const double gm = PAIR ? g + g : g;
+  s.x = s.x - Nm * s.px;
+  s.y = s.y - Nm * s.py;
+  const double f = __builtin_fma(a.x, s.px, __builtin_fma(a.y, s.py, azB)) * gm;
+  a.px = a.px + a.x * Nm + f * s.px;
+  a.py = a.py + a.y * Nm + f * s.py;
   a.d = a.d - f * c.B;
   return NormL;
 }
@@ -253,28 +261,45 @@ TP_DEV void mpole_back_fast(St& s, Ad& a, const Cx& c, const double* __restrict_
   const double b = KK == TP_KICK_QUAD ? p[6 + 2] : (KK == TP_KICK_SEXT ? p[6 + 4] : 0.0);
   const double Kb1 = K1 * b, Kb2 = K2 * b;
   double g1 = 0.0, g2 = 0.0;
-#define TP_STEP_BACK(D1, D2)                                              \
-  for (int st = 0; st < ns; ++st) {                                       \
+  // The forward's steps (mpole_steps) undone from the exit: the half-drift pairs where
+  // steps meet undone as one (PAIR), each step's path sum formed as the forward formed it.
+#define TP_KICKS_BACK(D2)                                                 \
+  if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, Kb1, g1);        \
+  if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb1, g1);        \
+  S = S + D2;                                                             \
+  if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, Kb2, g2);        \
+  if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb2, g1);        \
+  S = S + D2;                                                             \
+  if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, Kb1, g1);        \
+  if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb1, g1);
+#define TP_STEP_BACK(D1, D1PAIR, D2)                                      \
+  {                                                                       \
     double S = D1;                                                        \
-    if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, Kb1, g1);      \
-    if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb1, g1);      \
-    S = S + D2;                                                           \
-    if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, Kb2, g2);      \
-    if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb2, g1);      \
-    S = S + D2;                                                           \
-    if constexpr (KK == TP_KICK_QUAD) kick_quad_back(s, a, Kb1, g1);      \
-    if constexpr (KK == TP_KICK_SEXT) kick_sext_back(s, a, Kb1, g1);      \
+    for (int st = 1; st < ns; ++st) {                                     \
+      TP_KICKS_BACK(D2)                                                   \
+      const double N = D1PAIR;                                            \
+      S = S + N;                                                          \
+      s.z = s.z - __builtin_fma(S, c.B, -Lsb);                            \
+      a.d = __builtin_fma(a.z, S, a.d);                                   \
+      S = N;                                                              \
+    }                                                                     \
+    TP_KICKS_BACK(D2)                                                     \
     S = S + D1;                                                           \
     s.z = s.z - __builtin_fma(S, c.B, -Lsb);                              \
     a.d = __builtin_fma(a.z, S, a.d);                                     \
   }
-  if constexpr (SER) {
-    const DSer k1 = dser(c, L1), k2 = dser(c, L2);
-    TP_STEP_BACK(drift_xy_back_ser(s, a, c, L1, k1, azB), drift_xy_back_ser(s, a, c, L2, k2, azB))
-  } else {
-    TP_STEP_BACK(drift_xy_back<false>(s, a, c, L1, azB), drift_xy_back<false>(s, a, c, L2, azB))
+  if (ns > 0) {
+    if constexpr (SER) {
+      const DSer k1 = dser(c, L1), k2 = dser(c, L2);
+      TP_STEP_BACK(drift_xy_back_ser(s, a, c, L1, k1, azB), drift_xy_back_ser<true>(s, a, c, L1, k1, azB),
+                   drift_xy_back_ser(s, a, c, L2, k2, azB))
+    } else {
+      TP_STEP_BACK(drift_xy_back<false>(s, a, c, L1, azB), (drift_xy_back<false, true>(s, a, c, L1, azB)),
+                   drift_xy_back<false>(s, a, c, L2, azB))
+    }
   }
 #undef TP_STEP_BACK
+#undef TP_KICKS_BACK
   if (grad) {
     if constexpr (KK == TP_KICK_QUAD) gacc = __builtin_fma(K1, g1, __builtin_fma(K2, g2, gacc));
     if constexpr (KK == TP_KICK_SEXT) gacc = gacc + g1 / b;

@@ -106,12 +106,11 @@ TP_DEV void esbend_body_back(St& s, Ad& a, const Cx& c, const double* __restrict
   const double* poly = p + 24;
 #if TP_FAST
   if constexpr (KK == TP_KICK_NONE) {
-    // Pure dipole: the forward kernel applies every sub-body as a rotation of (px, pzmx),
-    // pzmx = pz - 1 - x irho, with x formed at the end and the path terms telescoped (see
-    // esbend_body).  Undone as a whole: the inverse rotations in reverse order (the step is a
-    // palindrome) bring (px, pzmx) back to the element's entry, and, the rotations being
-    // orthogonal, the same matrices carry the adjoint of (px, pzmx) back -- one loop for
-    // both.  The element map, with B = beti + delta, dd = B^2 - 1 - py^2 and
+    // Pure dipole: the forward kernel applies the body as one rotation of (px, pzmx),
+    // pzmx = pz - 1 - x irho (its sub-bodies' rotations composed), with x formed at the end
+    // and the path terms telescoped (see esbend_body).  Undone as a whole: the inverse
+    // rotation brings (px, pzmx) back to the element's entry, and, the rotation being
+    // orthogonal, the same matrix carries the adjoint of (px, pzmx) back.  The element map, with B = beti + delta, dd = B^2 - 1 - py^2 and
     // pz(a) = sqrt(B^2 - a^2 - py^2):
     //   b0 = pz(px) - 1 - x irho, (aN, bN) = R (px, b0), x' = (pz(aN) - 1 - bN) rho,
     //   S = Ltot + rho (asin(px/d2) - asin(aN/d2)), y' = y + py S, z' = z + B S - Lbtot,
@@ -129,19 +128,14 @@ TP_DEV void esbend_body_back(St& s, Ad& a, const Cx& c, const double* __restrict
       const double apzN = a.x * rho;          // x' = (pzN - 1 - bN) rho
       double ga = a.px - adif * ipzN - apzN * aN * ipzN;  // d/d aN
       double gb = -apzN;                                   // d/d bN
-      double ra = aN, rb = bN;
-      for (int st = 0; st < ns; ++st) {
-#define TP_IROT(CS, SN)                                   \
-  {                                                       \
-    const double ra_ = __builtin_fma(ra, CS, -(rb * SN)); \
-    rb = __builtin_fma(ra, SN, rb * CS);                  \
-    ra = ra_;                                             \
-    const double ga_ = __builtin_fma(ga, CS, -(gb * SN)); \
-    gb = __builtin_fma(ga, SN, gb * CS);                  \
-    ga = ga_;                                             \
-  }
-        TP_IROT(c1, s1) TP_IROT(c2, s2) TP_IROT(c2, s2) TP_IROT(c1, s1)
-#undef TP_IROT
+      // the body's one rotation (rot_body, as the forward applies it) undone
+      const Rot R = rot_body(c1, s1, c2, s2, ns);
+      const double ra = __builtin_fma(aN, R.c, -(bN * R.s));
+      const double rb = __builtin_fma(aN, R.s, bN * R.c);
+      {
+        const double ga_ = __builtin_fma(ga, R.c, -(gb * R.s));
+        gb = __builtin_fma(ga, R.s, gb * R.c);
+        ga = ga_;
       }
       double ipz0;
       const double pz0 = pxyz_r(c, ra, py, ipz0);

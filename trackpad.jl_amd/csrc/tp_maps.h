@@ -209,6 +209,39 @@ TP_DEV double drift_xy(St& s, const Cx& c, double L) {
   s.y = s.y + NormL * s.py;
   return NormL;
 }
+// Two drifts of length L with nothing between them (the last half-drift of a Yoshida step
+// and the first of the next): the momenta, and so NormL, are the same for both, and
+// x + 2 NormL px replaces (x + NormL px) + NormL px with one rounding less.  Returns the NormL
+// of one drift.
+template <bool SER>
+TP_DEV double drift_xy_pair(St& s, const Cx& c, double L) {
+  const double NormL = SER ? inv_pz(s, c, L) : L * rsqrt_nr(c.A - s.px * s.px - s.py * s.py);
+  const double N2 = NormL + NormL;
+  s.x = __builtin_fma(N2, s.px, s.x);
+  s.y = __builtin_fma(N2, s.py, s.y);
+  return NormL;
+}
+
+// Rotation by an angle as its (cos, sin) pair; rot_mul composes two (the angles add).
+struct Rot {
+  double c, s;
+};
+TP_DEV Rot rot_mul(const Rot& u, const Rot& v) {
+  return Rot{__builtin_fma(u.c, v.c, -(u.s * v.s)), __builtin_fma(u.s, v.c, u.c * v.s)};
+}
+// The rotation of a pure-dipole exact bend's whole body: each Yoshida step rotates by
+// 2 phi1 + 2 phi2 (sub-bodies L1 L2 L2 L1, cos/sin in the record), ns steps in all; the
+// power by repeated squaring.  Wave-uniform values.
+TP_DEV Rot rot_body(double c1, double s1, double c2, double s2, int ns) {
+  const Rot h = rot_mul(Rot{c1, s1}, Rot{c2, s2});
+  Rot st = rot_mul(h, h);
+  Rot r{1.0, 0.0};
+  for (int n = ns; n > 0; n >>= 1) {
+    if (n & 1) r = rot_mul(r, st);
+    if (n > 1) st = rot_mul(st, st);
+  }
+  return r;
+}
 #endif
 
 // ---------------------------------------------------------------------------------------

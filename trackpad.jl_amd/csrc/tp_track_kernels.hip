@@ -41,40 +41,49 @@ TP_DEV void mpole_steps(St* s, const Cx* c, const double* __restrict__ p, int ns
   const double* poly = p + 6;
 #if TP_FAST
   if constexpr (EXH) {
-    // path length summed per Yoshida step (drift_xy)
+    // Path length summed per Yoshida step (drift_xy).  Consecutive steps meet at two
+    // half-drifts of length L1 with no kick between them (... K1 D1 | D1 K1 ...): they see the
+    // same momenta, so they are applied as one (drift_xy_pair), and the pair's NormL is
+    // split between the two steps' path sums.
     const double Lsb = 2.0 * L1b + 2.0 * L2b;
-#define TP_YSTEP(k, KICK1, KICK2)                                 \
-  {                                                               \
-    double S_ = drift_xy<SER>(s[k], c[k], L1);                  \
-    KICK1;                                                        \
-    S_ = S_ + drift_xy<SER>(s[k], c[k], L2);                    \
-    KICK2;                                                        \
-    S_ = S_ + drift_xy<SER>(s[k], c[k], L2);                    \
-    KICK1;                                                        \
-    S_ = S_ + drift_xy<SER>(s[k], c[k], L1);                    \
-    s[k].z = s[k].z + __builtin_fma(S_, c[k].B, -Lsb);            \
+    if (nsteps <= 0) return;
+    double S_[PPL];
+#define TP_YBODY(k, KICK1, KICK2)                         \
+  KICK1;                                                  \
+  S_[k] = S_[k] + drift_xy<SER>(s[k], c[k], L2);          \
+  KICK2;                                                  \
+  S_[k] = S_[k] + drift_xy<SER>(s[k], c[k], L2);          \
+  KICK1;
+#define TP_YSTEPS(KICK1, KICK2)                                                    \
+  {                                                                                \
+    _Pragma("unroll") for (int k = 0; k < PPL; ++k) S_[k] = drift_xy<SER>(s[k], c[k], L1); \
+    for (int st = 1; st < nsteps; ++st) {                                          \
+      _Pragma("unroll") for (int k = 0; k < PPL; ++k) {                            \
+        TP_YBODY(k, KICK1, KICK2)                                                  \
+        const double N_ = drift_xy_pair<SER>(s[k], c[k], L1);                     \
+        s[k].z = s[k].z + __builtin_fma(S_[k] + N_, c[k].B, -Lsb);                 \
+        S_[k] = N_;                                                                \
+      }                                                                            \
+    }                                                                              \
+    _Pragma("unroll") for (int k = 0; k < PPL; ++k) {                              \
+      TP_YBODY(k, KICK1, KICK2)                                                    \
+      S_[k] = S_[k] + drift_xy<SER>(s[k], c[k], L1);                               \
+      s[k].z = s[k].z + __builtin_fma(S_[k], c[k].B, -Lsb);                        \
+    }                                                                              \
   }
     if constexpr (MM > 0) {
       double cf[2 * MM + 2];
 #pragma unroll
       for (int i = 0; i < 2 * MM + 2; ++i) cf[i] = poly[i];
-      for (int st = 0; st < nsteps; ++st) {
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) TP_YSTEP(k, kick_cf<MM>(s[k], cf, K1), kick_cf<MM>(s[k], cf, K2))
-      }
+      TP_YSTEPS(kick_cf<MM>(s[k], cf, K1), kick_cf<MM>(s[k], cf, K2))
     } else if constexpr (KK == TP_KICK_SEXT) {
       const double Kb1 = K1 * poly[4], Kb2 = K2 * poly[4];
-      for (int st = 0; st < nsteps; ++st) {
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) TP_YSTEP(k, kick_sext_kb(s[k], Kb1), kick_sext_kb(s[k], Kb2))
-      }
+      TP_YSTEPS(kick_sext_kb(s[k], Kb1), kick_sext_kb(s[k], Kb2))
     } else {
-      for (int st = 0; st < nsteps; ++st) {
-#pragma unroll
-        for (int k = 0; k < PPL; ++k) TP_YSTEP(k, kick<KK>(s[k], poly, m, K1), kick<KK>(s[k], poly, m, K2))
-      }
+      TP_YSTEPS(kick<KK>(s[k], poly, m, K1), kick<KK>(s[k], poly, m, K2))
     }
-#undef TP_YSTEP
+#undef TP_YSTEPS
+#undef TP_YBODY
     return;
   }
 #endif
@@ -199,12 +208,15 @@ TP_DEV void esbend_body(St* s, const Cx* c, const double* __restrict__ p, int ns
     // exact_bend_body (src/tracking.jl:858-894) is a rotation of (px, pzmx), pzmx = pz -
     // (1 + x irho), by the sub-step's angle: px' = px cs + pzmx sn, and from the body's own
     // x' = (pzn - pzmx cs + px sn - 1)/irho, pzmx' = pzn - (1 + x' irho) = pzmx cs - px sn.
-    // Every sub-step is applied as that rotation (4 instructions instead of a square root,
-    // a series arcsine and the O(1) cancellation of x'); x is formed once, at the end, from
-    // pz - 1 = (dp1^2 - 1 - px^2 - py^2) / (pz + 1) without cancellation, and the path terms
-    // asin(px_k/d2) - asin(px_k+1/d2) of the sub-bodies telescope to one difference.
+    // With identity kicks between them the 4 nsteps sub-body rotations compose to one, by
+    // the body's whole angle (rot_body, formed once per element: 4 instructions per lane
+    // instead of a square root, a series arcsine and the O(1) cancellation of x' per
+    // sub-body); x is formed once, at the end, from pz - 1 = (dp1^2 - 1 - px^2 - py^2) /
+    // (pz + 1) without cancellation, and the path terms asin(px_k/d2) - asin(px_k+1/d2) of
+    // the sub-bodies telescope to one difference.
     const double Ltot = (double)nsteps * (2.0 * L1 + 2.0 * L2);
     const double Lbtot = (double)nsteps * (2.0 * L1b + 2.0 * L2b);
+    const Rot R = rot_body(c1, s1, c2, s2, nsteps);
 #pragma unroll
     for (int k = 0; k < PPL; ++k) {
       const double Bm1 = c[k].B - 1.0;                 // exact: dp1 - 1 of the reference's dp1
@@ -228,15 +240,10 @@ TP_DEV void esbend_body(St* s, const Cx* c, const double* __restrict__ p, int ns
         }
         continue;
       }
-      for (int st = 0; st < nsteps; ++st) {
-#define TP_ROT(CS, SN)                                   \
-  {                                                      \
-    const double a_ = __builtin_fma(a, CS, b * SN);      \
-    b = __builtin_fma(b, CS, -(a * SN));                 \
-    a = a_;                                              \
-  }
-        TP_ROT(c1, s1) TP_ROT(c2, s2) TP_ROT(c2, s2) TP_ROT(c1, s1)
-#undef TP_ROT
+      {
+        const double a_ = __builtin_fma(a, R.c, b * R.s);
+        b = __builtin_fma(b, R.c, -(a * R.s));
+        a = a_;
       }
       const double pzn = pxyz(c[k], a, s[k].py);
       const double pn1 = __builtin_fma(-a, a, dd) * rcp_nr(pzn + 1.0);
