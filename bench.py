@@ -87,9 +87,10 @@ MEASURED_BYTES_PER_PARTICLE_LAUNCH = {("dba", 0): (25578.2 * _FETCH_SCALE + 4749
 # FP64 flops the fast DBA kernel executes per particle-turn, from rocprofv3 PMC on the same
 # launch (2 SQ_INSTS_VALU_FMA_F64 + SQ_INSTS_VALU_MUL_F64 + SQ_INSTS_VALU_ADD_F64 per 64-particle
 # tile-turn): fewer than the reference's 313,587 because the fast build does not repeat work
-# (pure-dipole bend sub-bodies as rotations, drifts as a two-FMA series, the edges' fringe
-# term reused across the ring's identical bends).
-MEASURED_EXECUTED_FLOPS_PER_PARTICLE_TURN = {("dba", "fast"): 167_144}
+# (a pure-dipole bend body as one rotation, drifts as a two-FMA series, the two half-drifts
+# where Yoshida steps meet as one drift, the edges' fringe term reused across the ring's
+# identical bends).
+MEASURED_EXECUTED_FLOPS_PER_PARTICLE_TURN = {("dba", "fast"): 142_178}
 
 
 def visible_cores() -> int:

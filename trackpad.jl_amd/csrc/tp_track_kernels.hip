@@ -255,6 +255,49 @@ TP_DEV void esbend_body(St* s, const Cx* c, const double* __restrict__ p, int ns
       pz[k] = pzn;
     }
     return;
+  } else {
+    // Consecutive steps meet at two sub-bodies of length L1 with no kick between them.
+    // Forest's body is the exact flow of its length, so the pair is one body of length 2 L1
+    // (sub-step angle doubled: cos = 1 - 2 s1^2, sin = 2 c1 s1) -- one square root and one
+    // arcsine difference instead of two.  The cosine is formed from s1 alone: c1^2 - s1^2
+    // carries c1's rounding twice, and the body's x, which cancels O(1) terms, is sensitive
+    // to the pair's norm (that form doubled the x error of a 10-turn DA scan).
+    const double c11 = __builtin_fma(-2.0 * s1, s1, 1.0), s11 = 2.0 * (c1 * s1);
+    const double L11 = L1 + L1, L11b = L1b + L1b;
+#define TP_EBODY_STEP(k, KICK1, KICK2, LE, LEB, CE, SE)                            \
+  KICK1;                                                                            \
+  TP_AFTER_KICK(k);                                                                 \
+  ebody(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k], pz[k]);              \
+  KICK2;                                                                            \
+  TP_AFTER_KICK(k);                                                                 \
+  ebody(s[k], c[k], irho, rho, L2, L2b, c2, s2, d2[k], id2[k], pz[k]);              \
+  KICK1;                                                                            \
+  TP_AFTER_KICK(k);                                                                 \
+  ebody(s[k], c[k], irho, rho, LE, LEB, CE, SE, d2[k], id2[k], pz[k]);
+#define TP_EBODY_STEPS(KICK1, KICK2)                                                \
+  {                                                                                 \
+    _Pragma("unroll") for (int k = 0; k < PPL; ++k)                                 \
+        ebody(s[k], c[k], irho, rho, L1, L1b, c1, s1, d2[k], id2[k], pz[k]);        \
+    for (int st = 1; st < nsteps; ++st) {                                           \
+      _Pragma("unroll") for (int k = 0; k < PPL; ++k) {                             \
+        TP_EBODY_STEP(k, KICK1, KICK2, L11, L11b, c11, s11)                         \
+      }                                                                             \
+    }                                                                               \
+    _Pragma("unroll") for (int k = 0; k < PPL; ++k) {                               \
+      TP_EBODY_STEP(k, KICK1, KICK2, L1, L1b, c1, s1)                               \
+    }                                                                               \
+  }
+    if constexpr (MM > 0) {
+      double cf[2 * MM + 2];
+#pragma unroll
+      for (int i = 0; i < 2 * MM + 2; ++i) cf[i] = poly[i];
+      TP_EBODY_STEPS(kick_cf<MM>(s[k], cf, K1), kick_cf<MM>(s[k], cf, K2))
+    } else {
+      TP_EBODY_STEPS(kick<KK>(s[k], poly, m, K1), kick<KK>(s[k], poly, m, K2))
+    }
+#undef TP_EBODY_STEPS
+#undef TP_EBODY_STEP
+    return;
   }
 #endif
   if constexpr (MM > 0) {
