@@ -73,16 +73,16 @@ HBM_PEAK = 8.0e12
 # 1e6-particle DBA launch of 100 turns, per register budget of the kernel (waves/SIMD; 0 = the
 # compiler's, 4 waves): FETCH_SIZE x 46,875 / 23,456.4 (the counter reads 23,456.4 KiB for a
 # plain 48 MB device copy in the same run) + WRITE_SIZE x 1.00 (exact on the same copy), per
-# 1e6 particles.  Budget 0: 25,578.2 + 47,491.3 KiB (the part above the 46,875 KiB of
+# 1e6 particles.  Budget 0: 25,588.4 + 47,459.3 KiB (the part above the 46,875 KiB of
 # coordinates is the tile queue's atomics), independent of the turns per launch since the
 # state stays in registers in between; budgets 5 and 6 add the part of their register spills
-# (scratch) that leaves the L2 over those 100 turns: 5 waves ~3.5 B per particle-turn, 0.2 % of
-# the kernel's time at most at 8 TB/s; at 6 waves the spills reach the hot paths (3.5 GB per
-# launch), which is why the autotuner does not pick that budget for this ring.
+# (scratch) that leaves the L2 over those 100 turns: 5 waves ~0.9 B per particle-turn, a
+# negligible share of the kernel's time at 8 TB/s; at 6 waves the spills reach the hot paths
+# (6.3 GB per launch), which is why the autotuner does not pick that budget for this ring.
 _FETCH_SCALE = 46875.0 / 23456.4
-MEASURED_BYTES_PER_PARTICLE_LAUNCH = {("dba", 0): (25578.2 * _FETCH_SCALE + 47491.3) * 1024 / 1e6,
-                                      ("dba", 5): (42841.6 * _FETCH_SCALE + 360434.5) * 1024 / 1e6,
-                                      ("dba", 6): (205373.9 * _FETCH_SCALE + 3123003.4) * 1024 / 1e6}
+MEASURED_BYTES_PER_PARTICLE_LAUNCH = {("dba", 0): (25588.4 * _FETCH_SCALE + 47459.3) * 1024 / 1e6,
+                                      ("dba", 5): (38009.6 * _FETCH_SCALE + 112941.9) * 1024 / 1e6,
+                                      ("dba", 6): (49959.4 * _FETCH_SCALE + 6135626.5) * 1024 / 1e6}
 
 # FP64 flops the fast DBA kernel executes per particle-turn, from rocprofv3 PMC on the same
 # launch (2 SQ_INSTS_VALU_FMA_F64 + SQ_INSTS_VALU_MUL_F64 + SQ_INSTS_VALU_ADD_F64 per 64-particle
