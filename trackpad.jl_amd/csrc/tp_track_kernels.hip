@@ -261,7 +261,7 @@ TP_DEV void esbend_body(St* s, const Cx* c, const double* __restrict__ p, int ns
     // (sub-step angle doubled: cos = 1 - 2 s1^2, sin = 2 c1 s1) -- one square root and one
     // arcsine difference instead of two.  The cosine is formed from s1 alone: c1^2 - s1^2
     // carries c1's rounding twice, and the body's x, which cancels O(1) terms, is sensitive
-    // to the pair's norm (that form doubled the x error of a 10-turn DA scan).
+    // to the pair's norm (that form gave 2-3x the x error of a 10-turn DA scan).
     const double c11 = __builtin_fma(-2.0 * s1, s1, 1.0), s11 = 2.0 * (c1 * s1);
     const double L11 = L1 + L1, L11b = L1b + L1b;
 #define TP_EBODY_STEP(k, KICK1, KICK2, LE, LEB, CE, SE)                            \
