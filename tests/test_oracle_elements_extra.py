@@ -192,3 +192,32 @@ def test_tabulated_wake_interpolates():
     r = np.zeros((1, 6))
     r[0, 4] = 1e-3  # ahead of the source: t > times[0] * fliphalf
     assert line([LongitudinalWake(times, wakes, scale=scale)], r)[0, 5] == 0.0
+
+
+def test_crab_cavity_kicks_are_one_potential():
+    """A thin crab cavity is a kick from one potential, V = -(volt/E) x sin((k z + phi) beti) /
+    beti-scaled, so its px kick and its delta kick must be each other's derivative
+    (Panofsky-Wenzel): with the reference's canonical pairs (x, px), (y, py), (delta, z), the
+    map is symplectic, J^T S J = S for the oracle's dual-number Jacobian -- which a wrong sign
+    or a dropped k beti factor in the delta kick breaks, while the thin kick's det J = 1 holds
+    either way.  For L > 0 the reference evaluates the kick's phase from the entry z, before
+    the first half-drift (src/tracking.jl:1535-1542); that map does not even keep volume
+    (det J - 1 ~ 2e-7 here): the oracle keeps the reference's order, and the test records the
+    defect."""
+    S = np.zeros((6, 6))
+    for a, b in ((0, 1), (2, 3), (5, 4)):
+        S[a, b], S[b, a] = 1.0, -1.0
+    beam = Beam(3e9)
+    r0 = np.array([1e-3, 2e-4, -5e-4, 1e-4, 2e-3, 1e-3])
+    for volt, freq, phi in ((2.0e6, 5e8, 0.3), (-1.5e6, 1.3e9, -1.1)):
+        lat = Lattice([Drift(0.5), CrabCavity(0.0, volt=volt, freq=freq, phi=phi, energy=3e9), Quadrupole(0.4, 1.0)])
+        _, J, lost = O.jacobian(lat, beam, r0, 1)
+        assert not lost
+        d = J.T @ S @ J - S
+        assert np.max(np.abs(d)) <= 1e-13 * np.max(np.abs(J)) ** 2, np.max(np.abs(d))
+        # the delta kick is there: J[delta, x] = -k volt/E beti cos(...) is far from zero
+        assert abs(J[5, 0]) > 1e-3
+    lat = Lattice([CrabCavity(0.3, volt=2.0e6, freq=5e8, phi=0.3, energy=3e9)])
+    _, J, lost = O.jacobian(lat, beam, r0, 1)
+    assert not lost
+    assert abs(np.linalg.det(J) - 1.0) > 1e-9 and np.max(np.abs(J.T @ S @ J - S)) > 1e-9
