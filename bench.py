@@ -71,7 +71,7 @@ HBM_PEAK = 8.0e12
 
 # HBM bytes per particle of one tracking-kernel launch, measured with rocprofv3 --pmc on the
 # 1e6-particle DBA launch of 100 turns, per register budget of the kernel (waves/SIMD; 0 = the
-# compiler's, 4 waves): FETCH_SIZE x 46,875 / 23,456.4 (the counter reads 23,456.4 KiB for a
+# compiler's, 3 waves): FETCH_SIZE x 46,875 / 23,456.4 (the counter reads 23,456.4 KiB for a
 # plain 48 MB device copy in the same run) + WRITE_SIZE x 1.00 (exact on the same copy), per
 # 1e6 particles.  Budget 0: 25,588.4 + 47,459.3 KiB (the part above the 46,875 KiB of
 # coordinates is the tile queue's atomics), independent of the turns per launch since the
