@@ -195,9 +195,13 @@ TP_DEV void sub6(St& s, const double* __restrict__ t) {
 // with FT_MISALIGN); lin selects the linearised drift (variant with FT_LINH).
 // DCH: the element may change delta (changes_delta); with DCH = false the code that renews
 // the delta hoists (the RF kick, the other families, misalignments) is not compiled in.
+// slot >= 0: the element's strength-gradient slot; the wave's sum of the lanes' gradients is
+// added to row[slot] inside the magnet's case, so the other element types reach the loop's
+// back-edge straight from their maps.  Every lane adds the sum: one read and one write of the
+// same LDS word by the whole wave, so the row gains it once.
 template <unsigned FEAT, bool SER_OK, bool DCH = true>
 TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, EdgeMemo* mm, const tp_rec& rec, const double* __restrict__ pool,
-                            const double* __restrict__ minv, double beti, bool lin, double& gacc) {
+                            const double* __restrict__ minv, double beti, bool lin, int32_t slot, double* row) {
   // the series drift needs two more hoisted values; kernels with general kicks never take it
   constexpr bool SER = SER_OK && (FEAT & FT_GENKICK) == 0;
   const double* __restrict__ p = pool + rec.off;
@@ -215,12 +219,15 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, EdgeMemo* mm, const tp_rec& rec
       drift_back<SER>(s, a, c, p[0], p[1], lin);
       break;
     case TP_MPOLE: {
+      double gacc = 0.0;
 #define TP_CALL(KK) mpole_back<KK, SER>(s, a, c, p, ns, m, g, gacc, lin)
       TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
+      if (slot >= 0) row[slot] += wave_sum(gacc);
       break;
     }
     case TP_THINMP: {
+      double gacc = 0.0;
 #define TP_CALL(KK) kick_back<KK>(s, a, p, m, 1.0, -1, gacc)
       TP_KICK_SWITCH(FEAT, TP_CALL)
 #undef TP_CALL
@@ -306,9 +313,9 @@ TP_DEV void apply_elem_back(St& s, Ad& a, Cx& c, EdgeMemo* mm, const tp_rec& rec
 template <unsigned FEAT, bool SER_OK>
 TP_DEV void apply_elem_back_dch(St& s, Ad& a, Cx& c, EdgeMemo* mm, const tp_rec& rec,
                                 const double* __restrict__ pool, const double* __restrict__ minv, double beti,
-                                bool lin, double& gacc) {
+                                bool lin, int32_t slot, double* row) {
   if constexpr ((FEAT & FT_MISALIGN) != 0) {
-    apply_elem_back<FEAT, SER_OK, true>(s, a, c, mm, rec, pool, minv, beti, lin, gacc);
+    apply_elem_back<FEAT, SER_OK, true>(s, a, c, mm, rec, pool, minv, beti, lin, slot, row);
   } else {
     constexpr bool SER = SER_OK && (FEAT & FT_GENKICK) == 0;
     const double* __restrict__ p = pool + rec.off;
@@ -355,12 +362,6 @@ TP_DEV void back_turns(St& s, Ad& l, Cx& c, EdgeMemo* mm, const int32_t* __restr
         if (e >= 2) ri_next = seq[e - 2];
       }
     };
-    auto add_grad = [&](int32_t slot, double gacc) {
-      if (slot >= 0) {
-        const double w = wave_sum(gacc);
-        if (lane == 0) row[slot] += w;
-      }
-    };
     while (e >= 0) {
       // elements that leave delta unchanged: the delta hoists are invariant in this loop
       // (changes_delta); an element that renews them ends it and runs below
@@ -369,17 +370,13 @@ TP_DEV void back_turns(St& s, Ad& l, Cx& c, EdgeMemo* mm, const int32_t* __restr
         const int32_t slot = slot_next;
         if (changes_delta<FEAT>(rec)) break;
         advance();
-        double gacc = 0.0;
-        apply_elem_back<FEAT, SER_OK, false>(s, l, c, mm, rec, pool, minv, beti, lin, gacc);
-        add_grad(slot, gacc);
+        apply_elem_back<FEAT, SER_OK, false>(s, l, c, mm, rec, pool, minv, beti, lin, slot, row);
       }
       if (e < 0) break;
       const tp_rec rec = rec_next;
       const int32_t slot = slot_next;
       advance();
-      double gacc = 0.0;
-      apply_elem_back_dch<FEAT, SER_OK>(s, l, c, mm, rec, pool, minv, beti, lin, gacc);
-      add_grad(slot, gacc);
+      apply_elem_back_dch<FEAT, SER_OK>(s, l, c, mm, rec, pool, minv, beti, lin, slot, row);
       --e;
     }
   }
