@@ -345,7 +345,7 @@ template <unsigned FEAT, bool SER_OK>
 TP_DEV void back_turns(St& s, Ad& l, Cx& c, EdgeMemo* mm, const int32_t* __restrict__ seq_rows, const tp_rec* __restrict__ recs,
                        const double* __restrict__ pool, const double* __restrict__ minv,
                        const int32_t* __restrict__ slot_of_pos, int32_t n_seq, int32_t nturns, int32_t turn_last,
-                       int32_t n_rows, int32_t row_turn0, double beti, bool lin, double* row, int lane) {
+                       int32_t n_rows, int32_t row_turn0, double beti, bool lin, double* row) {
   for (int32_t t = 0; t < nturns; ++t) {
     const int32_t* __restrict__ seq =
         seq_rows + (n_rows > 1 ? (int64_t)(turn_last - t - row_turn0) * n_seq : 0);
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void adjoint_kernel(const int32_
   mkctx(s, c, a.beti);
   const bool lin = (FEAT & FT_LINH) != 0 && a.exact_h == 0;
   back_turns<FEAT, false>(s, l, c, &mm, seq, recs, pool, a.minv, slot_of_pos, a.n_seq, a.nturns, a.turn_last, a.n_rows,
-                   a.row_turn0, a.beti, lin, lds + wave * a.n_slots, lane);
+                   a.row_turn0, a.beti, lin, lds + wave * a.n_slots);
   if (i < a.n) {
     double* r = lam + i;
     r[0] = l.x;
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(256) TP_WAVES(WPE) void batch_back_kernel(const TpV
       const int32_t nt = (j == d.nseg - 1) ? d.nturns - j * d.ckpt_stride : d.ckpt_stride;
       back_turns<FEAT, true>(s, l, c, &mm, seq + d.seq_off, recs + d.rec_off, pool + d.pool_off,
                        minv ? minv + d.pool_off : nullptr, slots + d.pos_off, d.n_seq, nt,
-                       d.turn0 + j * d.ckpt_stride + nt - 1, d.n_rows, d.row_turn0, d.beti, lin, row, lane);
+                       d.turn0 + j * d.ckpt_stride + nt - 1, d.n_rows, d.row_turn0, d.beti, lin, row);
     }
     if (li < d.n) {
       double* r = d.lam + li;
